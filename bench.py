@@ -1,0 +1,175 @@
+"""Benchmark: Munknowns/s to relative residual 1e-10 on one MI355X.
+
+Workload (default): the unit-square Poisson problem on 8193 x 8193 nodes with
+64 seeded Gaussian bumps as right-hand side and g = sin(3x) cosh(3y) on the
+boundary.  One step = one complete solve from the zero interior guess until
+||r||_2 <= 1e-10 ||r_0||_2, with the problem already resident in HBM (the
+iterate is restored on the device between steps, outside the timed region).
+
+Reported: the mixed-precision solver (FP32 V(2,2) cycles inside FP64
+iterative refinement) as ``value``; the plain FP64 V(2,2) multigrid on the same
+problem beside it (``fp64``); the dominant kernel's roofline from hipEvent
+timings of a profiled solve; and the oracle's CPU multigrid on a bounded
+sample of the same workload.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--seg 8192]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def solve_times(scheme, steps, warmup, rtol):
+    for _ in range(warmup):
+        scheme.restore_iterate()
+        scheme.solve(rtol, 100)
+    wall, dev, cycles, rel = [], [], [], []
+    for _ in range(steps):
+        scheme.restore_iterate()  # device copy, before the clock starts
+        t0 = time.perf_counter()
+        st = scheme.solve(rtol, 100)  # returns after the final stream sync
+        wall.append(time.perf_counter() - t0)
+        dev.append(st["seconds"])
+        cycles.append(st["cycles"])
+        rel.append(st["rel_residual"])
+    return wall, dev, cycles, rel
+
+
+def profile_solve(p, mixed, rtol):
+    """One solve with every launch bracketed by hipEvents; returns per
+    (kernel, level) launches / ms / algorithmic bytes."""
+    from vlp2d_amd import MultiGridSchemeHIP, Options
+    from vlp2d_amd._lib import FLAG_PROFILE
+    s = MultiGridSchemeHIP(p.seg_x, p.seg_y, p.hx, p.hy, p.f, options=Options.performance(mixed, flags=FLAG_PROFILE))
+    s.un0[:] = p.initial("zero")
+    s.init_after_boundaries_and_initial_iteration_inited()
+    s.save_iterate()
+    s.solve(rtol, 100)  # warm
+    s.restore_iterate()
+    s.reset_kernel_times()
+    s.solve(rtol, 100)
+    kt = s.kernel_times()
+    s.cleanup()
+    return kt
+
+
+def cpu_baseline(seg, rtol):
+    """Oracle (C++ restatement of the reference CPU multigrid, OpenMP over
+    rows) on the same formulas at a reduced size: FP64 V(2,2) to rtol."""
+    from oracle import Oracle
+    from vlp2d_amd import problems
+    p = problems.config(3, seg)
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    cores = max(1, min(cores, 16))
+    o = Oracle(p.seg_x, p.seg_y, p.hx, p.hy, p.f, "double", nthreads=cores)
+    o.set_u(p.initial("zero"))
+    t0 = time.perf_counter()
+    k, hist = o.plain_solve(2, 2, 3, rtol, 100)
+    dt = time.perf_counter() - t0
+    o.close()
+    return {"value": p.unknowns / dt / 1e6, "unit": "Munknowns/s", "cores": cores, "kind": "port",
+            "sample": f"oracle FP64 V(2,2) RB-GS multigrid, {seg + 1}x{seg + 1} replica of the workload, "
+                      f"{k} cycles to rel. residual {hist[-1]:.2e} in {dt:.2f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--seg", type=int, default=8192)
+    ap.add_argument("--rtol", type=float, default=1e-10)
+    ap.add_argument("--cpu-seg", type=int, default=2048)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-profile", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              "(the multigrid path runs on one MI355X, see DESIGN.md for how it would shard)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    from vlp2d_amd import MultiGridSchemeHIP, Options, problems
+
+    t0 = time.perf_counter()
+    p = problems.config(3, args.seg)
+    log(f"problem {p.name}: {p.shape} nodes, {p.unknowns} unknowns ({time.perf_counter() - t0:.1f}s to generate)")
+
+    res = {}
+    for mixed in (True, False):
+        s = MultiGridSchemeHIP(p.seg_x, p.seg_y, p.hx, p.hy, p.f, options=Options.performance(mixed))
+        s.un0[:] = p.initial("zero")
+        s.init_after_boundaries_and_initial_iteration_inited()
+        s.save_iterate()
+        wall, dev, cyc, rel = solve_times(s, args.steps, args.warmup, args.rtol)
+        res[mixed] = {"wall": wall, "dev": dev, "cycles": cyc, "rel": rel, "info": s.info()}
+        log(f"{'mixed' if mixed else 'fp64 '}: cycles {cyc}, wall ms {[round(w * 1e3, 3) for w in wall]}, "
+            f"rel {max(rel):.2e}")
+        s.cleanup()
+
+    mixed_ms = 1e3 * float(np.mean(res[True]["wall"]))
+    fp64_ms = 1e3 * float(np.mean(res[False]["wall"]))
+    value = p.unknowns / (mixed_ms * 1e-3) / 1e6
+
+    roofline = None
+    kernels = None
+    if not args.no_profile:
+        kt = profile_solve(p, True, args.rtol)
+        tot = sum(e[3] for e in kt)
+        dom = max(kt, key=lambda e: e[3])
+        name, lvl, launches, ms, nbytes = dom
+        per_launch_bytes = nbytes / launches
+        avg_s = ms * 1e-3 / launches
+        achieved = per_launch_bytes / avg_s / 1e9
+        roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                    "kernel": f"{name}@level{lvl}", "avg_launch_us": round(avg_s * 1e6, 3),
+                    "alg_bytes_per_launch": per_launch_bytes, "share_of_solve": round(ms / tot, 3)}
+        kernels = sorted(([e[0], e[1], e[2], round(e[3], 4), e[4]] for e in kt), key=lambda e: -e[3])[:12]
+        alg_total = sum(e[4] for e in kt)
+        roofline["solve_alg_bytes"] = alg_total
+        roofline["solve_effective_frac"] = round(alg_total / (mixed_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+
+    cpu = None if args.no_cpu else cpu_baseline(args.cpu_seg, args.rtol)
+
+    line = {
+        "metric": "Munknowns/s to rel. residual 1e-10: FP32+FP64-refine vs FP64 MG; % HBM roofline",
+        "value": round(value, 2),
+        "unit": "Munknowns/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(mixed_ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 V-cycles + f64 refinement",
+        "data": "synthetic (seeded Gaussian bumps, SplitMix64)",
+        "config": {"workload": f"poisson_unit_square_{args.seg + 1}x{args.seg + 1}_gaussian_bumps64_"
+                               f"mixed_fp32_vcycles_fp64_refinement_to_rel_residual_{args.rtol:g}",
+                   "nodes": [p.seg_x + 1, p.seg_y + 1], "unknowns": p.unknowns, "cycle": "V(2,2) RB-GS, HW restriction",
+                   "parallelism": "single GPU"},
+        "fp64": {"value": round(p.unknowns / (fp64_ms * 1e-3) / 1e6, 2), "ms_per_step": round(fp64_ms, 4),
+                 "cycles": res[False]["cycles"][0]},
+        "mixed_cycles": res[True]["cycles"][0],
+        "speedup_mixed_vs_fp64": round(fp64_ms / mixed_ms, 3),
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "kernels_top": kernels,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

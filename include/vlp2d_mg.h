@@ -1,0 +1,181 @@
+/*
+ * vlp2d_mg.h -- C ABI of the gfx950 multigrid Dirichlet solver for the
+ * 5-point Poisson/Laplace problem on a rectangle (u_xx + u_yy = -f).
+ *
+ * Drop-in boundary: this ABI replaces, for one solver, the reference's
+ * solver-method interface IScheme<T> = ISchemeInit<T> + ISchemeCalculate<T>
+ * (VisualLaplacePoisson2D/Model/CPU/IScheme.cs:16-45) as implemented by
+ * MultiGridSchemeCU<T> (VisualLaplacePoisson2D/Model/CUDA/MultiGridSchemeCU.cs)
+ * on top of Iterative1DScheme<T> (VisualLaplacePoisson2D/Model/CUDA/
+ * Iterative1DScheme.cs:13-70).  A C# P/Invoke class binds these entry points.
+ *
+ * Conventions
+ *   - cdecl, blittable arguments only, no exceptions cross the boundary.
+ *   - Every int-returning call returns VLP2D_OK (0) or a negative VLP2D_E_*
+ *     code; vlp2d_mg_last_error() then describes the failure.
+ *   - Grids: segX x segY segments, nx = segX+1 by ny = segY+1 nodes, host
+ *     arrays row-major with index i*ny + j (j contiguous), i along x -- the
+ *     layout of the reference GPU schemes (MultiGridSchemeCU.cs:41-45).
+ *   - segX and segY must be powers of two >= 4 and <= 32768 (the reference
+ *     rounds segments up to powers of two before building the scheme,
+ *     VisualLaplacePoisson2D/Model/CPU/StepsRecalculator.cs:168-172).
+ *   - Calls on one handle are serialised by the caller (one worker thread,
+ *     VisualLaplacePoisson2D/Model/CPU/VLPRectangleModel.cs:494-502), except
+ *     vlp2d_mg_cancel(), which may come from any thread.
+ */
+#ifndef VLP2D_MG_H
+#define VLP2D_MG_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VLP2D_OK 0
+#define VLP2D_E_ARG (-1)       /* null pointer, bad option value            */
+#define VLP2D_E_SIZE (-2)      /* segments not a power of two in [4, 32768] */
+#define VLP2D_E_DEVICE (-3)    /* no such device, or a HIP runtime failure  */
+#define VLP2D_E_OOM (-4)       /* device allocation failed                  */
+#define VLP2D_E_NONFINITE (-5) /* NaN/Inf in inputs or in the iterate       */
+#define VLP2D_E_NOTCONV (-6)   /* max_cycles reached before rtol            */
+#define VLP2D_E_CANCELLED (-7) /* vlp2d_mg_cancel() was called              */
+#define VLP2D_E_STATE (-8)     /* call out of order                          */
+
+/* Precision of the V-cycles. */
+#define VLP2D_PREC_FP64 0  /* plain FP64 multigrid                            */
+#define VLP2D_PREC_MIXED 1 /* FP32 V-cycles inside FP64 iterative refinement  */
+
+/* Option flags (vlp2d_mg_opts.flags). */
+#define VLP2D_FLAG_NO_GRAPH 1u   /* launch kernels directly, no hipGraph replay   */
+#define VLP2D_FLAG_PROFILE 2u    /* time every kernel class with hipEvents        */
+#define VLP2D_FLAG_SIMPLE 4u     /* one operator per launch (no fused kernels)    */
+
+typedef struct vlp2d_mg_ctx *vlp2d_mg_handle;
+
+typedef struct {
+  int nu1;         /* pre-smoothing RB-GS sweeps            (reference: 3) */
+  int nu2;         /* post-smoothing RB-GS sweeps           (reference: 3) */
+  int nu_coarse;   /* sweeps on the coarsest level          (reference: 3) */
+  int precision;   /* VLP2D_PREC_FP64 or VLP2D_PREC_MIXED                  */
+  unsigned flags;  /* VLP2D_FLAG_*                                          */
+  int coarse_nodes;/* levels with nx*ny <= this run inside one workgroup;
+                      0 = library default, -1 = never                       */
+} vlp2d_mg_opts;
+
+typedef struct {
+  int cycles;             /* V-cycles (FP64) or refinement steps (mixed) run */
+  int levels;             /* multigrid levels                                 */
+  double r0;              /* ||fn - Lap_h u0||_2 over interior nodes (FP64)   */
+  double rel_residual;    /* final ||r||_2 / r0                               */
+  double seconds;         /* device time of the solve, hipEvents              */
+  double munknowns_per_s; /* interior unknowns / seconds / 1e6                */
+} vlp2d_mg_stats;
+
+typedef struct {
+  int levels;
+  int nx, ny;                    /* level-0 nodes                           */
+  long long device_bytes;        /* device memory held by the handle now    */
+  long long device_peak_bytes;   /* high-water mark of device_bytes         */
+  int checkpoints;               /* forward cycles recorded for the adjoint */
+} vlp2d_mg_info;
+
+/* Reference defaults: V(3,3), 3 coarsest sweeps, FP64, graphs on. */
+void vlp2d_mg_default_opts(vlp2d_mg_opts *opts);
+
+/* Constructor.  Replaces MultiGridSchemeCU<T>(segX, segY, hx, hy, fKsi, eps,
+ * device) (MultiGridSchemeCU.cs:32-134).  f: PDE right-hand side sampled at
+ * all nx*ny nodes (boundary entries ignored), or NULL for the Laplace
+ * equation; the library stores fn = -f as the reference does
+ * (MultiGridSchemeCU.cs:41-45).  eps is kept for the reference stop rule and
+ * reported back by vlp2d_mg_cycle's caller contract.  opts may be NULL. */
+int vlp2d_mg_create(int segX, int segY, double hx, double hy, const double *f, double eps, int device,
+                    const vlp2d_mg_opts *opts, vlp2d_mg_handle *out);
+
+/* Uploads the level-0 iterate: Dirichlet boundary values plus the initial
+ * guess, nx*ny doubles.  Replaces initAfterBoundariesAndInitialIterationInited
+ * (MultiGridSchemeCU.cs:263-266). */
+int vlp2d_mg_upload(vlp2d_mg_handle h, const double *u);
+
+/* One multigrid step.  Replaces doIteration (MultiGridSchemeCU.cs:140-146).
+ * FP64: one V(nu1,nu2) cycle; *stop_metric = max over interior nodes of
+ * |u - u_before_post_smoothing| on level 0, the quantity the reference
+ * compares with eps (MultiGridScheme.cs:69,111).  Mixed: one FP32 V-cycle on
+ * the FP64 residual followed by the FP64 update; *stop_metric = max |update|.
+ * stop_metric may be NULL. */
+int vlp2d_mg_cycle(vlp2d_mg_handle h, double *stop_metric);
+
+/* Downloads the level-0 iterate (nx*ny doubles).  Replaces getArray
+ * (MultiGridSchemeCU.cs:252-261). */
+int vlp2d_mg_download(vlp2d_mg_handle h, double *u);
+
+/* ||fn - Lap_h u||_2 over interior nodes, FP64 (no counterpart in the
+ * reference; the relative-residual stop of vlp2d_mg_solve uses it). */
+int vlp2d_mg_residual_norm(vlp2d_mg_handle h, double *norm);
+
+/* Runs cycles from the current iterate until ||r||_2 <= rtol * ||r_0||_2 or
+ * max_cycles.  Replaces the caller's loop `for iter < maxIters:
+ * doIteration` (VLPRectangleModel.cs:579-614) with a device-side loop.
+ * Returns VLP2D_E_NOTCONV (stats filled) if rtol was not reached. */
+int vlp2d_mg_solve(vlp2d_mg_handle h, double rtol, int max_cycles, vlp2d_mg_stats *stats);
+
+/* Gradient of a loss L(u) through the exact discrete solve (no counterpart in
+ * the reference).  dL_du: nx*ny (interior entries used).  Solves the adjoint
+ * system A lambda = dL_du with the same multigrid (A = -Lap_h is symmetric)
+ * to rtol, then dL_df = lambda (interior; 0 on the boundary) and dL_dg =
+ * B^T lambda (boundary nodes; 0 inside and at the four corners), where g are
+ * the Dirichlet values.  The handle's iterate is left unchanged. */
+int vlp2d_mg_adjoint_exact(vlp2d_mg_handle h, const double *dL_du, double rtol, int max_cycles, double *dL_df,
+                           double *dL_dg);
+
+/* Runs K FP64 V(nu1,nu2) cycles from the current iterate in Poisson form,
+ * storing one device checkpoint of u per cycle, so that
+ * vlp2d_mg_adjoint_cycles can differentiate u_K = Phi^K(u_0; f, g). */
+int vlp2d_mg_forward_cycles(vlp2d_mg_handle h, int K);
+
+/* Transposed sweep through the K cycles recorded by vlp2d_mg_forward_cycles.
+ * dL_du: adjoint of u_K (nx*ny).  Outputs (nx*ny each, any may be NULL):
+ * dL_df (interior), dL_dg (boundary nodes), dL_du0 (interior initial guess). */
+int vlp2d_mg_adjoint_cycles(vlp2d_mg_handle h, const double *dL_du, double *dL_df, double *dL_dg,
+                            double *dL_du0);
+
+/* Sizes and device-memory accounting of the handle. */
+int vlp2d_mg_get_info(vlp2d_mg_handle h, vlp2d_mg_info *info);
+
+/* Per-kernel-class accounting, one entry per (kernel class, level) that was
+ * launched: launches, total milliseconds (hipEvents; collected only under
+ * VLP2D_FLAG_PROFILE, else 0) and algorithmic bytes (the compulsory HBM
+ * traffic of the operators launched).  Returns the number of entries; at
+ * most cap are written.  names receives pointers to static strings.  No
+ * counterpart in the reference (its CUDA event helper, UtilsCU.cs:215-223,
+ * is unused by MultiGrid). */
+int vlp2d_mg_kernel_times(vlp2d_mg_handle h, const char **names, int *levels, long long *launches, double *total_ms,
+                          double *alg_bytes, int cap);
+int vlp2d_mg_reset_kernel_times(vlp2d_mg_handle h);
+
+/* Device-side copy of the level-0 iterate and its restoration (no PCIe
+ * transfer), so repeated solves start from the same state.  Restoring also
+ * clears a pending cancel request and the recorded forward cycles.  No
+ * counterpart in the reference, whose model re-runs the whole setup. */
+int vlp2d_mg_save_iterate(vlp2d_mg_handle h);
+int vlp2d_mg_restore_iterate(vlp2d_mg_handle h);
+
+/* Requests cancellation; safe from any thread.  The next cycle boundary of a
+ * running vlp2d_mg_solve returns VLP2D_E_CANCELLED.  Replaces
+ * cancelIterations (MultiGridSchemeCU.cs:270, a no-op in the reference). */
+void vlp2d_mg_cancel(vlp2d_mg_handle h);
+
+/* Frees all device memory and the handle.  Replaces cleanup
+ * (MultiGridSchemeCU.cs:272-285).  NULL is accepted. */
+void vlp2d_mg_destroy(vlp2d_mg_handle h);
+
+/* Message of the last failure on h; with h == NULL, of the last failed
+ * vlp2d_mg_create on the calling thread.  Never NULL. */
+const char *vlp2d_mg_last_error(vlp2d_mg_handle h);
+
+/* ABI version, (major << 16) | minor. */
+int vlp2d_mg_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VLP2D_MG_H */

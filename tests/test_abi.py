@@ -1,0 +1,74 @@
+"""The C-ABI library loads and exports every entry point include/vlp2d_mg.h
+declares; argument validation that happens before any device call."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from vlp2d_amd import _lib
+from vlp2d_amd._lib import load_library
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    txt = open(os.path.join(ROOT, "include", "vlp2d_mg.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(vlp2d_mg_\w+)\s*\(", txt)))
+
+
+def test_header_matches_binding_table():
+    assert _declared() == sorted(_lib.EXPORTS)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = load_library()
+    for name in _declared():
+        assert hasattr(lib, name), name
+
+
+def test_default_opts_and_version():
+    lib = load_library()
+    o = _lib.Opts()
+    lib.vlp2d_mg_default_opts(ctypes.byref(o))
+    assert (o.nu1, o.nu2, o.nu_coarse, o.precision, o.flags, o.coarse_nodes) == (3, 3, 3, 0, 0, 0)
+    assert lib.vlp2d_mg_version() >> 16 == 1
+
+
+@pytest.mark.parametrize("sx,sy", [(6, 8), (8, 2), (0, 8), (65536, 4), (-4, 4)])
+def test_create_rejects_bad_sizes_before_touching_a_device(sx, sy):
+    lib = load_library()
+    h = ctypes.c_void_p()
+    rc = lib.vlp2d_mg_create(sx, sy, 0.1, 0.1, None, 1e-13, 0, None, ctypes.byref(h))
+    assert rc == _lib.E_SIZE and not h.value
+    assert b"power" in lib.vlp2d_mg_last_error(None)
+
+
+def test_create_rejects_bad_steps_and_options():
+    lib = load_library()
+    h = ctypes.c_void_p()
+    assert lib.vlp2d_mg_create(8, 8, -1.0, 0.1, None, 1e-13, 0, None, ctypes.byref(h)) == _lib.E_ARG
+    o = _lib.Opts(2, 2, 0, 0, 0, 0)
+    assert lib.vlp2d_mg_create(8, 8, 0.1, 0.1, None, 1e-13, 0, ctypes.byref(o), ctypes.byref(h)) == _lib.E_ARG
+    o = _lib.Opts(2, 2, 1, 7, 0, 0)
+    assert lib.vlp2d_mg_create(8, 8, 0.1, 0.1, None, 1e-13, 0, ctypes.byref(o), ctypes.byref(h)) == _lib.E_ARG
+    assert lib.vlp2d_mg_create(8, 8, 0.1, 0.1, None, 1e-13, 0, None, None) == _lib.E_ARG
+
+
+def test_create_rejects_nonfinite_rhs():
+    import numpy as np
+    lib = load_library()
+    f = np.zeros((9, 9))
+    f[3, 4] = np.nan
+    h = ctypes.c_void_p()
+    rc = lib.vlp2d_mg_create(8, 8, 0.1, 0.1, _lib.ptr(f), 1e-13, 0, None, ctypes.byref(h))
+    assert rc == _lib.E_NONFINITE
+
+
+def test_null_handle_calls_are_rejected():
+    lib = load_library()
+    assert lib.vlp2d_mg_cycle(None, None) == _lib.E_ARG
+    assert lib.vlp2d_mg_solve(None, 1e-10, 10, None) == _lib.E_ARG
+    lib.vlp2d_mg_destroy(None)
+    lib.vlp2d_mg_cancel(None)

@@ -1,0 +1,214 @@
+"""GPU parity: the HIP multigrid (through the C ABI) against the oracle on the
+same seeded inputs.  The library evaluates the reference point formulas in the
+same order without FMA contraction, so iterates are expected to agree bit for
+bit; the bar written in each assert is the one the comparison must meet."""
+import numpy as np
+import pytest
+
+from oracle import Oracle, adjoint_cycles, dst_adjoint, dst_solve, mixed_solve, plain_solve
+from vlp2d_amd import MultiGridSchemeHIP, Options, VLP2DError, problems
+from vlp2d_amd import _lib
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def _random(seg_x, seg_y, seed, laplace=False):
+    rng = np.random.default_rng(seed)
+    f = None if laplace else rng.standard_normal((seg_x + 1, seg_y + 1))
+    g = np.zeros((seg_x + 1, seg_y + 1))
+    g[0, :], g[-1, :] = rng.standard_normal(seg_y + 1), rng.standard_normal(seg_y + 1)
+    g[:, 0], g[:, -1] = rng.standard_normal(seg_x + 1), rng.standard_normal(seg_x + 1)
+    return f, g
+
+
+def _mean_init(g):
+    u = g.copy()
+    b = np.concatenate([g[0, :], g[-1, :], g[:, 0], g[:, -1]])
+    u[1:-1, 1:-1] = (b.min() + b.max()) / 2
+    return u
+
+
+PARITY_CASES = [
+    # seg_x, seg_y, hx, hy, laplace
+    (4, 4, 0.25, 0.25, False),
+    (8, 16, 1 / 8, 1 / 16, False),      # unequal steps
+    (16, 16, 1 / 16, 1 / 16, True),     # Laplace kernels
+    (64, 32, 1 / 64, 1 / 64, False),
+    (128, 256, np.e / 128, np.pi / 256, False),  # Poisson1 shape, unequal steps
+    (256, 256, 1 / 256, 1 / 256, True),
+    (512, 128, 1 / 512, 1 / 512, False),
+]
+
+
+@pytest.mark.parametrize("seg_x,seg_y,hx,hy,laplace", PARITY_CASES)
+def test_parity_mode_cycles_bitwise(seg_x, seg_y, hx, hy, laplace):
+    """Reference parity mode: V(3,3), half weighting, red-first RB-GS, 3
+    coarsest sweeps, Mean initial guess; every iterate and the update norm
+    equal the oracle's after each of 6 cycles."""
+    _say(f"parity {seg_x}x{seg_y} laplace={laplace}")
+    f, g = _random(seg_x, seg_y, seg_x * 7 + seg_y, laplace)
+    u0 = _mean_init(g)
+    o = Oracle(seg_x, seg_y, hx, hy, f)
+    o.set_u(u0)
+    s = MultiGridSchemeHIP(seg_x, seg_y, hx, hy, f, eps=1e-13)
+    s.un0[:] = u0
+    s.init_after_boundaries_and_initial_iteration_inited()
+    for k in range(6):
+        m_ref = o.vcycle(3, 3, 3)
+        s.do_iteration(k)
+        ug, ur = s.get_array(), o.get_u()
+        assert np.array_equal(ug, ur), (k, np.abs(ug - ur).max())
+        assert s.last_metric == m_ref
+    s.cleanup()
+
+
+def test_config1_reference_stop_rule_cycle_count():
+    """Config 1 (257^2 Laplace, g = x^2 - y^2) through the reference loop:
+    cycles until the update norm <= eps, same count as the oracle, and the
+    discrete solution equals the harmonic quadratic."""
+    p = problems.config(1)
+    _say("config1 257^2 reference loop")
+    o = Oracle(p.seg_x, p.seg_y, p.hx, p.hy, None)
+    o.set_u(p.initial("mean"))
+    s = MultiGridSchemeHIP(p.seg_x, p.seg_y, p.hx, p.hy, None, eps=1e-13)
+    s.un0[:] = p.initial("mean")
+    s.init_after_boundaries_and_initial_iteration_inited()
+    n_ref = n_gpu = 0
+    while n_ref < 40 and o.vcycle(3, 3, 3) > 1e-13:
+        n_ref += 1
+    while n_gpu < 40 and s.do_iteration(n_gpu) >= 1e-13:
+        n_gpu += 1
+    assert n_gpu == n_ref
+    assert np.array_equal(s.get_array(), o.get_u())
+    assert np.abs(s.get_array() - p.exact).max() < 1e-12
+    s.cleanup()
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+@pytest.mark.parametrize("cfg,seg", [(2, 512), (3, 256), (1, 128)])
+def test_performance_mode_matches_oracle(cfg, seg, mixed):
+    """V(2,2) to relative residual 1e-10 from a zero interior, FP64 or FP32
+    cycles in FP64 refinement: same number of cycles as the oracle and the
+    same iterate (bitwise), which is the DST-I solution to solver accuracy."""
+    p = problems.config(cfg, seg)
+    _say(f"perf cfg{cfg} seg{seg} mixed={mixed}")
+    u0 = p.initial("zero")
+    if mixed:
+        ur, kr, hr = mixed_solve(p.seg_x, p.seg_y, p.hx, p.hy, p.f, u0, 2, 2, 3, 1e-10, 60)
+    else:
+        ur, kr, hr = plain_solve(p.seg_x, p.seg_y, p.hx, p.hy, p.f, u0, 2, 2, 3, 1e-10, 60)
+    s = MultiGridSchemeHIP(p.seg_x, p.seg_y, p.hx, p.hy, p.f, options=Options.performance(mixed))
+    s.un0[:] = u0
+    s.init_after_boundaries_and_initial_iteration_inited()
+    st = s.solve(1e-10, 60)
+    assert st["cycles"] == kr
+    assert st["rel_residual"] <= 1e-10
+    assert abs(st["rel_residual"] - hr[-1]) <= 1e-6 * hr[-1]
+    ug = s.get_array()
+    assert np.array_equal(ug, ur), np.abs(ug - ur).max()
+    ex = dst_solve(p.f, p.g, p.hx, p.hy)
+    assert np.abs(ug - ex).max() <= 1e-8 * np.abs(ex).max()
+    s.cleanup()
+
+
+def test_config2_full_size_parity():
+    """Config 2 at its full size (4097 x 2049, coarsest 5 x 3): FP64 GPU vs
+    the oracle's double multigrid at equal cycle counts (V(3,3), Mean init).
+    Bar from the specification: max|du| <= 1e-12 max|u|; bitwise expected."""
+    p = problems.config(2)
+    _say("config2 4097x2049 parity")
+    u0 = p.initial("mean")
+    o = Oracle(p.seg_x, p.seg_y, p.hx, p.hy, p.f)
+    o.set_u(u0)
+    s = MultiGridSchemeHIP(p.seg_x, p.seg_y, p.hx, p.hy, p.f, eps=1e-13)
+    s.un0[:] = u0
+    s.init_after_boundaries_and_initial_iteration_inited()
+    for k in range(4):
+        mr = o.vcycle(3, 3, 3)
+        s.do_iteration(k)
+        assert s.last_metric == mr
+    ug, ur = s.get_array(), o.get_u()
+    assert np.abs(ug - ur).max() <= 1e-12 * np.abs(ur).max()
+    assert np.array_equal(ug, ur)
+    s.cleanup()
+
+
+def test_mixed_full_size_8193():
+    """Config 3 at full size (8193^2): FP32 V(2,2) cycles inside FP64
+    refinement reach relative residual 1e-10 (size-independent property),
+    and agree with the plain FP64 GPU multigrid solution to the accuracy the
+    residual bound implies."""
+    p = problems.config(3)
+    _say("config3 8193^2 mixed vs fp64")
+    out = {}
+    for mixed in (True, False):
+        s = MultiGridSchemeHIP(p.seg_x, p.seg_y, p.hx, p.hy, p.f, options=Options.performance(mixed))
+        s.un0[:] = p.initial("zero")
+        s.init_after_boundaries_and_initial_iteration_inited()
+        st = s.solve(1e-10, 40)
+        assert st["rel_residual"] <= 1e-10
+        out[mixed] = (s.get_array().copy(), st)
+        _say(f"  mixed={mixed} cycles={st['cycles']} t={st['seconds']:.4f}s")
+        s.cleanup()
+    d = np.abs(out[True][0] - out[False][0]).max()
+    assert d <= 1e-7 * np.abs(out[False][0]).max()
+
+
+# ------------------------------------------------------------------ adjoint
+@pytest.mark.parametrize("seg_x,seg_y,hx,hy,K", [(16, 16, 1 / 16, 1 / 16, 2), (64, 32, 0.01, 0.02, 3)])
+def test_adjoint_cycles_match_oracle(seg_x, seg_y, hx, hy, K):
+    _say(f"adjoint cycles {seg_x}x{seg_y} K={K}")
+    f, g = _random(seg_x, seg_y, 99)
+    rng = np.random.default_rng(5)
+    ubar = rng.standard_normal(g.shape)
+    s = MultiGridSchemeHIP(seg_x, seg_y, hx, hy, f, options=Options.performance())
+    s.un0[:] = g
+    s.init_after_boundaries_and_initial_iteration_inited()
+    s.forward_cycles(K)
+    fb, gb, u0b = s.adjoint_cycles(ubar)
+    rf, rg, ru = adjoint_cycles(seg_x, seg_y, hx, hy, ubar, K, 2, 2, 3)
+    assert np.array_equal(fb, rf) and np.array_equal(gb, rg) and np.array_equal(u0b, ru)
+    s.cleanup()
+
+
+def test_adjoint_exact_matches_dst_gradient():
+    seg, h = 128, 1 / 128
+    _say("adjoint exact 129^2")
+    rng = np.random.default_rng(8)
+    ubar = np.zeros((seg + 1, seg + 1))
+    ubar[1:-1, 1:-1] = rng.standard_normal((seg - 1, seg - 1))
+    s = MultiGridSchemeHIP(seg, seg, h, h, np.zeros((seg + 1, seg + 1)), options=Options.performance())
+    fb, gb = s.adjoint_exact(ubar, 1e-12, 60)
+    lam, gex = dst_adjoint(ubar, h, h)
+    assert np.abs(fb - lam).max() <= 1e-9 * np.abs(lam).max()
+    assert np.abs(gb - gex).max() <= 1e-9 * np.abs(gex).max()
+    s.cleanup()
+
+
+# ------------------------------------------------------------ error handling
+def test_errors_are_reported_not_faulted():
+    f, g = _random(16, 16, 3)
+    s = MultiGridSchemeHIP(16, 16, 1 / 16, 1 / 16, f, options=Options.performance())
+    bad = g.copy()
+    bad[0, 3] = np.inf
+    with pytest.raises(VLP2DError) as e:
+        _lib.check(s.lib, s.h, s.lib.vlp2d_mg_upload(s.h, _lib.ptr(bad)))
+    assert e.value.code == _lib.E_NONFINITE
+    s.un0[:] = g
+    s.init_after_boundaries_and_initial_iteration_inited()
+    st = s.solve(1e-14, 1, raise_on_notconv=False)
+    assert st["rc"] == _lib.E_NOTCONV and st["cycles"] == 1
+    s.cancel_iterations()
+    with pytest.raises(VLP2DError) as e:
+        s.solve(1e-14, 50)
+    assert e.value.code == _lib.E_CANCELLED
+    s.init_after_boundaries_and_initial_iteration_inited()  # upload clears the request
+    assert s.solve(1e-10, 50)["rel_residual"] <= 1e-10
+    with pytest.raises(VLP2DError) as e:
+        s.adjoint_cycles(g)
+    assert e.value.code == _lib.E_STATE
+    s.cleanup()

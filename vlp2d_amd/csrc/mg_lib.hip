@@ -1,0 +1,953 @@
+// libvlp2d_mg: gfx950 multigrid Dirichlet solver behind the C ABI declared in
+// include/vlp2d_mg.h.
+//
+// One handle = one device, one non-blocking HIP stream, one level hierarchy
+// resident in HBM.  The fixed launch sequence of a V-cycle (or of one mixed-
+// precision refinement step) is captured once into a hipGraph and replayed;
+// the only host synchronisation per cycle is the 8-byte read of the residual
+// norm (or update norm) that decides whether to stop.
+//
+// Reference behaviour mirrored (paths relative to VisualLaplacePoisson2D/):
+//   Model/CPU/MultiGridScheme.cs:23-61    hierarchy: nLevels = min(log2 segX,
+//                                         log2 segY), h_l = (h*seg)/seg_l
+//   Model/CPU/MultiGridScheme.cs:93-115   V-cycle order, snapshot before the
+//                                         level-0 post-smoothing
+//   Model/CUDA/MultiGridSchemeCU.cs:140-146  doIteration -> update-norm test
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/vlp2d_mg.h"
+#include "mg_kernels.hpp"
+
+using namespace vlp2d;
+
+namespace {
+
+thread_local std::string g_create_error;
+
+enum KClass {
+  KC_SMOOTH,
+  KC_RESID,
+  KC_RESTRICT,
+  KC_FILL,
+  KC_PROLONG,
+  KC_COPY,
+  KC_MAXDIFF,
+  KC_RESNORM,
+  KC_REDUCE,
+  KC_UPDATE,
+  KC_ADJ,
+  KC_COUNT
+};
+const char *kKClassName[KC_COUNT] = {"rb_half",   "residual",  "restrict", "fill",         "prolong", "copy_interior",
+                                     "absdiff_max", "resnorm64", "reduce", "update_mixed", "adjoint"};
+
+template <typename T> struct Lvl {
+  T *u = nullptr, *rhs = nullptr, *res = nullptr;
+  int nx = 0, ny = 0, pitch = 0;
+  T hx2 = 0, hy2 = 0, coef = 0;
+  bool eq = true, lap = false;
+  double ix2 = 0, iy2 = 0;  // 1/hx^2, 1/hy^2 in double (adjoint)
+  double ax = 0, ay = 0, b = 0;  // smoother coefficients in double (adjoint)
+  size_t elems() const { return (size_t)nx * pitch; }
+};
+
+template <typename T> struct Hier {
+  std::vector<Lvl<T>> lv;
+  bool full = false;  // levels >= 1 and residual buffers allocated
+};
+
+struct PendingEv {
+  int kc, lvl;
+  hipEvent_t a, b;
+};
+constexpr int kMaxLv = 16;
+
+}  // namespace
+
+struct vlp2d_mg_ctx {
+  int dev = 0;
+  hipStream_t st = nullptr;
+  int segX = 0, segY = 0, nx = 0, ny = 0, L = 0;
+  double hx = 0, hy = 0, eps = 0;
+  bool laplace = false;
+  vlp2d_mg_opts o{};
+  Hier<double> h64;   // the problem: level 0 u (with boundary), rhs = fn
+  Hier<float> h32;    // FP32 correction hierarchy (mixed precision)
+  Hier<double> ha;    // adjoint-solve hierarchy (exact adjoint)
+  double *snap = nullptr;
+  double *part = nullptr;
+  int part_cap = 0;
+  double *dscal = nullptr;  // device scalars
+  double *hscal = nullptr;  // pinned host scalars
+  bool r32_valid = false;   // h32 level-0 rhs holds (float)(fn - Lap u64)
+  // transposed-cycle buffers
+  std::vector<double *> aub, arb, aresb;
+  std::vector<double *> ckpt;
+  int K = 0;
+  // graphs: 0 cycle64+metric, 1 cycle64+norm, 2 mixed step, 3 adjoint-solve cycle64+norm, 4 adjoint mixed step
+  hipGraphExec_t gexec[6] = {};
+  hipGraph_t graph[6] = {};
+  // profiling
+  bool profile = false;
+  std::vector<PendingEv> pend;
+  long long kc_launch[KC_COUNT][kMaxLv] = {};
+  double kc_ms[KC_COUNT][kMaxLv] = {};
+  double kc_bytes[KC_COUNT][kMaxLv] = {};
+  double *saved = nullptr;  // copy of the level-0 iterate (save/restore)
+  std::atomic<int> cancel{0};
+  std::string err;
+  long long bytes = 0, peak = 0;
+};
+
+namespace {
+
+using Ctx = vlp2d_mg_ctx;
+
+#define HIPCHK(ctx, call)                                                                          \
+  do {                                                                                             \
+    hipError_t e_ = (call);                                                                        \
+    if (e_ != hipSuccess) {                                                                        \
+      (ctx)->err = std::string(#call) + ": " + hipGetErrorString(e_);                              \
+      return e_ == hipErrorOutOfMemory ? VLP2D_E_OOM : VLP2D_E_DEVICE;                             \
+    }                                                                                              \
+  } while (0)
+
+#define RCHK(call)          \
+  do {                      \
+    int r_ = (call);        \
+    if (r_ != VLP2D_OK) return r_; \
+  } while (0)
+
+int fail(Ctx *c, int code, const std::string &msg) {
+  c->err = msg;
+  return code;
+}
+
+template <typename T> int dalloc(Ctx *c, T **p, size_t n) {
+  hipError_t e = hipMalloc((void **)p, n * sizeof(T));
+  if (e != hipSuccess) {
+    *p = nullptr;
+    (void)hipGetLastError();
+    c->err = "hipMalloc of " + std::to_string(n * sizeof(T)) + " bytes failed: " + hipGetErrorString(e);
+    return e == hipErrorOutOfMemory ? VLP2D_E_OOM : VLP2D_E_DEVICE;
+  }
+  c->bytes += (long long)(n * sizeof(T));
+  c->peak = std::max(c->peak, c->bytes);
+  return VLP2D_OK;
+}
+template <typename T> void dfree(Ctx *c, T *&p, size_t n) {
+  if (p) {
+    (void)hipFree(p);
+    c->bytes -= (long long)(n * sizeof(T));
+    p = nullptr;
+  }
+}
+
+inline int round_pitch(int ny) { return (ny + 31) / 32 * 32; }
+
+// ---- launches -----------------------------------------------------------
+// Every launch is tagged with its kernel class, the level it works on and its
+// algorithmic bytes (compulsory HBM traffic of the operator: each array it
+// must read or write streamed once).  Under VLP2D_FLAG_PROFILE each launch is
+// bracketed by hipEvents on the handle's stream.
+template <typename F> void launch(Ctx *c, int kc, int lvl, double bytes, F &&f) {
+  lvl = std::max(0, std::min(lvl, kMaxLv - 1));
+  if (c->profile) {
+    PendingEv pe{kc, lvl, nullptr, nullptr};
+    (void)hipEventCreate(&pe.a);
+    (void)hipEventCreate(&pe.b);
+    (void)hipEventRecord(pe.a, c->st);
+    f();
+    (void)hipEventRecord(pe.b, c->st);
+    c->pend.push_back(pe);
+  } else {
+    f();
+  }
+  c->kc_launch[kc][lvl] += 1;
+  c->kc_bytes[kc][lvl] += bytes;
+}
+
+void collect_profile(Ctx *c) {
+  for (auto &pe : c->pend) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, pe.a, pe.b) == hipSuccess) c->kc_ms[pe.kc][pe.lvl] += ms;
+    (void)hipEventDestroy(pe.a);
+    (void)hipEventDestroy(pe.b);
+  }
+  c->pend.clear();
+}
+
+inline dim3 grid2(int nj, int ni) { return dim3((unsigned)std::max(1, (nj + BX - 1) / BX), (unsigned)std::max(1, (ni + BY - 1) / BY)); }
+const dim3 kBlock(BX, BY);
+
+// partial-reduction grid: x over j chunks, y capped rows of blocks
+inline dim3 grid_red(int ny, int nx) {
+  int gx = std::max(1, (ny - 2 + BX - 1) / BX);
+  int gy = std::max(1, std::min((nx - 2 + BY - 1) / BY, 64));
+  return dim3(gx, gy);
+}
+
+template <typename T> double nodes(const Lvl<T> &v) { return (double)v.nx * (double)v.ny; }
+
+template <typename T> void fill(Ctx *c, T *p, size_t n, T v, int lvl = 0) {
+  launch(c, KC_FILL, lvl, (double)n * sizeof(T), [&] {
+    int blocks = (int)std::min<size_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_fill<T>, dim3(std::max(blocks, 1)), dim3(256), 0, c->st, p, (long long)n, v);
+  });
+}
+
+// one colour: reads the other colour's u and this colour's f, writes this
+// colour's u -> 1.5 arrays
+template <typename T> void half_sweep(Ctx *c, Lvl<T> &v, int lvl, int color) {
+  int mcount = (v.ny - 2 + 1) / 2;
+  dim3 g = grid2(mcount, v.nx - 2);
+  launch(c, KC_SMOOTH, lvl, 1.5 * sizeof(T) * nodes(v), [&] {
+    if (v.eq) {
+      if (v.lap)
+        hipLaunchKernelGGL((k_rb_half<T, true, true>), g, kBlock, 0, c->st, v.u, v.rhs, v.nx, v.ny, v.pitch, v.hx2, v.hy2, v.coef, color);
+      else
+        hipLaunchKernelGGL((k_rb_half<T, true, false>), g, kBlock, 0, c->st, v.u, v.rhs, v.nx, v.ny, v.pitch, v.hx2, v.hy2, v.coef, color);
+    } else {
+      if (v.lap)
+        hipLaunchKernelGGL((k_rb_half<T, false, true>), g, kBlock, 0, c->st, v.u, v.rhs, v.nx, v.ny, v.pitch, v.hx2, v.hy2, v.coef, color);
+      else
+        hipLaunchKernelGGL((k_rb_half<T, false, false>), g, kBlock, 0, c->st, v.u, v.rhs, v.nx, v.ny, v.pitch, v.hx2, v.hy2, v.coef, color);
+    }
+  });
+}
+
+template <typename T> void smooth(Ctx *c, Lvl<T> &v, int lvl, int n) {
+  for (int s = 0; s < n; ++s) {
+    half_sweep(c, v, lvl, 0);
+    half_sweep(c, v, lvl, 1);
+  }
+}
+
+// V-cycle on hierarchy H (MultiGridScheme.cs:93-115).  If snap != nullptr the
+// level-0 iterate is copied there before post-smoothing.
+template <typename T> void vcycle(Ctx *c, Hier<T> &H, double *snap) {
+  const int L = (int)H.lv.size();
+  const double s = sizeof(T);
+  for (int l = 0; l <= L - 2; ++l) {
+    Lvl<T> &f = H.lv[l];
+    Lvl<T> &g = H.lv[l + 1];
+    smooth(c, f, l, c->o.nu1);
+    launch(c, KC_RESID, l, 3 * s * nodes(f), [&] {
+      hipLaunchKernelGGL(k_residual<T>, grid2(f.ny - 2, f.nx - 2), kBlock, 0, c->st, f.res, f.rhs, f.u, f.nx, f.ny, f.pitch, f.hx2, f.hy2);
+    });
+    launch(c, KC_RESTRICT, l, s * (nodes(f) + nodes(g)), [&] {
+      hipLaunchKernelGGL(k_restrict<T>, grid2(g.ny - 2, g.nx - 2), kBlock, 0, c->st, g.rhs, f.res, g.nx, g.ny, g.pitch, f.pitch);
+    });
+    fill(c, g.u, g.elems(), T(0), l + 1);
+  }
+  smooth(c, H.lv[L - 1], L - 1, c->o.nu_coarse);
+  for (int l = L - 2; l >= 0; --l) {
+    Lvl<T> &f = H.lv[l];
+    Lvl<T> &g = H.lv[l + 1];
+    launch(c, KC_PROLONG, l, s * (2 * nodes(f) + nodes(g)), [&] {
+      hipLaunchKernelGGL(k_prolong<T>, grid2(g.ny - 1, g.nx - 1), kBlock, 0, c->st, f.u, g.u, g.nx, g.ny, f.pitch, g.pitch);
+    });
+    if (l == 0 && snap) {
+      launch(c, KC_COPY, l, 2 * s * nodes(f), [&] {
+        hipLaunchKernelGGL(k_copy_interior<T>, grid2(f.ny - 2, f.nx - 2), kBlock, 0, c->st, (T *)(void *)snap, f.u, f.nx, f.ny, f.pitch);
+      });
+    }
+    smooth(c, f, l, c->o.nu2);
+  }
+}
+
+// residual norm^2 of level 0 of H (FP64) -> dscal[slot]; optional FP32 copy
+void resnorm(Ctx *c, Lvl<double> &v, int slot, float *r32) {
+  dim3 g = grid_red(v.ny, v.nx);
+  int np = (int)(g.x * g.y);
+  launch(c, KC_RESNORM, 0, (16.0 + (r32 ? 4.0 : 0.0)) * nodes(v), [&] {
+    hipLaunchKernelGGL(k_resnorm64, g, kBlock, 0, c->st, v.u, v.rhs, v.nx, v.ny, v.pitch, (double)v.hx2, (double)v.hy2, c->part, r32);
+  });
+  launch(c, KC_REDUCE, 0, 8.0 * np, [&] { hipLaunchKernelGGL(k_reduce<false>, dim3(1), dim3(RED_THREADS), 0, c->st, c->part, np, c->dscal + slot); });
+}
+
+void maxdiff(Ctx *c, Lvl<double> &v, const double *snap, int slot) {
+  dim3 g = grid_red(v.ny, v.nx);
+  int np = (int)(g.x * g.y);
+  launch(c, KC_MAXDIFF, 0, 16.0 * nodes(v), [&] {
+    hipLaunchKernelGGL(k_absdiff_max<double>, g, kBlock, 0, c->st, v.u, snap, v.nx, v.ny, v.pitch, c->part);
+  });
+  launch(c, KC_REDUCE, 0, 8.0 * np, [&] { hipLaunchKernelGGL(k_reduce<true>, dim3(1), dim3(RED_THREADS), 0, c->st, c->part, np, c->dscal + slot); });
+}
+
+// one mixed-precision refinement step on problem level P (FP64) using h32:
+// e = V-cycle(rhs32) from 0; u += e (max|e| -> dscal[0]); r32 = fn - Lap u,
+// ||r||^2 -> dscal[1]
+void mixed_step(Ctx *c, Lvl<double> &P) {
+  Lvl<float> &z = c->h32.lv[0];
+  fill(c, z.u, z.elems(), 0.0f);
+  vcycle(c, c->h32, nullptr);
+  dim3 g = grid_red(P.ny, P.nx);
+  int np = (int)(g.x * g.y);
+  launch(c, KC_UPDATE, 0, 20.0 * nodes(P), [&] { hipLaunchKernelGGL(k_update_mixed, g, kBlock, 0, c->st, P.u, z.u, P.nx, P.ny, P.pitch, c->part); });
+  launch(c, KC_REDUCE, 0, 8.0 * np, [&] { hipLaunchKernelGGL(k_reduce<true>, dim3(1), dim3(RED_THREADS), 0, c->st, c->part, np, c->dscal + 0); });
+  resnorm(c, P, 1, z.rhs);
+}
+
+// ---- graph wrapper --------------------------------------------------------
+template <typename F> int run_program(Ctx *c, int slot, F &&body) {
+  bool use_graph = !(c->o.flags & VLP2D_FLAG_NO_GRAPH) && !c->profile;
+  if (!use_graph) {
+    body();
+    HIPCHK(c, hipGetLastError());
+    return VLP2D_OK;
+  }
+  if (!c->gexec[slot]) {
+    HIPCHK(c, hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+    body();
+    hipError_t e = hipStreamEndCapture(c->st, &c->graph[slot]);
+    if (e != hipSuccess) return fail(c, VLP2D_E_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    HIPCHK(c, hipGraphInstantiate(&c->gexec[slot], c->graph[slot], nullptr, nullptr, 0));
+  }
+  HIPCHK(c, hipGraphLaunch(c->gexec[slot], c->st));
+  return VLP2D_OK;
+}
+
+int sync_read(Ctx *c, int nslots) {
+  HIPCHK(c, hipMemcpyAsync(c->hscal, c->dscal, sizeof(double) * nslots, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  if (c->profile) collect_profile(c);
+  return VLP2D_OK;
+}
+
+// ---- hierarchy construction ------------------------------------------------
+// Steps follow MultiGridScheme.cs:44-59: lng = h*seg, h_l = lng/seg_l in the
+// level's arithmetic type.
+template <typename T> void level_params(Lvl<T> &v, T hx0, T hy0, int segX, int segY, int sx, int sy) {
+  T lngX = hx0 * (T)segX, lngY = hy0 * (T)segY;
+  T hx = lngX / (T)sx, hy = lngY / (T)sy;
+  v.hx2 = hx * hx;
+  v.hy2 = hy * hy;
+  v.coef = T(1) / (T(2) / v.hx2 + T(2) / v.hy2);
+  T d = hx - hy;
+  v.eq = (d < T(0) ? -d : d) < (T)1e-15;
+  v.ix2 = 1.0 / (double)v.hx2;
+  v.iy2 = 1.0 / (double)v.hy2;
+  if (v.eq) {
+    v.ax = 0.25;
+    v.ay = 0.25;
+    v.b = 0.25 * (double)v.hx2;
+  } else {
+    v.ax = (double)v.coef / (double)v.hx2;
+    v.ay = (double)v.coef / (double)v.hy2;
+    v.b = (double)v.coef;
+  }
+}
+
+template <typename T> void shape_hier(Ctx *c, Hier<T> &H, T hx0, T hy0) {
+  H.lv.resize(c->L);
+  int sx = c->segX, sy = c->segY;
+  for (int l = 0; l < c->L; ++l) {
+    Lvl<T> &v = H.lv[l];
+    v.nx = sx + 1;
+    v.ny = sy + 1;
+    v.pitch = round_pitch(v.ny);
+    level_params(v, hx0, hy0, c->segX, c->segY, sx, sy);
+    sx /= 2;
+    sy /= 2;
+  }
+}
+
+// Allocate levels >= 1 (u, rhs) and residual buffers of levels < L-1.
+template <typename T> int complete_hier(Ctx *c, Hier<T> &H) {
+  if (H.full) return VLP2D_OK;
+  for (int l = 0; l < c->L; ++l) {
+    Lvl<T> &v = H.lv[l];
+    if (l >= 1) {
+      if (!v.u) RCHK(dalloc(c, &v.u, v.elems()));
+      if (!v.rhs) {
+        RCHK(dalloc(c, &v.rhs, v.elems()));
+        fill(c, v.rhs, v.elems(), T(0));
+      }
+      fill(c, v.u, v.elems(), T(0));
+    }
+    if (l < c->L - 1 && !v.res) {
+      RCHK(dalloc(c, &v.res, v.elems()));
+      fill(c, v.res, v.elems(), T(0));
+    }
+  }
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  H.full = true;
+  return VLP2D_OK;
+}
+
+template <typename T> void free_hier(Ctx *c, Hier<T> &H) {
+  for (auto &v : H.lv) {
+    dfree(c, v.u, v.elems());
+    dfree(c, v.rhs, v.elems());
+    dfree(c, v.res, v.elems());
+  }
+  H.lv.clear();
+  H.full = false;
+}
+
+int ensure_h32(Ctx *c) {
+  if (!c->h32.lv.empty() && c->h32.full) return VLP2D_OK;
+  shape_hier(c, c->h32, (float)c->hx, (float)c->hy);
+  Lvl<float> &z = c->h32.lv[0];
+  z.lap = false;
+  RCHK(dalloc(c, &z.u, z.elems()));
+  RCHK(dalloc(c, &z.rhs, z.elems()));
+  fill(c, z.u, z.elems(), 0.0f);
+  fill(c, z.rhs, z.elems(), 0.0f);
+  return complete_hier(c, c->h32);
+}
+
+int ensure_aux(Ctx *c) {
+  if (c->snap) return VLP2D_OK;
+  Lvl<double> &v = c->h64.lv[0];
+  RCHK(dalloc(c, &c->snap, v.elems()));
+  fill(c, c->snap, v.elems(), 0.0);
+  return VLP2D_OK;
+}
+
+void invalidate_graphs(Ctx *c) {
+  for (int k = 0; k < 6; ++k) {
+    if (c->gexec[k]) (void)hipGraphExecDestroy(c->gexec[k]);
+    if (c->graph[k]) (void)hipGraphDestroy(c->graph[k]);
+    c->gexec[k] = nullptr;
+    c->graph[k] = nullptr;
+  }
+}
+
+bool is_pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
+
+int check_finite(const double *a, size_t n) {
+  for (size_t k = 0; k < n; ++k)
+    if (!std::isfinite(a[k])) return 0;
+  return 1;
+}
+
+int upload_grid(Ctx *c, double *dst, int pitch, const double *src) {
+  HIPCHK(c, hipMemcpy2DAsync(dst, (size_t)pitch * sizeof(double), src, (size_t)c->ny * sizeof(double),
+                             (size_t)c->ny * sizeof(double), (size_t)c->nx, hipMemcpyHostToDevice, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  return VLP2D_OK;
+}
+int download_grid(Ctx *c, double *dst, const double *src, int pitch) {
+  HIPCHK(c, hipMemcpy2DAsync(dst, (size_t)c->ny * sizeof(double), src, (size_t)pitch * sizeof(double),
+                             (size_t)c->ny * sizeof(double), (size_t)c->nx, hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  return VLP2D_OK;
+}
+
+// Solve on the FP64 problem level P (u, rhs = fn) with hierarchy H64 (levels
+// >= 1 used only in FP64 mode).  Graph slots: base (FP64) / base+1 (mixed).
+int solve_impl(Ctx *c, Hier<double> &H, int slot_fp64, int slot_mixed, double rtol, int max_cycles,
+               vlp2d_mg_stats *st) {
+  Lvl<double> &P = H.lv[0];
+  const bool mixed = c->o.precision == VLP2D_PREC_MIXED;
+  if (mixed) RCHK(ensure_h32(c));
+  else RCHK(complete_hier(c, H));
+  hipEvent_t e0, e1;
+  HIPCHK(c, hipEventCreate(&e0));
+  HIPCHK(c, hipEventCreate(&e1));
+  HIPCHK(c, hipEventRecord(e0, c->st));
+  resnorm(c, P, 1, mixed ? c->h32.lv[0].rhs : nullptr);
+  HIPCHK(c, hipGetLastError());
+  RCHK(sync_read(c, 2));
+  const double r0 = std::sqrt(c->hscal[1]);
+  if (mixed) c->r32_valid = (&H == &c->h64);
+  if (!std::isfinite(r0)) return fail(c, VLP2D_E_NONFINITE, "initial residual is not finite");
+  double rel = r0 > 0 ? 1.0 : 0.0;
+  int k = 0;
+  int rc = VLP2D_OK;
+  while (r0 > 0 && rel > rtol) {
+    if (k >= max_cycles) {
+      rc = VLP2D_E_NOTCONV;
+      break;
+    }
+    if (c->cancel.load()) {
+      rc = VLP2D_E_CANCELLED;
+      break;
+    }
+    if (mixed) {
+      RCHK(run_program(c, slot_mixed, [&] { mixed_step(c, P); }));
+    } else {
+      RCHK(run_program(c, slot_fp64, [&] {
+        vcycle(c, H, nullptr);
+        resnorm(c, P, 1, nullptr);
+      }));
+    }
+    RCHK(sync_read(c, 2));
+    ++k;
+    rel = std::sqrt(c->hscal[1]) / r0;
+    if (!std::isfinite(rel)) {
+      rc = VLP2D_E_NONFINITE;
+      c->err = "residual became non-finite";
+      break;
+    }
+  }
+  HIPCHK(c, hipEventRecord(e1, c->st));
+  HIPCHK(c, hipEventSynchronize(e1));
+  float ms = 0.f;
+  HIPCHK(c, hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (st) {
+    st->cycles = k;
+    st->levels = c->L;
+    st->r0 = r0;
+    st->rel_residual = rel;
+    st->seconds = ms * 1e-3;
+    double unknowns = (double)(c->nx - 2) * (double)(c->ny - 2);
+    st->munknowns_per_s = st->seconds > 0 ? unknowns / st->seconds / 1e6 : 0.0;
+  }
+  if (rc == VLP2D_E_NOTCONV) c->err = "max_cycles reached before rtol";
+  if (rc == VLP2D_E_CANCELLED) c->err = "cancelled";
+  return rc;
+}
+
+// ---- transposed cycle -------------------------------------------------------
+int ensure_adj_buffers(Ctx *c) {
+  if (!c->aub.empty()) return VLP2D_OK;
+  c->aub.assign(c->L, nullptr);
+  c->arb.assign(c->L, nullptr);
+  c->aresb.assign(c->L, nullptr);
+  for (int l = 0; l < c->L; ++l) {
+    size_t n = c->h64.lv[l].elems();
+    RCHK(dalloc(c, &c->aub[l], n));
+    RCHK(dalloc(c, &c->arb[l], n));
+    fill(c, c->aub[l], n, 0.0);
+    fill(c, c->arb[l], n, 0.0);
+    if (l < c->L - 1) {
+      RCHK(dalloc(c, &c->aresb[l], n));
+      fill(c, c->aresb[l], n, 0.0);
+    }
+  }
+  return VLP2D_OK;
+}
+
+void half_sweep_T(Ctx *c, int l, int color) {
+  Lvl<double> &v = c->h64.lv[l];
+  launch(c, KC_ADJ, l, 16.0 * nodes(v), [&] {
+    hipLaunchKernelGGL(k_adj_gather, grid2(v.ny, v.nx), kBlock, 0, c->st, c->aub[l], v.nx, v.ny, v.pitch, v.ax, v.ay, color);
+  });
+  launch(c, KC_ADJ, l, 16.0 * nodes(v), [&] {
+    hipLaunchKernelGGL(k_adj_color, grid2(v.ny - 2, v.nx - 2), kBlock, 0, c->st, c->aub[l], c->arb[l], v.nx, v.ny, v.pitch, v.b, color);
+  });
+}
+void smooth_T(Ctx *c, int l, int n) {
+  for (int s = 0; s < n; ++s) {
+    half_sweep_T(c, l, 1);
+    half_sweep_T(c, l, 0);
+  }
+}
+void cycle_T(Ctx *c, int l) {
+  if (l == c->L - 1) {
+    smooth_T(c, l, c->o.nu_coarse);
+    return;
+  }
+  Lvl<double> &f = c->h64.lv[l];
+  Lvl<double> &g = c->h64.lv[l + 1];
+  smooth_T(c, l, c->o.nu2);
+  launch(c, KC_ADJ, l, 8.0 * (nodes(f) + nodes(g)), [&] {
+    hipLaunchKernelGGL(k_adj_prolongT, grid2(g.ny, g.nx), kBlock, 0, c->st, c->aub[l + 1], c->aub[l], g.nx, g.ny, g.pitch, f.pitch);
+  });
+  fill(c, c->arb[l + 1], g.elems(), 0.0, l + 1);
+  cycle_T(c, l + 1);
+  launch(c, KC_ADJ, l, 8.0 * (nodes(f) + nodes(g)), [&] {
+    hipLaunchKernelGGL(k_adj_restrictT, grid2(f.ny, f.nx), kBlock, 0, c->st, c->aresb[l], c->arb[l + 1], f.nx, f.ny, f.pitch, g.nx, g.ny, g.pitch);
+  });
+  launch(c, KC_ADJ, l, 40.0 * nodes(f), [&] {
+    hipLaunchKernelGGL(k_adj_residualT, grid2(f.ny, f.nx), kBlock, 0, c->st, c->aub[l], c->arb[l], c->aresb[l], f.nx, f.ny, f.pitch, f.ix2, f.iy2);
+  });
+  smooth_T(c, l, c->o.nu1);
+}
+
+}  // namespace
+
+// =============================================================================
+// C ABI
+// =============================================================================
+extern "C" {
+
+void vlp2d_mg_default_opts(vlp2d_mg_opts *o) {
+  if (!o) return;
+  o->nu1 = 3;
+  o->nu2 = 3;
+  o->nu_coarse = 3;
+  o->precision = VLP2D_PREC_FP64;
+  o->flags = 0;
+  o->coarse_nodes = 0;
+}
+
+int vlp2d_mg_version(void) { return (1 << 16) | 0; }
+
+int vlp2d_mg_create(int segX, int segY, double hx, double hy, const double *f, double eps, int device,
+                    const vlp2d_mg_opts *opts, vlp2d_mg_handle *out) {
+  g_create_error.clear();
+  if (!out) {
+    g_create_error = "out handle pointer is NULL";
+    return VLP2D_E_ARG;
+  }
+  *out = nullptr;
+  if (!is_pow2(segX) || !is_pow2(segY) || segX < 4 || segY < 4 || segX > 32768 || segY > 32768) {
+    g_create_error = "segX and segY must be powers of two in [4, 32768]";
+    return VLP2D_E_SIZE;
+  }
+  if (!(hx > 0) || !(hy > 0) || !std::isfinite(hx) || !std::isfinite(hy)) {
+    g_create_error = "hx and hy must be finite and positive";
+    return VLP2D_E_ARG;
+  }
+  vlp2d_mg_opts o;
+  vlp2d_mg_default_opts(&o);
+  if (opts) o = *opts;
+  if (o.nu1 < 0 || o.nu2 < 0 || o.nu_coarse < 1 || o.nu1 > 32 || o.nu2 > 32 || o.nu_coarse > 1024 ||
+      (o.precision != VLP2D_PREC_FP64 && o.precision != VLP2D_PREC_MIXED)) {
+    g_create_error = "invalid options (nu1, nu2 in [0,32], nu_coarse in [1,1024], precision 0 or 1)";
+    return VLP2D_E_ARG;
+  }
+  const int nx = segX + 1, ny = segY + 1;
+  if (f && !check_finite(f, (size_t)nx * ny)) {
+    // only interior values are used, but a NaN anywhere is a caller error
+    g_create_error = "f contains NaN or Inf";
+    return VLP2D_E_NONFINITE;
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) {
+    (void)hipGetLastError();
+    g_create_error = "device " + std::to_string(device) + " not available (" + std::to_string(ndev) + " visible)";
+    return VLP2D_E_DEVICE;
+  }
+  Ctx *c = new Ctx();
+  c->dev = device;
+  c->segX = segX;
+  c->segY = segY;
+  c->nx = nx;
+  c->ny = ny;
+  c->hx = hx;
+  c->hy = hy;
+  c->eps = eps;
+  c->laplace = (f == nullptr);
+  c->o = o;
+  c->profile = (o.flags & VLP2D_FLAG_PROFILE) != 0;
+  c->L = std::min((int)std::log2((double)segX + 0.5), (int)std::log2((double)segY + 0.5));
+  auto bail = [&](int rc) {
+    g_create_error = c->err;
+    vlp2d_mg_destroy(c);
+    return rc;
+  };
+  if (hipSetDevice(device) != hipSuccess) {
+    c->err = "hipSetDevice failed";
+    return bail(VLP2D_E_DEVICE);
+  }
+  if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) {
+    c->err = "hipStreamCreate failed";
+    return bail(VLP2D_E_DEVICE);
+  }
+  shape_hier(c, c->h64, hx, hy);
+  Lvl<double> &v = c->h64.lv[0];
+  v.lap = c->laplace;
+  int rc;
+  if ((rc = dalloc(c, &v.u, v.elems())) != VLP2D_OK) return bail(rc);
+  if ((rc = dalloc(c, &v.rhs, v.elems())) != VLP2D_OK) return bail(rc);
+  fill(c, v.u, v.elems(), 0.0);
+  fill(c, v.rhs, v.elems(), 0.0);
+  c->part_cap = 1 << 16;
+  if ((rc = dalloc(c, &c->part, (size_t)c->part_cap)) != VLP2D_OK) return bail(rc);
+  if ((rc = dalloc(c, &c->dscal, 8)) != VLP2D_OK) return bail(rc);
+  if (hipHostMalloc((void **)&c->hscal, 8 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+    c->err = "hipHostMalloc failed";
+    return bail(VLP2D_E_OOM);
+  }
+  if (f) {
+    // fn = -f on interior nodes, 0 on the boundary (MultiGridSchemeCU.cs:41-45)
+    std::vector<double> fn((size_t)nx * ny, 0.0);
+    for (int i = 1; i < nx - 1; ++i)
+      for (int j = 1; j < ny - 1; ++j) fn[(size_t)i * ny + j] = -f[(size_t)i * ny + j];
+    if ((rc = upload_grid(c, v.rhs, v.pitch, fn.data())) != VLP2D_OK) return bail(rc);
+  }
+  if (hipStreamSynchronize(c->st) != hipSuccess) {
+    c->err = "stream synchronisation failed";
+    return bail(VLP2D_E_DEVICE);
+  }
+  *out = c;
+  return VLP2D_OK;
+}
+
+int vlp2d_mg_upload(vlp2d_mg_handle c, const double *u) {
+  if (!c) return VLP2D_E_ARG;
+  if (!u) return fail(c, VLP2D_E_ARG, "u is NULL");
+  if (!check_finite(u, (size_t)c->nx * c->ny)) return fail(c, VLP2D_E_NONFINITE, "u contains NaN or Inf");
+  HIPCHK(c, hipSetDevice(c->dev));
+  c->r32_valid = false;
+  c->K = 0;
+  c->cancel.store(0);  // a cancel request holds until the next upload
+  return upload_grid(c, c->h64.lv[0].u, c->h64.lv[0].pitch, u);
+}
+
+int vlp2d_mg_download(vlp2d_mg_handle c, double *u) {
+  if (!c) return VLP2D_E_ARG;
+  if (!u) return fail(c, VLP2D_E_ARG, "u is NULL");
+  HIPCHK(c, hipSetDevice(c->dev));
+  return download_grid(c, u, c->h64.lv[0].u, c->h64.lv[0].pitch);
+}
+
+int vlp2d_mg_cycle(vlp2d_mg_handle c, double *stop_metric) {
+  if (!c) return VLP2D_E_ARG;
+  HIPCHK(c, hipSetDevice(c->dev));
+  Lvl<double> &P = c->h64.lv[0];
+  if (c->o.precision == VLP2D_PREC_MIXED) {
+    RCHK(ensure_h32(c));
+    if (!c->r32_valid) {
+      resnorm(c, P, 1, c->h32.lv[0].rhs);
+      HIPCHK(c, hipGetLastError());
+      c->r32_valid = true;
+    }
+    RCHK(run_program(c, 2, [&] { mixed_step(c, P); }));
+  } else {
+    RCHK(complete_hier(c, c->h64));
+    RCHK(ensure_aux(c));
+    RCHK(run_program(c, 0, [&] {
+      vcycle(c, c->h64, c->snap);
+      maxdiff(c, P, c->snap, 0);
+    }));
+  }
+  RCHK(sync_read(c, 2));
+  if (stop_metric) *stop_metric = c->hscal[0];
+  if (std::isnan(c->hscal[0])) return fail(c, VLP2D_E_NONFINITE, "iterate became non-finite");
+  return VLP2D_OK;
+}
+
+int vlp2d_mg_residual_norm(vlp2d_mg_handle c, double *norm) {
+  if (!c || !norm) return VLP2D_E_ARG;
+  HIPCHK(c, hipSetDevice(c->dev));
+  resnorm(c, c->h64.lv[0], 1, nullptr);
+  HIPCHK(c, hipGetLastError());
+  RCHK(sync_read(c, 2));
+  *norm = std::sqrt(c->hscal[1]);
+  return VLP2D_OK;
+}
+
+int vlp2d_mg_solve(vlp2d_mg_handle c, double rtol, int max_cycles, vlp2d_mg_stats *st) {
+  if (!c) return VLP2D_E_ARG;
+  if (!(rtol >= 0) || max_cycles < 0) return fail(c, VLP2D_E_ARG, "rtol must be >= 0 and max_cycles >= 0");
+  HIPCHK(c, hipSetDevice(c->dev));
+  return solve_impl(c, c->h64, 1, 2, rtol, max_cycles, st);
+}
+
+int vlp2d_mg_adjoint_exact(vlp2d_mg_handle c, const double *dL_du, double rtol, int max_cycles, double *dL_df,
+                           double *dL_dg) {
+  if (!c) return VLP2D_E_ARG;
+  if (!dL_du) return fail(c, VLP2D_E_ARG, "dL_du is NULL");
+  if (!(rtol >= 0) || max_cycles < 0) return fail(c, VLP2D_E_ARG, "rtol must be >= 0 and max_cycles >= 0");
+  if (!check_finite(dL_du, (size_t)c->nx * c->ny)) return fail(c, VLP2D_E_NONFINITE, "dL_du contains NaN or Inf");
+  HIPCHK(c, hipSetDevice(c->dev));
+  // Adjoint system A lambda = dL_du, A = -Lap_h on the interior with zero
+  // boundary: in the scheme's form Lap_h lambda = fn with fn = -dL_du.
+  if (c->ha.lv.empty()) {
+    shape_hier(c, c->ha, c->hx, c->hy);
+    Lvl<double> &a = c->ha.lv[0];
+    a.lap = false;
+    RCHK(dalloc(c, &a.u, a.elems()));
+    RCHK(dalloc(c, &a.rhs, a.elems()));
+  }
+  Lvl<double> &a = c->ha.lv[0];
+  fill(c, a.u, a.elems(), 0.0);
+  fill(c, a.rhs, a.elems(), 0.0);
+  HIPCHK(c, hipGetLastError());
+  // stage dL_du in a.u, negate into a.rhs, then clear a.u
+  RCHK(upload_grid(c, a.u, a.pitch, dL_du));
+  launch(c, KC_COPY, 0, 16.0 * nodes(a), [&] { hipLaunchKernelGGL(k_neg_interior, grid2(a.ny - 2, a.nx - 2), kBlock, 0, c->st, a.rhs, a.u, a.nx, a.ny, a.pitch); });
+  fill(c, a.u, a.elems(), 0.0);
+  HIPCHK(c, hipGetLastError());
+  c->r32_valid = false;
+  vlp2d_mg_stats st{};
+  int rc = solve_impl(c, c->ha, 3, 4, rtol, max_cycles, &st);
+  c->r32_valid = false;
+  if (rc != VLP2D_OK && rc != VLP2D_E_NOTCONV) return rc;
+  const size_t n = (size_t)c->nx * c->ny;
+  if (dL_df) {
+    RCHK(download_grid(c, dL_df, a.u, a.pitch));
+    for (int i = 0; i < c->nx; ++i)
+      for (int j = 0; j < c->ny; ++j)
+        if (i == 0 || j == 0 || i == c->nx - 1 || j == c->ny - 1) dL_df[(size_t)i * c->ny + j] = 0.0;
+  }
+  if (dL_dg) {
+    // reuse a.rhs as scratch for B^T lambda
+    Lvl<double> &v = c->h64.lv[0];
+    launch(c, KC_ADJ, 0, 16.0 * nodes(a), [&] { hipLaunchKernelGGL(k_adj_bT, grid2(a.ny, a.nx), kBlock, 0, c->st, a.rhs, a.u, a.nx, a.ny, a.pitch, v.ix2, v.iy2); });
+    HIPCHK(c, hipGetLastError());
+    RCHK(download_grid(c, dL_dg, a.rhs, a.pitch));
+  }
+  (void)n;
+  if (c->profile) collect_profile(c);
+  return rc;
+}
+
+int vlp2d_mg_forward_cycles(vlp2d_mg_handle c, int K) {
+  if (!c) return VLP2D_E_ARG;
+  if (K < 1 || K > 4096) return fail(c, VLP2D_E_ARG, "K must be in [1, 4096]");
+  HIPCHK(c, hipSetDevice(c->dev));
+  RCHK(complete_hier(c, c->h64));
+  Lvl<double> &P = c->h64.lv[0];
+  while ((int)c->ckpt.size() < K) {
+    double *p = nullptr;
+    RCHK(dalloc(c, &p, P.elems()));
+    c->ckpt.push_back(p);
+  }
+  for (int k = 0; k < K; ++k) {
+    if (c->cancel.load()) return fail(c, VLP2D_E_CANCELLED, "cancelled");
+    HIPCHK(c, hipMemcpyAsync(c->ckpt[k], P.u, P.elems() * sizeof(double), hipMemcpyDeviceToDevice, c->st));
+    vcycle(c, c->h64, nullptr);
+    HIPCHK(c, hipGetLastError());
+  }
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  if (c->profile) collect_profile(c);
+  c->K = K;
+  c->r32_valid = false;
+  return VLP2D_OK;
+}
+
+int vlp2d_mg_adjoint_cycles(vlp2d_mg_handle c, const double *dL_du, double *dL_df, double *dL_dg, double *dL_du0) {
+  if (!c) return VLP2D_E_ARG;
+  if (!dL_du) return fail(c, VLP2D_E_ARG, "dL_du is NULL");
+  if (c->K < 1) return fail(c, VLP2D_E_STATE, "vlp2d_mg_forward_cycles must run first");
+  if (!check_finite(dL_du, (size_t)c->nx * c->ny)) return fail(c, VLP2D_E_NONFINITE, "dL_du contains NaN or Inf");
+  HIPCHK(c, hipSetDevice(c->dev));
+  RCHK(ensure_adj_buffers(c));
+  Lvl<double> &P = c->h64.lv[0];
+  RCHK(upload_grid(c, c->aub[0], P.pitch, dL_du));
+  fill(c, c->arb[0], P.elems(), 0.0);
+  // The V-cycle is affine in (u, fn, g): its transpose needs none of the
+  // forward states, so the reverse sweep applies the transposed cycle K times
+  // without reading the checkpoints back.
+  for (int k = c->K - 1; k >= 0; --k) {
+    if (c->cancel.load()) return fail(c, VLP2D_E_CANCELLED, "cancelled");
+    cycle_T(c, 0);
+    HIPCHK(c, hipGetLastError());
+  }
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  if (c->profile) collect_profile(c);
+  const int nx = c->nx, ny = c->ny;
+  std::vector<double> ub((size_t)nx * ny), rb((size_t)nx * ny);
+  RCHK(download_grid(c, ub.data(), c->aub[0], P.pitch));
+  RCHK(download_grid(c, rb.data(), c->arb[0], P.pitch));
+  for (int i = 0; i < nx; ++i)
+    for (int j = 0; j < ny; ++j) {
+      size_t q = (size_t)i * ny + j;
+      bool inner = i > 0 && i < nx - 1 && j > 0 && j < ny - 1;
+      if (dL_df) dL_df[q] = inner ? -rb[q] : 0.0;
+      if (dL_dg) dL_dg[q] = inner ? 0.0 : ub[q];
+      if (dL_du0) dL_du0[q] = inner ? ub[q] : 0.0;
+    }
+  return VLP2D_OK;
+}
+
+int vlp2d_mg_get_info(vlp2d_mg_handle c, vlp2d_mg_info *info) {
+  if (!c || !info) return VLP2D_E_ARG;
+  info->levels = c->L;
+  info->nx = c->nx;
+  info->ny = c->ny;
+  info->device_bytes = c->bytes;
+  info->device_peak_bytes = c->peak;
+  info->checkpoints = c->K;
+  return VLP2D_OK;
+}
+
+int vlp2d_mg_kernel_times(vlp2d_mg_handle c, const char **names, int *levels, long long *launches, double *total_ms,
+                          double *alg_bytes, int cap) {
+  if (!c) return VLP2D_E_ARG;
+  int n = 0;
+  for (int k = 0; k < KC_COUNT; ++k)
+    for (int l = 0; l < kMaxLv; ++l) {
+      if (c->kc_launch[k][l] == 0) continue;
+      if (n < cap) {
+        if (names) names[n] = kKClassName[k];
+        if (levels) levels[n] = l;
+        if (launches) launches[n] = c->kc_launch[k][l];
+        if (total_ms) total_ms[n] = c->kc_ms[k][l];
+        if (alg_bytes) alg_bytes[n] = c->kc_bytes[k][l];
+      }
+      ++n;
+    }
+  return n;
+}
+
+int vlp2d_mg_reset_kernel_times(vlp2d_mg_handle c) {
+  if (!c) return VLP2D_E_ARG;
+  for (int k = 0; k < KC_COUNT; ++k)
+    for (int l = 0; l < kMaxLv; ++l) {
+      c->kc_launch[k][l] = 0;
+      c->kc_ms[k][l] = 0;
+      c->kc_bytes[k][l] = 0;
+    }
+  return VLP2D_OK;
+}
+
+int vlp2d_mg_save_iterate(vlp2d_mg_handle c) {
+  if (!c) return VLP2D_E_ARG;
+  HIPCHK(c, hipSetDevice(c->dev));
+  Lvl<double> &P = c->h64.lv[0];
+  if (!c->saved) RCHK(dalloc(c, &c->saved, P.elems()));
+  HIPCHK(c, hipMemcpyAsync(c->saved, P.u, P.elems() * sizeof(double), hipMemcpyDeviceToDevice, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  return VLP2D_OK;
+}
+
+int vlp2d_mg_restore_iterate(vlp2d_mg_handle c) {
+  if (!c) return VLP2D_E_ARG;
+  if (!c->saved) return fail(c, VLP2D_E_STATE, "vlp2d_mg_save_iterate must run first");
+  HIPCHK(c, hipSetDevice(c->dev));
+  Lvl<double> &P = c->h64.lv[0];
+  HIPCHK(c, hipMemcpyAsync(P.u, c->saved, P.elems() * sizeof(double), hipMemcpyDeviceToDevice, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  c->r32_valid = false;
+  c->K = 0;
+  c->cancel.store(0);
+  return VLP2D_OK;
+}
+
+void vlp2d_mg_cancel(vlp2d_mg_handle c) {
+  if (c) c->cancel.store(1);
+}
+
+void vlp2d_mg_destroy(vlp2d_mg_handle c) {
+  if (!c) return;
+  (void)hipSetDevice(c->dev);
+  if (c->st) (void)hipStreamSynchronize(c->st);
+  collect_profile(c);
+  invalidate_graphs(c);
+  free_hier(c, c->h64);
+  free_hier(c, c->h32);
+  free_hier(c, c->ha);
+  if (!c->h64.lv.empty()) {
+  }
+  size_t n0 = (size_t)c->nx * round_pitch(c->ny);
+  dfree(c, c->snap, n0);
+  dfree(c, c->saved, n0);
+  for (size_t l = 0; l < c->aub.size(); ++l) {
+    size_t n = (size_t)((c->segX >> l) + 1) * round_pitch((c->segY >> l) + 1);
+    dfree(c, c->aub[l], n);
+    dfree(c, c->arb[l], n);
+    dfree(c, c->aresb[l], n);
+  }
+  for (auto &p : c->ckpt) dfree(c, p, n0);
+  dfree(c, c->part, (size_t)c->part_cap);
+  dfree(c, c->dscal, 8);
+  if (c->hscal) (void)hipHostFree(c->hscal);
+  if (c->st) (void)hipStreamDestroy(c->st);
+  delete c;
+}
+
+const char *vlp2d_mg_last_error(vlp2d_mg_handle c) {
+  if (!c) return g_create_error.c_str();
+  return c->err.c_str();
+}
+
+}  // extern "C"
