@@ -90,7 +90,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--seg", type=int, default=8192)
     ap.add_argument("--rtol", type=float, default=1e-10)
-    ap.add_argument("--cpu-seg", type=int, default=2048)
+    ap.add_argument("--cpu-seg", type=int, default=8192)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
     args = ap.parse_args()

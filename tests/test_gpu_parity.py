@@ -66,6 +66,56 @@ def test_parity_mode_cycles_bitwise(seg_x, seg_y, hx, hy, laplace):
     s.cleanup()
 
 
+KERNEL_PATHS = {
+    "default": {},
+    "fused_all_levels": {"coarse_nodes": -1},        # tiles down to the coarsest level
+    "one_workgroup_all": {"coarse_nodes": 1 << 30},  # whole cycle in k_coarse
+    "one_op_per_launch": {"flags": _lib.FLAG_SIMPLE},
+    "no_graph": {"flags": _lib.FLAG_NO_GRAPH},
+}
+
+
+@pytest.mark.parametrize("path", sorted(KERNEL_PATHS))
+@pytest.mark.parametrize("seg_x,seg_y,hx,hy,laplace", [(256, 256, 1 / 256, 1 / 256, True),
+                                                       (128, 256, np.e / 128, np.pi / 256, False),
+                                                       (512, 64, 1 / 512, 1 / 512, False)])
+def test_kernel_paths_bitwise(path, seg_x, seg_y, hx, hy, laplace):
+    """Every kernel path (fused LDS tiles, one-workgroup small levels, one
+    operator per launch, graph or direct launches) gives the oracle's iterate
+    bit for bit, in parity mode (V(3,3) + update norm) and in the mixed
+    V(2,2) solve."""
+    _say(f"paths {path} {seg_x}x{seg_y}")
+    f, g = _random(seg_x, seg_y, 17 + seg_x, laplace)
+    u0 = _mean_init(g)
+    o = Oracle(seg_x, seg_y, hx, hy, f)
+    o.set_u(u0)
+    s = MultiGridSchemeHIP(seg_x, seg_y, hx, hy, f, eps=1e-13, options=Options(**KERNEL_PATHS[path]))
+    s.un0[:] = u0
+    s.init_after_boundaries_and_initial_iteration_inited()
+    for k in range(3):
+        m_ref = o.vcycle(3, 3, 3)
+        s.do_iteration(k)
+        assert np.array_equal(s.get_array(), o.get_u()), (k, np.abs(s.get_array() - o.get_u()).max())
+        assert s.last_metric == m_ref
+    s.cleanup()
+    z = np.zeros_like(g) if laplace else f
+    ur, kr, _ = mixed_solve(seg_x, seg_y, hx, hy, z, g, 2, 2, 3, 1e-10, 40)
+    s = MultiGridSchemeHIP(seg_x, seg_y, hx, hy, z, options=Options.performance(True, **KERNEL_PATHS[path]))
+    s.un0[:] = g
+    s.init_after_boundaries_and_initial_iteration_inited()
+    st = s.solve(1e-10, 40)
+    assert st["cycles"] == kr
+    assert np.array_equal(s.get_array(), ur), np.abs(s.get_array() - ur).max()
+    # a second solve from the restored state replays the captured graphs
+    s.save_iterate()
+    s.restore_iterate()
+    s.un0[:] = g
+    s.init_after_boundaries_and_initial_iteration_inited()
+    st = s.solve(1e-10, 40)
+    assert st["cycles"] == kr and np.array_equal(s.get_array(), ur)
+    s.cleanup()
+
+
 def test_config1_reference_stop_rule_cycle_count():
     """Config 1 (257^2 Laplace, g = x^2 - y^2) through the reference loop:
     cycles until the update norm <= eps, same count as the oracle, and the

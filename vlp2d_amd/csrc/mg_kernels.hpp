@@ -343,3 +343,25 @@ __global__ void __launch_bounds__(BX *BY) k_adj_bT(double *__restrict__ gb, cons
 }
 
 }  // namespace vlp2d
+
+namespace vlp2d {
+// Out-of-place refinement update: uout = uin + (double) e on interior nodes
+// (boundary nodes of uout already hold the Dirichlet values); partial max |e|.
+__global__ void __launch_bounds__(BX *BY) k_update_oop(const double *__restrict__ uin, double *__restrict__ uout,
+                                                       const float *__restrict__ e, int nx, int ny, int pitch,
+                                                       double *__restrict__ part) {
+  __shared__ double sm[BX * BY / 64];
+  const int j = blockIdx.x * BX + threadIdx.x + 1;
+  double m = 0.0;
+  if (j <= ny - 2)
+    for (int i = blockIdx.y * BY + threadIdx.y + 1; i <= nx - 2; i += gridDim.y * BY) {
+      const int c = i * pitch + j;
+      const double d = (double)e[c];
+      uout[c] = uin[c] + d;
+      m = nanmax(m, fabs(d));
+    }
+  const int tid = threadIdx.y * BX + threadIdx.x;
+  m = block_reduce<true>(m, sm, tid, BX * BY);
+  if (tid == 0) part[blockIdx.y * gridDim.x + blockIdx.x] = m;
+}
+}  // namespace vlp2d
