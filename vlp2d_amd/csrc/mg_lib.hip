@@ -29,6 +29,7 @@
 #include "../../include/vlp2d_mg.h"
 #include "mg_kernels.hpp"
 #include "mg_tiles.hpp"
+#include "mg_coarse.hpp"
 
 using namespace vlp2d;
 
@@ -67,6 +68,8 @@ template <typename T> struct Lvl {
   int nx = 0, ny = 0, pitch = 0;
   T hx2 = 0, hy2 = 0, coef = 0;
   bool eq = true, lap = false;
+  T ihx2 = 0, ihy2 = 0;          // 1/hx^2, 1/hy^2 in T (exact when pw2)
+  bool pw2 = false;              // hx^2 and hy^2 are powers of two
   double ix2 = 0, iy2 = 0;       // 1/hx^2, 1/hy^2 in double (adjoint)
   double ax = 0, ay = 0, b = 0;  // smoother coefficients in double (adjoint)
   int par = 0, rpar = 0;         // swap parity of u/u2 and rhs/rhs2
@@ -285,7 +288,10 @@ template <typename T> void vcycle_simple(Ctx *c, Hier<T> &H, bool zero0, double 
 }
 
 // ---- fused kernels -------------------------------------------------------------
-template <typename T> LevelView<T> view(const Lvl<T> &v, const T *uin, T *uout) {
+// Operands of one fused launch; the tile grid depends on the kernel's halo H
+// (tiles of tile_rows<T>() x (128 - 2H) output nodes).  The launch grid is
+// 8 * chunk workgroups (see tile_of_block).
+template <typename T> LevelView<T> view(const Lvl<T> &v, const T *uin, T *uout, int H) {
   LevelView<T> a;
   a.uin = uin;
   a.uout = uout;
@@ -298,9 +304,12 @@ template <typename T> LevelView<T> view(const Lvl<T> &v, const T *uin, T *uout) 
   a.coef = v.coef;
   a.eq = v.eq ? 1 : 0;
   a.lap = v.lap ? 1 : 0;
-  a.tiles_i = v.tiles_i;
-  a.tiles_j = v.tiles_j;
-  a.chunk = v.chunk;
+  a.ihx2 = v.ihx2;
+  a.ihy2 = v.ihy2;
+  a.pw2 = v.pw2 ? 1 : 0;
+  a.tiles_i = (v.nx - 2 + tile_rows<T>() - 1) / tile_rows<T>();
+  a.tiles_j = (v.ny - 2 + tile_cols(H) - 1) / tile_cols(H);
+  a.chunk = (a.tiles_i * a.tiles_j + 7) / 8;
   return a;
 }
 
@@ -313,8 +322,8 @@ void launch_down_ns(Ctx *c, const LevelView<T> &a, bool zi, Lvl<T> &g, int grid)
 }
 
 template <typename T> void down(Ctx *c, Lvl<T> &f, Lvl<T> &g, int lvl, bool zi) {
-  LevelView<T> a = view(f, f.u, f.u2);
-  const int grid = 8 * f.chunk;
+  LevelView<T> a = view(f, f.u, f.u2, down_halo(std::min(c->o.nu1, 3)));
+  const int grid = 8 * a.chunk;
   const double s = sizeof(T);
   launch(c, KC_DOWN, lvl, s * ((zi ? 2.0 : 3.0) * nodes(f) + nodes(g)), [&] {
     switch (c->o.nu1) {
@@ -343,9 +352,10 @@ void launch_up_ns(Ctx *c, const LevelView<T> &a, int epi, const Lvl<T> &g, int g
   }
 }
 
-template <typename T> void up(Ctx *c, Lvl<T> &f, Lvl<T> &g, int lvl, int epi, const MixView &mx) {
-  LevelView<T> a = view(f, f.u2, f.u);
-  const int grid = 8 * f.chunk;
+// returns the launch grid (the number of partials an epilogue writes)
+template <typename T> int up(Ctx *c, Lvl<T> &f, Lvl<T> &g, int lvl, int epi, const MixView &mx) {
+  LevelView<T> a = view(f, f.u2, f.u, up_halo(std::min(c->o.nu2, 3), epi));
+  const int grid = 8 * a.chunk;
   const double s = sizeof(T);
   double bytes = s * (3.0 * nodes(f) + nodes(g));
   if (epi == EPI_MIXED) bytes = s * (2.0 * nodes(f) + nodes(g)) + 28.0 * nodes(f);
@@ -357,6 +367,7 @@ template <typename T> void up(Ctx *c, Lvl<T> &f, Lvl<T> &g, int lvl, int epi, co
       default: launch_up_ns<T, 3>(c, a, epi, g, grid, c->part, mx); break;
     }
   });
+  return grid;
 }
 
 template <typename T> void coarse(Ctx *c, Hier<T> &H, bool zero_first, double *snap) {
@@ -366,7 +377,8 @@ template <typename T> void coarse(Ctx *c, Hier<T> &H, bool zero_first, double *s
   double bytes = 0;
   for (int l = H.lc; l < L; ++l) {
     const Lvl<T> &v = H.lv[l];
-    a.lv[l] = CoarseLevel<T>{v.u, v.rhs, v.nx, v.ny, v.pitch, v.hx2, v.hy2, v.coef, v.eq ? 1 : 0, v.lap ? 1 : 0};
+    a.lv[l] = CoarseLevel<T>{v.u,    v.rhs,  v.nx,           v.ny,           v.pitch,       v.hx2, v.hy2,
+                             v.coef, v.ihx2, v.ihy2, v.eq ? 1 : 0, v.lap ? 1 : 0, v.pw2 ? 1 : 0};
     bytes += 3.0 * sizeof(T) * nodes(v);
   }
   a.l0 = H.lc;
@@ -381,12 +393,18 @@ template <typename T> void coarse(Ctx *c, Hier<T> &H, bool zero_first, double *s
 
 // V-cycle with fused kernels.  Levels [0, lc) run k_down / k_up (out of place:
 // u -> u2 on the way down, u2 -> u on the way up); [lc, L) run in k_coarse on
-// u.  The level-0 ascent performs epilogue `epi` (partials into c->part).
-template <typename T> void vcycle_fused(Ctx *c, Hier<T> &H, bool zero0, int epi, const MixView &mx, double *snap) {
+// u.  The level-0 ascent performs epilogue `epi` (partials into c->part);
+// returns the number of partials it writes (0 when lc == 0).
+template <typename T> int vcycle_fused(Ctx *c, Hier<T> &H, bool zero0, int epi, const MixView &mx, double *snap) {
   const int lc = H.lc;
   for (int l = 0; l < lc; ++l) down(c, H.lv[l], H.lv[l + 1], l, l > 0 || zero0);
   coarse(c, H, lc > 0 || zero0, lc == 0 ? snap : nullptr);
-  for (int l = lc - 1; l >= 0; --l) up(c, H.lv[l], H.lv[l + 1], l, l == 0 ? epi : EPI_NONE, mx);
+  int np = 0;
+  for (int l = lc - 1; l >= 0; --l) {
+    const int grid = up(c, H.lv[l], H.lv[l + 1], l, l == 0 ? epi : EPI_NONE, mx);
+    if (l == 0) np = grid;
+  }
+  return np;
 }
 
 // residual norm^2 of level 0 (FP64) -> dscal[slot]; optional FP32 copy
@@ -430,9 +448,9 @@ void cycle64(Ctx *c, Hier<double> &H, int epi) {
     if (epi == EPI_METRIC) maxdiff(c, P, c->snap, 0);
     return;
   }
-  vcycle_fused(c, H, false, epi, mx, nullptr);
-  if (epi == EPI_RESNORM) reduce_part(c, c->part, 8 * P.chunk, 1, false);
-  if (epi == EPI_METRIC) reduce_part(c, c->part, 8 * P.chunk, 0, true);
+  const int np = vcycle_fused(c, H, false, epi, mx, nullptr);
+  if (epi == EPI_RESNORM) reduce_part(c, c->part, np, 1, false);
+  if (epi == EPI_METRIC) reduce_part(c, c->part, np, 0, true);
 }
 
 // One mixed-precision refinement step on the FP64 problem level P, using h32:
@@ -462,10 +480,13 @@ void mixed_step(Ctx *c, Lvl<double> &P) {
   mx.r32out = z.rhs2;
   mx.hx2 = P.hx2;
   mx.hy2 = P.hy2;
+  mx.ihx2 = P.ihx2;
+  mx.ihy2 = P.ihy2;
+  mx.pw2 = P.pw2 ? 1 : 0;
   mx.part_max = c->part2;
-  vcycle_fused(c, Z, true, EPI_MIXED, mx, nullptr);
-  reduce_part(c, c->part2, 8 * z.chunk, 0, true);
-  reduce_part(c, c->part, 8 * z.chunk, 1, false);
+  const int np = vcycle_fused(c, Z, true, EPI_MIXED, mx, nullptr);
+  reduce_part(c, c->part2, np, 0, true);
+  reduce_part(c, c->part, np, 1, false);
 }
 
 void swap_mixed(Ctx *c, Lvl<double> &P) {
@@ -521,6 +542,10 @@ template <typename T> void level_params(Lvl<T> &v, T hx0, T hy0, int segX, int s
   v.coef = T(1) / (T(2) / v.hx2 + T(2) / v.hy2);
   T d = hx - hy;
   v.eq = (d < T(0) ? -d : d) < (T)1e-15;
+  int ex = 0, ey = 0;
+  v.pw2 = std::frexp((double)v.hx2, &ex) == 0.5 && std::frexp((double)v.hy2, &ey) == 0.5;
+  v.ihx2 = T(1) / v.hx2;
+  v.ihy2 = T(1) / v.hy2;
   v.ix2 = 1.0 / (double)v.hx2;
   v.iy2 = 1.0 / (double)v.hy2;
   if (v.eq) {
@@ -551,9 +576,6 @@ template <typename T> void shape_hier(Ctx *c, Hier<T> &H, T hx0, T hy0) {
     v.ny = sy + 1;
     v.pitch = round_pitch(v.ny);
     level_params(v, hx0, hy0, c->segX, c->segY, sx, sy);
-    v.tiles_i = (v.nx - 2 + TT - 1) / TT;
-    v.tiles_j = (v.ny - 2 + TT - 1) / TT;
-    v.chunk = (v.tiles_i * v.tiles_j + 7) / 8;
     if (!found && c->o.coarse_nodes >= 0 && (long long)v.nx * v.ny <= thr) {
       H.lc = l;
       found = true;
@@ -871,7 +893,11 @@ int vlp2d_mg_create(int segX, int segY, double hx, double hy, const double *f, d
   fill(c, v.u, v.elems(), 0.0);
   fill(c, v.u2, v.elems(), 0.0);
   fill(c, v.rhs, v.elems(), 0.0);
-  c->part_cap = std::max(1 << 16, 8 * v.chunk + 64);
+  {
+    // partials of the largest launch grid: FP64 tiles of 32 rows, >= 100 columns
+    long long tiles = (long long)((nx + 31) / 32) * ((ny + 99) / 100);
+    c->part_cap = (int)std::max<long long>(1 << 16, tiles + 64);
+  }
   if ((rc = dalloc(c, &c->part, (size_t)c->part_cap)) != VLP2D_OK) return bail(rc);
   if ((rc = dalloc(c, &c->part2, (size_t)c->part_cap)) != VLP2D_OK) return bail(rc);
   if ((rc = dalloc(c, &c->dscal, 8)) != VLP2D_OK) return bail(rc);

@@ -1,8 +1,9 @@
 // Fused, LDS-tiled multigrid kernels for gfx950.
 //
-// One workgroup (256 threads = 4 wavefronts) owns a 64 x 64 output tile of a
-// level and stages the tile plus a halo of H nodes into LDS.  Inside LDS it
-// runs several red-black Gauss-Seidel half-sweeps back to back (temporal
+// A workgroup of 256 threads (4 wavefronts) owns an output tile of TI rows by
+// TJ columns of one level and stages the tile plus a halo of H nodes into LDS
+// as LI = TI + 2H rows of 128 columns (TJ = 128 - 2H).  Inside LDS it runs
+// several red-black Gauss-Seidel half-sweeps back to back (temporal
 // blocking): half-sweep k updates only nodes at distance >= k from the staged
 // edge, so after S half-sweeps every node at distance >= S holds exactly the
 // value a global sweep-by-sweep schedule produces.  Within one colour the
@@ -10,41 +11,57 @@
 // reference's (VisualLaplacePoisson2D/Model/CPU/MultiGridSlidingIteration.cs:
 // 53-57, GridIterator.cs:184-188: red = (i+j) even first).
 //
-// LDS layout splits the colours: U[c][li*LJC + (lj>>1)] with c = (li+lj)&1.
-// The tile origin (gi0, gj0) always has gi0 + gj0 even, so local and global
-// colours agree.  A red node's four neighbours are black nodes at
-// (li+-1, m), (li, m-1+d), (li, m+d) with m = lj>>1 and d = lj&1, so the
-// 64 lanes of a wavefront read consecutive LDS words.
+// LDS layout splits the colours: colour c = (li+lj)&1 of staged node (li, lj)
+// lives at U[c*NC + li*64 + (lj>>1)], NC = LI*64.  Lane m of a wavefront owns
+// the column pair (2m, 2m+1); wavefront w handles the rows li = w + 4k.  All
+// rows of a wavefront have the parity of w, so row bounds and the column
+// parity of a colour are wavefront-uniform, and the four neighbours of a node
+// are the LDS words b -+ 64, b - 1 + d, b + d of the other colour (b = li*64 +
+// m, d the column parity): consecutive lanes read consecutive words.  The
+// tile origin (gi0, gj0) always has gi0 + gj0 even, so local and global
+// colours agree.
 //
 // Tiles read a halo that neighbouring tiles write, so every fused kernel is
 // out of place (uin -> uout); each level keeps two iterate buffers.
 //
 // Kernels:
-//   k_down  : [zero or staged iterate] -> S = 2*NS half-sweeps -> residual
-//             and half-weighting restriction of the coarse right-hand side
-//             owned by the tile (MultiGridScheme.cs:134-169) -> write tile.
-//   k_up    : staged fine iterate + additive bilinear prolongation of the
-//             coarse correction (MultiGridScheme.cs:171-186) -> S half-sweeps
-//             -> epilogue (none | residual-norm partials | update-norm
-//             partials | FP64 refinement update of a mixed-precision step).
-//   k_coarse: one workgroup runs the V-cycle on the small levels entirely.
+//   k_down : [zero or staged iterate] -> S = 2*NS half-sweeps -> residual and
+//            half-weighting restriction of the coarse right-hand side owned by
+//            the tile (MultiGridScheme.cs:134-169) -> write the tile.
+//   k_up   : staged fine iterate + additive bilinear prolongation of the
+//            coarse correction (MultiGridScheme.cs:171-186) -> S half-sweeps
+//            -> epilogue (none | residual-norm partials | update-norm partials
+//            | FP64 refinement update of a mixed-precision step).
 #pragma once
 #include "mg_kernels.hpp"
 
 namespace vlp2d {
 
-constexpr int TT = 64;        // output tile edge
-constexpr int TNT = 256;      // threads per tile workgroup
+constexpr int TNT = 256;  // threads per tile workgroup (4 wavefronts)
+constexpr int SJ = 128;   // staged columns per tile
+constexpr int SJC = 64;   // staged columns per colour = lanes per wavefront
+
+// Output tile rows: FP64 tiles are half as tall so a staged FP64 tile stays
+// near 45 KB of LDS (three workgroups per CU); FP32 tiles are 64 rows.
+template <typename T> __host__ __device__ constexpr int tile_rows() { return sizeof(T) == 8 ? 32 : 64; }
+__host__ __device__ constexpr int tile_cols(int H) { return SJ - 2 * H; }
+__host__ __device__ constexpr int down_halo(int ns) { return 2 * ns + 2; }
+__host__ __device__ constexpr int up_halo(int ns, int epi) { return 2 * ns + ((epi == 1 || epi == 3) ? 1 : 0); }
 
 template <typename T> struct LevelView {
-  const T *uin;   // staged iterate (ignored when zero-initialised)
-  T *uout;        // written iterate (tile interior nodes of the domain)
-  const T *f;     // right-hand side (fn on level 0, restricted residual below)
+  const T *uin;  // staged iterate (ignored when zero-initialised)
+  T *uout;       // written iterate (tile interior nodes of the domain)
+  const T *f;    // right-hand side (fn on level 0, restricted residual below)
   int nx, ny, pitch;
   T hx2, hy2, coef;
-  int eq, lap;
-  int tiles_i, tiles_j, chunk;  // tile grid and per-XCD chunk of tiles
+  T ihx2, ihy2;  // 1/hx2, 1/hy2: exact when pw2 (both squares are powers of two)
+  int eq, lap, pw2;
+  int tiles_i, tiles_j, chunk;  // tile grid of this launch and per-XCD chunk
 };
+
+// v / h2.  When h2 is a power of two, v * (1/h2) is the same correctly
+// rounded value, and a multiply is far cheaper than an IEEE division.
+template <typename T> __device__ inline T divh(T v, T h2, T ih2, int pw2) { return pw2 ? v * ih2 : v / h2; }
 
 // Mixed-precision epilogue operands (level 0 of the FP32 correction).
 struct MixView {
@@ -52,7 +69,8 @@ struct MixView {
   double *u64out;
   const double *fn64;
   float *r32out;
-  double hx2, hy2;
+  double hx2, hy2, ihx2, ihy2;
+  int pw2;
   double *part_max;  // partial max |e|
 };
 
@@ -65,158 +83,30 @@ __device__ inline int tile_of_block(int tiles, int chunk) {
   return t < tiles ? t : -1;
 }
 
+template <typename T, int H> struct Geo {
+  static constexpr int TI = tile_rows<T>();
+  static constexpr int TJ = tile_cols(H);
+  static constexpr int LI = TI + 2 * H;
+  static constexpr int NC = LI * SJC;  // LDS words per colour
+  static constexpr int RPW = (LI + 3) / 4;  // staged rows per wavefront
+};
+
 template <typename T>
-__device__ inline T tile_point(const T *o, int li, int lj, int LJC, T f, const LevelView<T> &a) {
-  const int m = lj >> 1;
-  const T uim = o[(li - 1) * LJC + m];
-  const T uip = o[(li + 1) * LJC + m];
-  const T ujm = o[li * LJC + ((lj - 1) >> 1)];
-  const T ujp = o[li * LJC + ((lj + 1) >> 1)];
+__device__ inline T gs_point(T uim, T uip, T ujm, T ujp, T f, const LevelView<T> &a) {
   if (a.eq) {
     if (a.lap) return (uim + uip + ujm + ujp) * T(0.25);
     return (uim + uip + ujm + ujp - f * a.hx2) * T(0.25);
   }
-  if (a.lap) return a.coef * ((uim + uip) / a.hx2 + (ujm + ujp) / a.hy2);
-  return a.coef * ((uim + uip) / a.hx2 + (ujm + ujp) / a.hy2 - f);
+  if (a.lap) return a.coef * (divh(uim + uip, a.hx2, a.ihx2, a.pw2) + divh(ujm + ujp, a.hy2, a.ihy2, a.pw2));
+  return a.coef * (divh(uim + uip, a.hx2, a.ihx2, a.pw2) + divh(ujm + ujp, a.hy2, a.ihy2, a.pw2) - f);
 }
 
-// One half-sweep of colour C over nodes at distance >= lo from the staged
-// edge.  fr[k] holds f for this thread's k-th colour-C item.
-template <typename T, int LI, int LJ, int KPT, int C>
-__device__ inline void tile_half(T (*U)[LI * (LJ / 2)], const T (&fr)[KPT], int lo, int gi0, int gj0,
-                                 const LevelView<T> &a) {
-  constexpr int LJC = LJ / 2, NC = LI * LJC;
-  const T *o = U[1 - C];
-  T *w = U[C];
-#pragma unroll
-  for (int k = 0; k < KPT; ++k) {
-    const int p = threadIdx.x + TNT * k;
-    if (p < NC) {
-      const int li = p / LJC, m = p - (p / LJC) * LJC;
-      const int lj = 2 * m + ((li + C) & 1);
-      const int gi = gi0 + li, gj = gj0 + lj;
-      if (li >= lo && li < LI - lo && lj >= lo && lj < LJ - lo && gi >= 1 && gi <= a.nx - 2 && gj >= 1 &&
-          gj <= a.ny - 2)
-        w[li * LJC + m] = tile_point(o, li, lj, LJC, fr[k], a);
-    }
-  }
-  __syncthreads();
+__device__ inline bool inside(int gi, int gj, int nx, int ny) {
+  return gi >= 1 && gi <= nx - 2 && gj >= 1 && gj <= ny - 2;
 }
 
-// S (even) red-black half-sweeps over a staged (LI x LJ) tile with origin
-// (gi0, gj0), red first.
-template <typename T, int LI, int LJ, int S, int KPT>
-__device__ inline void tile_sweeps(T (*U)[LI * (LJ / 2)], const T (&fr)[2][KPT], int gi0, int gj0,
-                                   const LevelView<T> &a) {
-#pragma unroll 1
-  for (int s = 0; s < S; s += 2) {
-    tile_half<T, LI, LJ, KPT, 0>(U, fr[0], s + 1, gi0, gj0, a);
-    tile_half<T, LI, LJ, KPT, 1>(U, fr[1], s + 2, gi0, gj0, a);
-  }
-}
-
-// residual fn - Lap(u) at local node (li, lj) from the staged tile
-template <typename T, int LI, int LJ>
-__device__ inline T tile_resid(T (*U)[LI * (LJ / 2)], int li, int lj, T rhs, T hx2, T hy2) {
-  constexpr int LJC = LJ / 2;
-  const int c = (li + lj) & 1;
-  const T *o = U[1 - c];
-  const int m = lj >> 1;
-  const T u = U[c][li * LJC + m];
-  const T u2 = u * T(2);
-  const T uim = o[(li - 1) * LJC + m], uip = o[(li + 1) * LJC + m];
-  const T ujm = o[li * LJC + ((lj - 1) >> 1)], ujp = o[li * LJC + ((lj + 1) >> 1)];
-  return rhs - ((uim + uip - u2) / hx2 + (ujm + ujp - u2) / hy2);
-}
-
-// Stage the (LI x LJ) region of uin (0 outside the domain or when ZI).
-template <typename T, int LI, int LJ, bool ZI>
-__device__ inline void tile_load(T (*U)[LI * (LJ / 2)], const LevelView<T> &a, int gi0, int gj0) {
-  constexpr int LJC = LJ / 2;
-  for (int p = threadIdx.x; p < LI * LJ; p += TNT) {
-    const int li = p / LJ, lj = p - (p / LJ) * LJ;
-    const int gi = gi0 + li, gj = gj0 + lj;
-    T v = T(0);
-    if (!ZI && gi >= 0 && gi < a.nx && gj >= 0 && gj < a.ny) v = a.uin[gi * a.pitch + gj];
-    U[(li + lj) & 1][li * LJC + (lj >> 1)] = v;
-  }
-}
-
-template <typename T, int LI, int LJ, int KPT>
-__device__ inline void tile_load_f(T (&fr)[2][KPT], const LevelView<T> &a, int gi0, int gj0) {
-  constexpr int LJC = LJ / 2, NC = LI * LJC;
-#pragma unroll
-  for (int c = 0; c < 2; ++c)
-#pragma unroll
-    for (int k = 0; k < KPT; ++k) {
-      const int p = threadIdx.x + TNT * k;
-      T v = T(0);
-      if (p < NC && !a.lap) {
-        const int li = p / LJC, m = p - (p / LJC) * LJC;
-        const int lj = 2 * m + ((li + c) & 1);
-        const int gi = gi0 + li, gj = gj0 + lj;
-        if (gi >= 1 && gi <= a.nx - 2 && gj >= 1 && gj <= a.ny - 2) v = a.f[gi * a.pitch + gj];
-      }
-      fr[c][k] = v;
-    }
-}
-
-template <typename T, int LI, int LJ, int H>
-__device__ inline void tile_store(T (*U)[LI * (LJ / 2)], const LevelView<T> &a, int gi0, int gj0) {
-  constexpr int LJC = LJ / 2;
-  for (int p = threadIdx.x; p < TT * TT; p += TNT) {
-    const int li = H + p / TT, lj = H + (p & (TT - 1));
-    const int gi = gi0 + li, gj = gj0 + lj;
-    if (gi >= 1 && gi <= a.nx - 2 && gj >= 1 && gj <= a.ny - 2)
-      a.uout[gi * a.pitch + gj] = U[(li + lj) & 1][li * LJC + (lj >> 1)];
-  }
-}
-
-// ---------------------------------------------------------------------------
-// Descent: smoothing + residual + restriction.  Coarse interior node (I, J)
-// belongs to the tile containing its fine centre (2I, 2J).
-// ---------------------------------------------------------------------------
-template <typename T, int NS, bool ZI>
-__global__ void __launch_bounds__(TNT) k_down(LevelView<T> a, T *__restrict__ rhsc, int ncx, int ncy, int pc) {
-  constexpr int S = 2 * NS, H = S + 2, LI = TT + 2 * H, LJ = TT + 2 * H, LJC = LJ / 2, NC = LI * LJC;
-  constexpr int KPT = (NC + TNT - 1) / TNT;
-  __shared__ T U[2][NC];
-  const int t = tile_of_block(a.tiles_i * a.tiles_j, a.chunk);
-  if (t < 0) return;
-  const int ti = t / a.tiles_j, tj = t - (t / a.tiles_j) * a.tiles_j;
-  const int gi0 = 1 + ti * TT - H, gj0 = 1 + tj * TT - H;
-  tile_load<T, LI, LJ, ZI>(U, a, gi0, gj0);
-  T fr[2][KPT];
-  tile_load_f<T, LI, LJ, KPT>(fr, a, gi0, gj0);
-  __syncthreads();
-  tile_sweeps<T, LI, LJ, S, KPT>(U, fr, gi0, gj0, a);
-  // restriction of the residual to the coarse nodes owned by this tile
-  for (int q = threadIdx.x; q < (TT / 2) * (TT / 2); q += TNT) {
-    const int I = 1 + ti * (TT / 2) + q / (TT / 2);
-    const int J = 1 + tj * (TT / 2) + (q & (TT / 2 - 1));
-    if (I > ncx - 2 || J > ncy - 2) continue;
-    const int gi = 2 * I, gj = 2 * J;
-    const int li = gi - gi0, lj = gj - gj0;
-    const T *f = a.f;
-    const int c = gi * a.pitch + gj;
-    const T r0 = tile_resid<T, LI, LJ>(U, li, lj, f[c], a.hx2, a.hy2);
-    const T rip = tile_resid<T, LI, LJ>(U, li + 1, lj, f[c + a.pitch], a.hx2, a.hy2);
-    const T rim = tile_resid<T, LI, LJ>(U, li - 1, lj, f[c - a.pitch], a.hx2, a.hy2);
-    const T rjp = tile_resid<T, LI, LJ>(U, li, lj + 1, f[c + 1], a.hx2, a.hy2);
-    const T rjm = tile_resid<T, LI, LJ>(U, li, lj - 1, f[c - 1], a.hx2, a.hy2);
-    rhsc[I * pc + J] = T(0.125) * (r0 * T(4) + (rip + rim + rjp + rjm));
-  }
-  tile_store<T, LI, LJ, H>(U, a, gi0, gj0);
-}
-
-// ---------------------------------------------------------------------------
-// Ascent: prolongation-add + smoothing (+ epilogue).
-// EPI 0: write the tile.  EPI 1: also partial sums of the squared residual of
-// the result (FP64 stop test).  EPI 2: also partial max of |u - u_after_
-// prolongation| (the reference update norm, MultiGridScheme.cs:69,111).
-// EPI 3 (FP32, level 0 of a mixed step): u64out = u64in + e, r32out =
-// (float)(fn64 - Lap u64out), partial sums of r^2 and partial max |e|.
-// ---------------------------------------------------------------------------
+// correction of fine interior node (gi, gj) from the coarse grid
+// (MultiGridScheme.cs:180-183; sums in the reference's order)
 template <typename T>
 __device__ inline T prolong_corr(const T *uc, int pc, int gi, int gj) {
   const int I = gi >> 1, J = gj >> 1;
@@ -229,14 +119,179 @@ __device__ inline T prolong_corr(const T *uc, int pc, int gi, int gj) {
   return (r0[0] + r0[1] + r0[pc] + r0[pc + 1]) * T(0.25);
 }
 
+template <typename T, int H> __device__ inline T lds_at(const T *U, int li, int lj) {
+  return U[((li + lj) & 1) * Geo<T, H>::NC + li * SJC + (lj >> 1)];
+}
+
+// Stage rows li = w + 4k (columns 2m, 2m+1 per lane) of uin into U.  With
+// uc != nullptr the additive bilinear prolongation is applied to interior
+// nodes on the way in.  All loads are issued before the LDS stores.
+template <typename T, int H, bool ZI>
+__device__ inline void stage(T *U, const LevelView<T> &a, int gi0, int gj0, const T *uc, int pc) {
+  using G = Geo<T, H>;
+  const int w = threadIdx.x >> 6, m = threadIdx.x & 63;
+  const int gj = gj0 + 2 * m;
+  T v0[G::RPW], v1[G::RPW];
+#pragma unroll
+  for (int k = 0; k < G::RPW; ++k) {
+    const int li = w + 4 * k, gi = gi0 + li;
+    T x0 = T(0), x1 = T(0);
+    if (!ZI && li < G::LI && gi >= 0 && gi < a.nx) {
+      const T *row = a.uin + gi * a.pitch;
+      if (gj >= 0 && gj < a.ny) x0 = row[gj];
+      if (gj + 1 >= 0 && gj + 1 < a.ny) x1 = row[gj + 1];
+    }
+    v0[k] = x0;
+    v1[k] = x1;
+  }
+  if (uc) {
+#pragma unroll
+    for (int k = 0; k < G::RPW; ++k) {
+      const int li = w + 4 * k, gi = gi0 + li;
+      if (li < G::LI && gi >= 1 && gi <= a.nx - 2) {
+        if (gj >= 1 && gj <= a.ny - 2) v0[k] = v0[k] + prolong_corr(uc, pc, gi, gj);
+        if (gj + 1 >= 1 && gj + 1 <= a.ny - 2) v1[k] = v1[k] + prolong_corr(uc, pc, gi, gj + 1);
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < G::RPW; ++k) {
+    const int li = w + 4 * k;
+    if (li < G::LI) {
+      U[(li & 1) * G::NC + li * SJC + m] = v0[k];
+      U[((li + 1) & 1) * G::NC + li * SJC + m] = v1[k];
+    }
+  }
+}
+
+// f at this thread's colour-c nodes: fr[c][k] at (w + 4k, 2m + ((w + c) & 1)).
+template <typename T, int H, int RPW>
+__device__ inline void load_f(T (&fr)[2][RPW], const LevelView<T> &a, int gi0, int gj0) {
+  using G = Geo<T, H>;
+  const int w = threadIdx.x >> 6, m = threadIdx.x & 63;
+#pragma unroll
+  for (int c = 0; c < 2; ++c)
+#pragma unroll
+    for (int k = 0; k < RPW; ++k) {
+      const int li = w + 4 * k;
+      const int gi = gi0 + li, gj = gj0 + 2 * m + ((li + c) & 1);
+      T v = T(0);
+      if (!a.lap && li < G::LI && inside(gi, gj, a.nx, a.ny)) v = a.f[gi * a.pitch + gj];
+      fr[c][k] = v;
+    }
+}
+
+// One half-sweep of colour C over nodes at distance >= lo from the staged edge.
+template <typename T, int H, int C, int RPW>
+__device__ inline void half_sweep_tile(T *U, const T (&fr)[RPW], int lo, int gi0, int gj0, const LevelView<T> &a) {
+  using G = Geo<T, H>;
+  const int w = threadIdx.x >> 6, m = threadIdx.x & 63;
+  const int d = (w + C) & 1;  // column parity of colour C on this wavefront's rows
+  const int lj = 2 * m + d, gj = gj0 + lj;
+  const bool col_ok = lj >= lo && lj < SJ - lo && gj >= 1 && gj <= a.ny - 2;
+  const T *o = U + (1 - C) * G::NC;
+  T *wr = U + C * G::NC;
+#pragma unroll
+  for (int k = 0; k < RPW; ++k) {
+    const int li = w + 4 * k, gi = gi0 + li;
+    if (li >= lo && li < G::LI - lo && gi >= 1 && gi <= a.nx - 2 && col_ok) {
+      const int b = li * SJC + m;
+      wr[b] = gs_point(o[b - SJC], o[b + SJC], o[b - 1 + d], o[b + d], fr[k], a);
+    }
+  }
+  __syncthreads();
+}
+
+template <typename T, int H, int S, int RPW>
+__device__ inline void sweeps_tile(T *U, const T (&fr)[2][RPW], int gi0, int gj0, const LevelView<T> &a) {
+#pragma unroll 1
+  for (int s = 0; s < S; s += 2) {
+    half_sweep_tile<T, H, 0, RPW>(U, fr[0], s + 1, gi0, gj0, a);
+    half_sweep_tile<T, H, 1, RPW>(U, fr[1], s + 2, gi0, gj0, a);
+  }
+}
+
+// residual rhs - Lap(u) at staged node (li, lj) (UtilsOpLap.cs:15-20 order)
+template <typename T, int H>
+__device__ inline T resid_tile(const T *U, int li, int lj, T rhs, const LevelView<T> &a) {
+  const T u2 = lds_at<T, H>(U, li, lj) * T(2);
+  const T uim = lds_at<T, H>(U, li - 1, lj), uip = lds_at<T, H>(U, li + 1, lj);
+  const T ujm = lds_at<T, H>(U, li, lj - 1), ujp = lds_at<T, H>(U, li, lj + 1);
+  return rhs - (divh(uim + uip - u2, a.hx2, a.ihx2, a.pw2) + divh(ujm + ujp - u2, a.hy2, a.ihy2, a.pw2));
+}
+
+// write the output part of the staged tile
+template <typename T, int H>
+__device__ inline void store_tile(const T *U, const LevelView<T> &a, int gi0, int gj0) {
+  using G = Geo<T, H>;
+  const int w = threadIdx.x >> 6, m = threadIdx.x & 63;
+#pragma unroll
+  for (int k = 0; k < G::RPW; ++k) {
+    const int li = w + 4 * k, gi = gi0 + li;
+    if (li >= H && li < H + G::TI && gi >= 1 && gi <= a.nx - 2) {
+#pragma unroll
+      for (int e = 0; e < 2; ++e) {
+        const int lj = 2 * m + e, gj = gj0 + lj;
+        if (lj >= H && lj < H + G::TJ && gj >= 1 && gj <= a.ny - 2)
+          a.uout[gi * a.pitch + gj] = U[((li + e) & 1) * G::NC + li * SJC + m];
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Descent: smoothing + residual + restriction.  Coarse interior node (I, J)
+// belongs to the tile whose output part contains its fine centre (2I, 2J).
+// ---------------------------------------------------------------------------
+template <typename T, int NS, bool ZI>
+__global__ void __launch_bounds__(TNT) k_down(LevelView<T> a, T *__restrict__ rhsc, int ncx, int ncy, int pc) {
+  constexpr int S = 2 * NS, H = down_halo(NS);
+  using G = Geo<T, H>;
+  __shared__ T U[2 * G::NC];
+  const int t = tile_of_block(a.tiles_i * a.tiles_j, a.chunk);
+  if (t < 0) return;
+  const int ti = t / a.tiles_j, tj = t - (t / a.tiles_j) * a.tiles_j;
+  const int gi0 = 1 + ti * G::TI - H, gj0 = 1 + tj * G::TJ - H;
+  stage<T, H, ZI>(U, a, gi0, gj0, nullptr, 0);
+  T fr[2][G::RPW];
+  load_f<T, H, G::RPW>(fr, a, gi0, gj0);
+  __syncthreads();
+  sweeps_tile<T, H, S, G::RPW>(U, fr, gi0, gj0, a);
+  const int w = threadIdx.x >> 6, m = threadIdx.x & 63;
+  const int J = 1 + tj * (G::TJ / 2) + m;
+#pragma unroll
+  for (int k = 0; k < G::TI / 8; ++k) {
+    const int I = 1 + ti * (G::TI / 2) + w + 4 * k;
+    if (m < G::TJ / 2 && I <= ncx - 2 && J <= ncy - 2) {
+      const int gi = 2 * I, gj = 2 * J;
+      const int li = gi - gi0, lj = gj - gj0;
+      const T *f = a.f + gi * a.pitch + gj;
+      const T r0 = resid_tile<T, H>(U, li, lj, f[0], a);
+      const T rip = resid_tile<T, H>(U, li + 1, lj, f[a.pitch], a);
+      const T rim = resid_tile<T, H>(U, li - 1, lj, f[-a.pitch], a);
+      const T rjp = resid_tile<T, H>(U, li, lj + 1, f[1], a);
+      const T rjm = resid_tile<T, H>(U, li, lj - 1, f[-1], a);
+      rhsc[I * pc + J] = T(0.125) * (r0 * T(4) + (rip + rim + rjp + rjm));
+    }
+  }
+  store_tile<T, H>(U, a, gi0, gj0);
+}
+
+// ---------------------------------------------------------------------------
+// Ascent: prolongation-add + smoothing (+ epilogue).
+// EPI 0: write the tile.  EPI 1: also partial sums of the squared residual of
+// the result (FP64 stop test).  EPI 2: also partial max of |u - u_after_
+// prolongation| (the reference update norm, MultiGridScheme.cs:69,111).
+// EPI 3 (FP32, level 0 of a mixed step): u64out = u64in + e, r32out =
+// (float)(fn64 - Lap u64out), partial sums of r^2 and partial max |e|; the
+// correction e itself is not written.
+// ---------------------------------------------------------------------------
 template <typename T, int NS, int EPI>
 __global__ void __launch_bounds__(TNT) k_up(LevelView<T> a, const T *__restrict__ uc, int pc, double *__restrict__ part,
                                             MixView mx) {
-  constexpr int S = 2 * NS, H = S + ((EPI == 1 || EPI == 3) ? 1 : 0);
-  constexpr int LI = TT + 2 * H, LJ = TT + 2 * H, LJC = LJ / 2, NC = LI * LJC;
-  constexpr int KPT = (NC + TNT - 1) / TNT;
-  constexpr int OPT = TT * TT / TNT;  // output nodes per thread
-  __shared__ T U[2][NC];
+  constexpr int S = 2 * NS, H = up_halo(NS, EPI);
+  using G = Geo<T, H>;
+  __shared__ T U[2 * G::NC];
   __shared__ double red[TNT / 64];
   const int t = tile_of_block(a.tiles_i * a.tiles_j, a.chunk);
   if (t < 0) {
@@ -248,198 +303,72 @@ __global__ void __launch_bounds__(TNT) k_up(LevelView<T> a, const T *__restrict_
     return;
   }
   const int ti = t / a.tiles_j, tj = t - (t / a.tiles_j) * a.tiles_j;
-  const int gi0 = 1 + ti * TT - H, gj0 = 1 + tj * TT - H;
-  // stage and prolongate
-  for (int p = threadIdx.x; p < LI * LJ; p += TNT) {
-    const int li = p / LJ, lj = p - (p / LJ) * LJ;
-    const int gi = gi0 + li, gj = gj0 + lj;
-    T v = T(0);
-    if (gi >= 0 && gi < a.nx && gj >= 0 && gj < a.ny) {
-      v = a.uin[gi * a.pitch + gj];
-      if (gi >= 1 && gi <= a.nx - 2 && gj >= 1 && gj <= a.ny - 2) v = v + prolong_corr(uc, pc, gi, gj);
-    }
-    U[(li + lj) & 1][li * LJC + (lj >> 1)] = v;
-  }
-  T fr[2][KPT];
-  tile_load_f<T, LI, LJ, KPT>(fr, a, gi0, gj0);
+  const int gi0 = 1 + ti * G::TI - H, gj0 = 1 + tj * G::TJ - H;
+  const int tid = threadIdx.x, w = tid >> 6, m = tid & 63;
+  stage<T, H, false>(U, a, gi0, gj0, uc, pc);
+  T fr[2][G::RPW];
+  load_f<T, H, G::RPW>(fr, a, gi0, gj0);
   __syncthreads();
-  T snap[EPI == 2 ? OPT : 1];
+  constexpr int NSNAP = EPI == 2 ? G::RPW : 1;
+  T snap[NSNAP][2];
   if (EPI == 2) {
 #pragma unroll
-    for (int k = 0; k < OPT; ++k) {
-      const int p = threadIdx.x + TNT * k;
-      const int li = H + p / TT, lj = H + (p & (TT - 1));
-      snap[k] = U[(li + lj) & 1][li * LJC + (lj >> 1)];
-    }
-  }
-  tile_sweeps<T, LI, LJ, S, KPT>(U, fr, gi0, gj0, a);
-  const int tid = threadIdx.x;
-  if (EPI == 0 || EPI == 1 || EPI == 2) {
-    double acc = 0.0;
+    for (int k = 0; k < NSNAP; ++k) {
+      const int li = w + 4 * k;
 #pragma unroll
-    for (int k = 0; k < OPT; ++k) {
-      const int p = tid + TNT * k;
-      const int li = H + p / TT, lj = H + (p & (TT - 1));
-      const int gi = gi0 + li, gj = gj0 + lj;
-      if (gi >= 1 && gi <= a.nx - 2 && gj >= 1 && gj <= a.ny - 2) {
-        const T u = U[(li + lj) & 1][li * LJC + (lj >> 1)];
-        a.uout[gi * a.pitch + gj] = u;
-        if (EPI == 1) {
-          const double r = (double)tile_resid<T, LI, LJ>(U, li, lj, a.f[gi * a.pitch + gj], a.hx2, a.hy2);
-          acc += r * r;
-        } else if (EPI == 2) {
-          acc = nanmax(acc, fabs((double)(u - snap[k])));
-        }
-      }
+      for (int e = 0; e < 2; ++e)
+        snap[k][e] = (li >= H && li < H + G::TI) ? U[((li + e) & 1) * G::NC + li * SJC + m] : T(0);
     }
-    if (EPI == 1 || EPI == 2) {
-      acc = block_reduce<EPI == 2>(acc, red, tid, TNT);
-      if (tid == 0) part[blockIdx.x] = acc;
-    }
-    return;
   }
-  // EPI == 3: FP64 refinement update and FP64 residual of the new iterate.
-  constexpr int W = TT + 2;
-  __shared__ double Wn[W * W];
-  for (int p = tid; p < W * W; p += TNT) {
-    const int li = H - 1 + p / W, lj = H - 1 + (p - (p / W) * W);
-    const int gi = gi0 + li, gj = gj0 + lj;
-    double v = 0.0;
-    if (gi >= 0 && gi < a.nx && gj >= 0 && gj < a.ny) {
-      v = mx.u64in[gi * a.pitch + gj];
-      if (gi >= 1 && gi <= a.nx - 2 && gj >= 1 && gj <= a.ny - 2)
-        v += (double)U[(li + lj) & 1][li * LJC + (lj >> 1)];
-    }
-    Wn[p] = v;
-  }
-  __syncthreads();
-  double ss = 0.0, em = 0.0;
+  sweeps_tile<T, H, S, G::RPW>(U, fr, gi0, gj0, a);
+  double acc = 0.0, em = 0.0;
 #pragma unroll
-  for (int k = 0; k < OPT; ++k) {
-    const int p = tid + TNT * k;
-    const int oi = p / TT, oj = p & (TT - 1);
-    const int li = H + oi, lj = H + oj;
-    const int gi = gi0 + li, gj = gj0 + lj;
-    if (gi >= 1 && gi <= a.nx - 2 && gj >= 1 && gj <= a.ny - 2) {
-      const int w = (oi + 1) * W + (oj + 1);
-      const double u = Wn[w];
-      const double u2 = u * 2.0;
+  for (int k = 0; k < G::RPW; ++k) {
+    const int li = w + 4 * k, gi = gi0 + li;
+    if (!(li >= H && li < H + G::TI && gi >= 1 && gi <= a.nx - 2)) continue;
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+      const int lj = 2 * m + e, gj = gj0 + lj;
+      if (!(lj >= H && lj < H + G::TJ && gj >= 1 && gj <= a.ny - 2)) continue;
       const int g = gi * a.pitch + gj;
-      const double r = mx.fn64[g] - ((Wn[w - W] + Wn[w + W] - u2) / mx.hx2 + (Wn[w - 1] + Wn[w + 1] - u2) / mx.hy2);
-      mx.u64out[g] = u;
-      mx.r32out[g] = (float)r;
-      ss += r * r;
-      em = nanmax(em, fabs((double)U[(li + lj) & 1][li * LJC + (lj >> 1)]));
-    }
-  }
-  ss = block_reduce<false>(ss, red, tid, TNT);
-  if (tid == 0) part[blockIdx.x] = ss;
-  em = block_reduce<true>(em, red, tid, TNT);
-  if (tid == 0) mx.part_max[blockIdx.x] = em;
-}
-
-// ---------------------------------------------------------------------------
-// Small levels: one workgroup runs the V-cycle from level l0 down to the
-// coarsest and back, in global memory (the working set sits in L2), with a
-// workgroup barrier between dependent steps.  Level l0 is zero-initialised
-// unless it is level 0 of the problem (zero_first = 0).
-// ---------------------------------------------------------------------------
-constexpr int CNT = 1024;
-constexpr int kMaxLevels = 16;
-
-template <typename T> struct CoarseLevel {
-  T *u;
-  const T *f;
-  int nx, ny, pitch;
-  T hx2, hy2, coef;
-  int eq, lap;
-};
-template <typename T> struct CoarseArgs {
-  CoarseLevel<T> lv[kMaxLevels];
-  int l0, L;
-  int nu1, nu2, nuc;
-  int zero_first;
-  double *snap;  // level-0 snapshot before post-smoothing (l0 == 0), or null
-};
-
-template <typename T>
-__device__ inline void cg_zero(const CoarseLevel<T> &v) {
-  const int n = (v.nx - 2) * (v.ny - 2);
-  for (int p = threadIdx.x; p < n; p += CNT) {
-    const int i = 1 + p / (v.ny - 2), j = 1 + p - (p / (v.ny - 2)) * (v.ny - 2);
-    v.u[i * v.pitch + j] = T(0);
-  }
-  __syncthreads();
-}
-
-template <typename T>
-__device__ inline void cg_smooth(const CoarseLevel<T> &v, int n) {
-  const int wc = (v.ny - 1) / 2;  // items per row and colour (ny odd)
-  const int items = (v.nx - 2) * wc;
-  for (int s = 0; s < n; ++s)
-    for (int color = 0; color < 2; ++color) {
-      for (int p = threadIdx.x; p < items; p += CNT) {
-        const int i = 1 + p / wc, m = p - (p / wc) * wc;
-        const int j = 2 * m + 1 + ((i + color + 1) & 1);
-        if (j > v.ny - 2) continue;
-        const int c = i * v.pitch + j;
-        T *u = v.u;
-        T r;
-        if (v.eq) {
-          if (v.lap) r = (u[c - v.pitch] + u[c + v.pitch] + u[c - 1] + u[c + 1]) * T(0.25);
-          else r = (u[c - v.pitch] + u[c + v.pitch] + u[c - 1] + u[c + 1] - v.f[c] * v.hx2) * T(0.25);
-        } else {
-          if (v.lap) r = v.coef * ((u[c - v.pitch] + u[c + v.pitch]) / v.hx2 + (u[c - 1] + u[c + 1]) / v.hy2);
-          else r = v.coef * ((u[c - v.pitch] + u[c + v.pitch]) / v.hx2 + (u[c - 1] + u[c + 1]) / v.hy2 - v.f[c]);
-        }
-        u[c] = r;
+      const T u = U[((li + e) & 1) * G::NC + li * SJC + m];
+      if (EPI == 0 || EPI == 1 || EPI == 2) a.uout[g] = u;
+      if (EPI == 1) {
+        const double r = (double)resid_tile<T, H>(U, li, lj, a.f[g], a);
+        acc += r * r;
+      } else if (EPI == 2) {
+        acc = nanmax(acc, fabs((double)(u - snap[EPI == 2 ? k : 0][e])));
+      } else if (EPI == 3) {
+        // u_next = u64 + e at the node and its four neighbours (boundary
+        // neighbours keep their Dirichlet value), then the FP64 residual.
+        const double uc0 = mx.u64in[g] + (double)u;
+        double uim = mx.u64in[g - a.pitch], uip = mx.u64in[g + a.pitch];
+        double ujm = mx.u64in[g - 1], ujp = mx.u64in[g + 1];
+        if (gi - 1 >= 1) uim = uim + (double)lds_at<T, H>(U, li - 1, lj);
+        if (gi + 1 <= a.nx - 2) uip = uip + (double)lds_at<T, H>(U, li + 1, lj);
+        if (gj - 1 >= 1) ujm = ujm + (double)lds_at<T, H>(U, li, lj - 1);
+        if (gj + 1 <= a.ny - 2) ujp = ujp + (double)lds_at<T, H>(U, li, lj + 1);
+        const double u2 = uc0 * 2.0;
+        const double r =
+            mx.fn64[g] - (divh(uim + uip - u2, mx.hx2, mx.ihx2, mx.pw2) + divh(ujm + ujp - u2, mx.hy2, mx.ihy2, mx.pw2));
+        mx.u64out[g] = uc0;
+        mx.r32out[g] = (float)r;
+        acc += r * r;
+        em = nanmax(em, fabs((double)u));
       }
-      __syncthreads();
     }
-}
-
-template <typename T>
-__device__ inline T cg_resid(const CoarseLevel<T> &v, int c) {
-  const T *u = v.u;
-  const T u2 = u[c] * T(2);
-  return v.f[c] - ((u[c - v.pitch] + u[c + v.pitch] - u2) / v.hx2 + (u[c - 1] + u[c + 1] - u2) / v.hy2);
-}
-
-template <typename T>
-__global__ void __launch_bounds__(CNT) k_coarse(CoarseArgs<T> a) {
-  for (int l = a.l0; l <= a.L - 2; ++l) {
-    const CoarseLevel<T> &f = a.lv[l];
-    const CoarseLevel<T> &g = a.lv[l + 1];
-    if (l > a.l0 || a.zero_first) cg_zero(f);
-    cg_smooth(f, a.nu1);
-    const int n = (g.nx - 2) * (g.ny - 2);
-    for (int p = threadIdx.x; p < n; p += CNT) {
-      const int I = 1 + p / (g.ny - 2), J = 1 + p - (p / (g.ny - 2)) * (g.ny - 2);
-      const int c = 2 * I * f.pitch + 2 * J;
-      const T r0 = cg_resid(f, c), rip = cg_resid(f, c + f.pitch), rim = cg_resid(f, c - f.pitch);
-      const T rjp = cg_resid(f, c + 1), rjm = cg_resid(f, c - 1);
-      ((T *)g.f)[I * g.pitch + J] = T(0.125) * (r0 * T(4) + (rip + rim + rjp + rjm));
-    }
-    __syncthreads();
   }
-  {
-    const CoarseLevel<T> &v = a.lv[a.L - 1];
-    if (a.L - 1 > a.l0 || a.zero_first) cg_zero(v);
-    cg_smooth(v, a.nuc);
+  if (EPI == 1 || EPI == 3) {
+    acc = block_reduce<false>(acc, red, tid, TNT);
+    if (tid == 0) part[blockIdx.x] = acc;
   }
-  for (int l = a.L - 2; l >= a.l0; --l) {
-    const CoarseLevel<T> &f = a.lv[l];
-    const CoarseLevel<T> &g = a.lv[l + 1];
-    const int n = (f.nx - 2) * (f.ny - 2);
-    for (int p = threadIdx.x; p < n; p += CNT) {
-      const int i = 1 + p / (f.ny - 2), j = 1 + p - (p / (f.ny - 2)) * (f.ny - 2);
-      const int c = i * f.pitch + j;
-      const T v = f.u[c] + prolong_corr(g.u, g.pitch, i, j);
-      f.u[c] = v;
-      if (l == 0 && a.snap) a.snap[c] = (double)v;
-    }
-    __syncthreads();
-    cg_smooth(f, a.nu2);
+  if (EPI == 2) {
+    acc = block_reduce<true>(acc, red, tid, TNT);
+    if (tid == 0) part[blockIdx.x] = acc;
+  }
+  if (EPI == 3) {
+    em = block_reduce<true>(em, red, tid, TNT);
+    if (tid == 0) mx.part_max[blockIdx.x] = em;
   }
 }
 
