@@ -1,0 +1,158 @@
+"""GPU checks of the benchmark configurations that are not bench lines:
+the adjoint through an exact solve (1025^2), the adjoint through K truncated
+V-cycles with checkpoints at 16385^2 (peak HBM reported), and the DD128
+cross-check of the mixed-precision solver on a 513^2 replica."""
+import numpy as np
+import pytest
+
+from oracle import Oracle, dst_adjoint, dst_solve
+from vlp2d_amd import MultiGridSchemeHIP, Options, problems
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def _solve(seg, h, f, g, rtol=1e-13, mixed=False):
+    s = MultiGridSchemeHIP(seg, seg, h, h, f, options=Options.performance(mixed))
+    u0 = np.zeros((seg + 1, seg + 1))
+    u0[0, :], u0[-1, :], u0[:, 0], u0[:, -1] = g[0, :], g[-1, :], g[:, 0], g[:, -1]
+    s.un0[:] = u0
+    s.init_after_boundaries_and_initial_iteration_inited()
+    st = s.solve(rtol, 80)
+    u = s.get_array().copy()
+    s.cleanup()
+    return u, st
+
+
+def test_config4_exact_adjoint_finite_differences():
+    """Config 4: 1025^2, L = 0.5 ||u - u_obs||^2 over interior nodes with
+    u_obs = S(f_true) + N(0, 1e-3), evaluated at f = 0, g = 0.  The adjoint
+    gradient (one adjoint multigrid solve) matches central finite differences
+    of GPU solves along 64 seeded directions to 1e-6 relative, and the CPU
+    FP64 gradient (DST-I solve of A lambda = dL/du, then B^T lambda) to 1e-9."""
+    seg, h = 1024, 1.0 / 1024
+    f_true, rng = problems.sine_modes_unit(seed=4, seg=seg)
+    zero = np.zeros_like(f_true)
+    _say("config4: forward u_obs")
+    u_true, _ = _solve(seg, h, f_true, zero)
+    u_obs = u_true.copy()
+    u_obs[1:-1, 1:-1] += 1e-3 * rng.normal((seg - 1) ** 2).reshape(seg - 1, seg - 1)
+    f0, g0 = zero.copy(), zero.copy()
+    u0, _ = _solve(seg, h, f0, g0)
+    ubar = np.zeros_like(u0)
+    ubar[1:-1, 1:-1] = (u0 - u_obs)[1:-1, 1:-1]
+
+    def loss(f, g):
+        u, st = _solve(seg, h, f, g)
+        assert st["rel_residual"] <= 1e-13
+        return 0.5 * np.sum((u - u_obs)[1:-1, 1:-1] ** 2)
+
+    _say("config4: adjoint")
+    s = MultiGridSchemeHIP(seg, seg, h, h, f0, options=Options.performance())
+    fbar, gbar = s.adjoint_exact(ubar, 1e-13, 80)
+    s.cleanup()
+    lam, gex = dst_adjoint(ubar, h, h)
+    assert np.abs(fbar - lam).max() <= 1e-9 * np.abs(lam).max()
+    assert np.abs(gbar - gex).max() <= 1e-9 * np.abs(gex).max()
+    assert gbar[0, 0] == gbar[0, -1] == gbar[-1, 0] == gbar[-1, -1] == 0.0
+    dirs = problems.SplitMix64(404)
+    worst = 0.0
+    for k in range(64):
+        df = dirs.normal((seg + 1) ** 2).reshape(seg + 1, seg + 1)
+        dg = dirs.normal((seg + 1) ** 2).reshape(seg + 1, seg + 1)
+        df[0, :] = df[-1, :] = df[:, 0] = df[:, -1] = 0.0
+        dg[1:-1, 1:-1] = 0.0
+        eps = 1.0  # L is quadratic in (f, g): central differences are exact for any step
+        fd = (loss(f0 + eps * df, g0 + eps * dg) - loss(f0 - eps * df, g0 - eps * dg)) / (2 * eps)
+        an = np.sum(fbar * df) + np.sum(gbar * dg)
+        worst = max(worst, abs(fd - an) / abs(an))
+    _say(f"config4: worst relative FD error over 64 directions {worst:.2e}")
+    assert worst <= 1e-6
+
+
+@pytest.mark.timeout(600)
+def test_config5_truncated_adjoint_16385():
+    """Config 5: 16385^2, 256 Gaussian bumps, boundary N(0,1) smoothed by the
+    9-tap binomial filter.  (a) FP32 V(2,2) cycles in FP64 refinement reach
+    rel. residual 1e-10.  (b) The gradient of L = 0.5 ||u_K - u_obs||^2
+    through K = 6 FP64 V(2,2) cycles (per-cycle checkpoints) matches central
+    finite differences along seeded directions to 1e-6; peak device memory
+    of the handle is reported."""
+    p = problems.config(5)
+    seg, h = p.seg_x, p.hx
+    _say("config5: mixed solve")
+    u, st = _solve(seg, h, p.f, p.g, rtol=1e-10, mixed=True)
+    assert st["rel_residual"] <= 1e-10
+    _say(f"config5: mixed {st['cycles']} steps, {st['seconds'] * 1e3:.1f} ms, {st['munknowns_per_s']:.0f} Munknowns/s")
+    rng = problems.SplitMix64(55)
+    u_obs = u + 1e-3 * rng.normal(u.size).reshape(u.shape)
+    del u
+    K = 6
+    u0 = p.initial("zero")
+
+    def forward(f, g):
+        s = MultiGridSchemeHIP(seg, seg, h, h, f, options=Options.performance())
+        x = u0.copy()
+        x[0, :], x[-1, :], x[:, 0], x[:, -1] = g[0, :], g[-1, :], g[:, 0], g[:, -1]
+        s.un0[:] = x
+        s.init_after_boundaries_and_initial_iteration_inited()
+        s.forward_cycles(K)
+        return s
+
+    s = forward(p.f, p.g)
+    uK = s.get_array().copy()
+    ubar = np.zeros_like(uK)
+    ubar[1:-1, 1:-1] = (uK - u_obs)[1:-1, 1:-1]
+    fbar, gbar, _ = s.adjoint_cycles(ubar)
+    info = s.info()
+    s.cleanup()
+    _say(f"config5: peak device memory {info['device_peak_bytes'] / 2**30:.2f} GiB, "
+         f"{info['checkpoints']} checkpoints")
+    assert info["checkpoints"] == K
+
+    def loss(f, g):
+        s = forward(f, g)
+        v = s.get_array()
+        r = 0.5 * float(np.sum((v - u_obs)[1:-1, 1:-1] ** 2))
+        s.cleanup()
+        return r
+
+    dirs = problems.SplitMix64(505)
+    worst = 0.0
+    for k in range(2):
+        df = dirs.normal(p.f.size).reshape(p.f.shape)
+        dg = dirs.normal(p.f.size).reshape(p.f.shape)
+        df[0, :] = df[-1, :] = df[:, 0] = df[:, -1] = 0.0
+        dg[1:-1, 1:-1] = 0.0
+        fd = (loss(p.f + df, p.g + dg) - loss(p.f - df, p.g - dg)) / 2.0
+        an = float(np.sum(fbar * df) + np.sum(gbar * dg))
+        worst = max(worst, abs(fd - an) / abs(an))
+        _say(f"config5: direction {k}: fd {fd:.12e} adjoint {an:.12e}")
+    assert worst <= 1e-6
+
+
+def test_dd128_crosscheck_513():
+    """Mixed-precision GPU solution of the 513^2 replica of config 3 against
+    the oracle's double-double multigrid (the reference's DD128 CPU path),
+    converged to its own floor; FP64 GPU multigrid and the DST-I solve on the
+    same inputs sit at double-precision rounding from it."""
+    p = problems.config(3, 512)
+    o = Oracle(p.seg_x, p.seg_y, p.hx, p.hy, p.f, "dd")
+    o.set_u(p.initial("zero"))
+    for _ in range(60):
+        if o.vcycle(3, 3, 3) < 1e-26:
+            break
+    hi, lo = o.get_u(with_lo=True)
+    scale = np.abs(hi).max()
+    um, stm = _solve(p.seg_x, p.hx, p.f, p.g, rtol=1e-10, mixed=True)
+    ud, std = _solve(p.seg_x, p.hx, p.f, p.g, rtol=1e-13, mixed=False)
+    e_mixed = np.abs((um - hi) - lo).max() / scale
+    e_fp64 = np.abs((ud - hi) - lo).max() / scale
+    e_dst = np.abs((dst_solve(p.f, p.g, p.hx, p.hy) - hi) - lo).max() / scale
+    _say(f"dd128 513^2: mixed rel err {e_mixed:.2e} ({stm['cycles']} steps), fp64 {e_fp64:.2e}, dst {e_dst:.2e}")
+    assert e_mixed <= 1e-8
+    assert e_fp64 <= 1e-12
+    assert e_dst <= 1e-12

@@ -30,6 +30,11 @@
 #include "mg_kernels.hpp"
 #include "mg_tiles.hpp"
 #include "mg_coarse.hpp"
+#include "mg_refine.hpp"
+
+#ifndef VLP2D_REFINE_STREAM
+#define VLP2D_REFINE_STREAM 1
+#endif
 
 using namespace vlp2d;
 
@@ -484,7 +489,32 @@ void mixed_step(Ctx *c, Lvl<double> &P) {
   mx.ihy2 = P.ihy2;
   mx.pw2 = P.pw2 ? 1 : 0;
   mx.part_max = c->part2;
-  const int np = vcycle_fused(c, Z, true, EPI_MIXED, mx, nullptr);
+  if (c->o.flags & VLP2D_FLAG_FUSED_REFINE) {
+    // refinement update inside the level-0 ascent of the FP32 cycle
+    const int np = vcycle_fused(c, Z, true, EPI_MIXED, mx, nullptr);
+    reduce_part(c, c->part2, np, 0, true);
+    reduce_part(c, c->part, np, 1, false);
+    return;
+  }
+  vcycle_fused(c, Z, true, EPI_NONE, mx, nullptr);  // correction e -> z.u
+  int np = 0;
+  if (VLP2D_REFINE_STREAM) {
+    const dim3 g((P.ny - 2 + 63) / 64, (P.nx - 2 + 4 * RR - 1) / (4 * RR));
+    np = (int)(g.x * g.y);
+    launch(c, KC_UPDATE, 0, 32.0 * nodes(P), [&] {
+      hipLaunchKernelGGL(k_refine_stream, g, dim3(256), 0, c->st, P.u, P.u2, z.u, P.rhs, z.rhs2, P.nx, P.ny,
+                         P.pitch, (double)P.hx2, (double)P.hy2, (double)P.ihx2, (double)P.ihy2, P.pw2 ? 1 : 0,
+                         c->part, c->part2);
+    });
+  } else {
+    const int ti = (P.nx - 2 + RF_TI - 1) / RF_TI, tj = (P.ny - 2 + RF_TJ - 1) / RF_TJ;
+    np = ti * tj;
+    launch(c, KC_UPDATE, 0, 32.0 * nodes(P), [&] {
+      hipLaunchKernelGGL(k_refine, dim3(np), dim3(TNT), 0, c->st, P.u, P.u2, z.u, P.rhs, z.rhs2, P.nx, P.ny, P.pitch,
+                         (double)P.hx2, (double)P.hy2, (double)P.ihx2, (double)P.ihy2, P.pw2 ? 1 : 0, ti, c->part,
+                         c->part2);
+    });
+  }
   reduce_part(c, c->part2, np, 0, true);
   reduce_part(c, c->part, np, 1, false);
 }
@@ -894,9 +924,11 @@ int vlp2d_mg_create(int segX, int segY, double hx, double hy, const double *f, d
   fill(c, v.u2, v.elems(), 0.0);
   fill(c, v.rhs, v.elems(), 0.0);
   {
-    // partials of the largest launch grid: FP64 tiles of 32 rows, >= 100 columns
+    // partials of the largest launch grid: FP64 tiles of 32 rows, >= 100
+    // columns; refinement tiles of RF_TI x RF_TJ
     long long tiles = (long long)((nx + 31) / 32) * ((ny + 99) / 100);
-    c->part_cap = (int)std::max<long long>(1 << 16, tiles + 64);
+    long long rtiles = (long long)((nx + RF_TI - 1) / RF_TI) * ((ny + RF_TJ - 1) / RF_TJ);
+    c->part_cap = (int)std::max<long long>(1 << 16, std::max(tiles, rtiles) + 64);
   }
   if ((rc = dalloc(c, &c->part, (size_t)c->part_cap)) != VLP2D_OK) return bail(rc);
   if ((rc = dalloc(c, &c->part2, (size_t)c->part_cap)) != VLP2D_OK) return bail(rc);

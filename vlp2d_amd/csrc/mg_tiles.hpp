@@ -41,9 +41,19 @@ constexpr int TNT = 256;  // threads per tile workgroup (4 wavefronts)
 constexpr int SJ = 128;   // staged columns per tile
 constexpr int SJC = 64;   // staged columns per colour = lanes per wavefront
 
-// Output tile rows: FP64 tiles are half as tall so a staged FP64 tile stays
-// near 45 KB of LDS (three workgroups per CU); FP32 tiles are 64 rows.
-template <typename T> __host__ __device__ constexpr int tile_rows() { return sizeof(T) == 8 ? 32 : 64; }
+// Output tile rows, 32 for both types (measured on MI355X: 64-row tiles
+// lower occupancy more than they save in halo traffic).  A staged FP64 tile
+// stays near 45 KB of LDS, three workgroups per CU.
+#ifndef VLP2D_ROWS64
+#define VLP2D_ROWS64 32
+#endif
+#ifndef VLP2D_ROWS32
+#define VLP2D_ROWS32 32
+#endif
+#ifndef VLP2D_TILE_COLMAJOR
+#define VLP2D_TILE_COLMAJOR 0
+#endif
+template <typename T> __host__ __device__ constexpr int tile_rows() { return sizeof(T) == 8 ? VLP2D_ROWS64 : VLP2D_ROWS32; }
 __host__ __device__ constexpr int tile_cols(int H) { return SJ - 2 * H; }
 __host__ __device__ constexpr int down_halo(int ns) { return 2 * ns + 2; }
 __host__ __device__ constexpr int up_halo(int ns, int epi) { return 2 * ns + ((epi == 1 || epi == 3) ? 1 : 0); }
@@ -75,12 +85,24 @@ struct MixView {
 };
 
 // Workgroup -> tile: consecutive workgroup ids land on different XCDs
-// (round-robin), so each XCD is given one contiguous band of tiles; tiles
-// sharing a halo then share that XCD's L2.
+// (round-robin), so each XCD is given one contiguous run of tiles, and the
+// workgroups an XCD runs together are neighbours that share halos in its L2.
 __device__ inline int tile_of_block(int tiles, int chunk) {
   const int b = blockIdx.x;
   const int t = (b & 7) * chunk + (b >> 3);
   return t < tiles ? t : -1;
+}
+
+// tile number -> (tile row, tile column): row-major (measured faster on
+// MI355X than column-major for these tile shapes), or column-major.
+__device__ inline void tile_coords(int t, int tiles_i, int tiles_j, int &ti, int &tj) {
+  if (VLP2D_TILE_COLMAJOR) {
+    tj = t / tiles_i;
+    ti = t - tj * tiles_i;
+  } else {
+    ti = t / tiles_j;
+    tj = t - ti * tiles_j;
+  }
 }
 
 template <typename T, int H> struct Geo {
@@ -250,7 +272,8 @@ __global__ void __launch_bounds__(TNT) k_down(LevelView<T> a, T *__restrict__ rh
   __shared__ T U[2 * G::NC];
   const int t = tile_of_block(a.tiles_i * a.tiles_j, a.chunk);
   if (t < 0) return;
-  const int ti = t / a.tiles_j, tj = t - (t / a.tiles_j) * a.tiles_j;
+  int ti, tj;
+  tile_coords(t, a.tiles_i, a.tiles_j, ti, tj);
   const int gi0 = 1 + ti * G::TI - H, gj0 = 1 + tj * G::TJ - H;
   stage<T, H, ZI>(U, a, gi0, gj0, nullptr, 0);
   T fr[2][G::RPW];
@@ -302,7 +325,8 @@ __global__ void __launch_bounds__(TNT) k_up(LevelView<T> a, const T *__restrict_
     }
     return;
   }
-  const int ti = t / a.tiles_j, tj = t - (t / a.tiles_j) * a.tiles_j;
+  int ti, tj;
+  tile_coords(t, a.tiles_i, a.tiles_j, ti, tj);
   const int gi0 = 1 + ti * G::TI - H, gj0 = 1 + tj * G::TJ - H;
   const int tid = threadIdx.x, w = tid >> 6, m = tid & 63;
   stage<T, H, false>(U, a, gi0, gj0, uc, pc);
@@ -370,6 +394,77 @@ __global__ void __launch_bounds__(TNT) k_up(LevelView<T> a, const T *__restrict_
     em = block_reduce<true>(em, red, tid, TNT);
     if (tid == 0) mx.part_max[blockIdx.x] = em;
   }
+}
+
+}  // namespace vlp2d
+
+namespace vlp2d {
+
+// ---------------------------------------------------------------------------
+// Refinement update of a mixed-precision step, after the FP32 correction
+// cycle: uout = uin + (double) e on interior nodes (boundary nodes of uout
+// already hold the Dirichlet values), r32out = (float)(fn - Lap uout) in FP64,
+// partial sums of r^2 and partial max |e| per workgroup.  A workgroup stages
+// its 32 x 64 output tile plus a one-node ring of uout in LDS, so every input
+// is read from HBM about once.
+// ---------------------------------------------------------------------------
+constexpr int RF_TI = 32, RF_TJ = 64, RF_W = RF_TJ + 2, RF_N = (RF_TI + 2) * RF_W;
+
+__global__ void __launch_bounds__(TNT) k_refine(const double *__restrict__ uin, double *__restrict__ uout,
+                                                const float *__restrict__ e, const double *__restrict__ fn,
+                                                float *__restrict__ r32out, int nx, int ny, int pitch, double hx2,
+                                                double hy2, double ihx2, double ihy2, int pw2, int tiles_i,
+                                                double *__restrict__ part_sum, double *__restrict__ part_max) {
+  __shared__ double W[RF_N];
+  __shared__ double red[TNT / 64];
+  const int t = blockIdx.x;
+  const int tj = t / tiles_i, ti = t - (t / tiles_i) * tiles_i;
+  const int gi0 = ti * RF_TI, gj0 = tj * RF_TJ;  // staged origin: output origin minus the ring
+  const int tid = threadIdx.x;
+  constexpr int KS = (RF_N + TNT - 1) / TNT;
+  double v[KS];
+#pragma unroll
+  for (int k = 0; k < KS; ++k) {
+    const int p = tid + TNT * k;
+    const int li = p / RF_W, lj = p - (p / RF_W) * RF_W;
+    const int gi = gi0 + li, gj = gj0 + lj;
+    double x = 0.0;
+    if (p < RF_N && gi < nx && gj < ny) {
+      const int g = gi * pitch + gj;
+      x = uin[g];
+      if (inside(gi, gj, nx, ny)) x = x + (double)e[g];
+    }
+    v[k] = x;
+  }
+#pragma unroll
+  for (int k = 0; k < KS; ++k) {
+    const int p = tid + TNT * k;
+    if (p < RF_N) W[p] = v[k];
+  }
+  __syncthreads();
+  const int w = tid >> 6, m = tid & 63;
+  double ss = 0.0, em = 0.0;
+#pragma unroll
+  for (int k = 0; k < RF_TI / 4; ++k) {
+    const int oi = w + 4 * k;
+    const int gi = gi0 + 1 + oi, gj = gj0 + 1 + m;
+    if (inside(gi, gj, nx, ny)) {
+      const int c = (oi + 1) * RF_W + (m + 1);
+      const int g = gi * pitch + gj;
+      const double u = W[c];
+      const double u2 = u * 2.0;
+      const double r = fn[g] - (divh(W[c - RF_W] + W[c + RF_W] - u2, hx2, ihx2, pw2) +
+                                divh(W[c - 1] + W[c + 1] - u2, hy2, ihy2, pw2));
+      uout[g] = u;
+      r32out[g] = (float)r;
+      ss += r * r;
+      em = nanmax(em, fabs((double)e[g]));
+    }
+  }
+  ss = block_reduce<false>(ss, red, tid, TNT);
+  if (tid == 0) part_sum[blockIdx.x] = ss;
+  em = block_reduce<true>(em, red, tid, TNT);
+  if (tid == 0) part_max[blockIdx.x] = em;
 }
 
 }  // namespace vlp2d
