@@ -20,6 +20,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -66,7 +68,9 @@ def profile_solve(p, mixed, rtol):
 
 def cpu_baseline(seg, rtol):
     """Oracle (C++ restatement of the reference CPU multigrid, OpenMP over
-    rows) on the same formulas at a reduced size: FP64 V(2,2) to rtol."""
+    rows) on the same formulas: FP64 V(2,2) to rtol.  Returns the baseline
+    record and the oracle's solution (the FP64 oracle the GPU result is
+    compared with)."""
     from oracle import Oracle
     from vlp2d_amd import problems
     p = problems.config(3, seg)
@@ -77,10 +81,35 @@ def cpu_baseline(seg, rtol):
     t0 = time.perf_counter()
     k, hist = o.plain_solve(2, 2, 3, rtol, 100)
     dt = time.perf_counter() - t0
+    u = o.get_u()
     o.close()
-    return {"value": p.unknowns / dt / 1e6, "unit": "Munknowns/s", "cores": cores, "kind": "port",
-            "sample": f"oracle FP64 V(2,2) RB-GS multigrid, {seg + 1}x{seg + 1} replica of the workload, "
-                      f"{k} cycles to rel. residual {hist[-1]:.2e} in {dt:.2f} s"}
+    rec = {"value": round(p.unknowns / dt / 1e6, 2), "unit": "Munknowns/s", "cores": cores, "kind": "port",
+           "sample": f"oracle FP64 V(2,2) RB-GS multigrid on the {seg + 1}x{seg + 1} workload, "
+                     f"{k} cycles to rel. residual {hist[-1]:.2e} in {dt:.2f} s"}
+    return rec, u
+
+
+def dd128_check(seg=512, rtol=1e-10):
+    """Error of the GPU mixed-precision solution of the 513^2 replica against
+    the oracle's double-double multigrid converged to its floor."""
+    import numpy as np
+    from oracle import Oracle
+    from vlp2d_amd import MultiGridSchemeHIP, Options, problems
+    p = problems.config(3, seg)
+    o = Oracle(p.seg_x, p.seg_y, p.hx, p.hy, p.f, "dd")
+    o.set_u(p.initial("zero"))
+    for _ in range(60):
+        if o.vcycle(3, 3, 3) < 1e-26:
+            break
+    hi, lo = o.get_u(with_lo=True)
+    o.close()
+    s = MultiGridSchemeHIP(p.seg_x, p.seg_y, p.hx, p.hy, p.f, options=Options.performance(True))
+    s.un0[:] = p.initial("zero")
+    s.init_after_boundaries_and_initial_iteration_inited()
+    s.solve(rtol, 100)
+    u = s.get_array()
+    s.cleanup()
+    return float(np.abs((u - hi) - lo).max() / np.abs(hi).max())
 
 
 def main():
@@ -99,7 +128,6 @@ def main():
               "(the multigrid path runs on one MI355X, see DESIGN.md for how it would shard)", file=sys.stderr)
         sys.exit(2)
 
-    import numpy as np
     from vlp2d_amd import MultiGridSchemeHIP, Options, problems
 
     t0 = time.perf_counter()
@@ -113,7 +141,8 @@ def main():
         s.init_after_boundaries_and_initial_iteration_inited()
         s.save_iterate()
         wall, dev, cyc, rel = solve_times(s, args.steps, args.warmup, args.rtol)
-        res[mixed] = {"wall": wall, "dev": dev, "cycles": cyc, "rel": rel, "info": s.info()}
+        res[mixed] = {"wall": wall, "dev": dev, "cycles": cyc, "rel": rel, "info": s.info(),
+                      "u": s.get_array().copy()}
         log(f"{'mixed' if mixed else 'fp64 '}: cycles {cyc}, wall ms {[round(w * 1e3, 3) for w in wall]}, "
             f"rel {max(rel):.2e}")
         s.cleanup()
@@ -132,16 +161,33 @@ def main():
         per_launch_bytes = nbytes / launches
         avg_s = ms * 1e-3 / launches
         achieved = per_launch_bytes / avg_s / 1e9
+        key = f"{name}@level{lvl}"
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                    "kernel": f"{name}@level{lvl}", "avg_launch_us": round(avg_s * 1e6, 3),
+                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(key),
+                    "kernel": key, "avg_launch_us": round(avg_s * 1e6, 3),
                     "alg_bytes_per_launch": per_launch_bytes, "share_of_solve": round(ms / tot, 3)}
         kernels = sorted(([e[0], e[1], e[2], round(e[3], 4), e[4]] for e in kt), key=lambda e: -e[3])[:12]
-        alg_total = sum(e[4] for e in kt)
-        roofline["solve_alg_bytes"] = alg_total
-        roofline["solve_effective_frac"] = round(alg_total / (mixed_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+    # whole-solve accounting with the per-operator byte model (unfused operators)
+    solve_bytes = {m: alg_bytes_per_operator(p, m, res[m]["cycles"][0]) for m in (True, False)}
+    solve_roofline = {
+        "model": "per-operator algorithmic bytes (each operator one streaming pass; V(2,2): 25 s B/node/cycle, "
+                 "refinement 40 B/node/step, residual check 16 B/node)",
+        "mixed": {"bytes": solve_bytes[True], "effective_GBps": round(solve_bytes[True] / (mixed_ms * 1e-3) / 1e9, 1),
+                  "frac": round(solve_bytes[True] / (mixed_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+        "fp64": {"bytes": solve_bytes[False], "effective_GBps": round(solve_bytes[False] / (fp64_ms * 1e-3) / 1e9, 1),
+                 "frac": round(solve_bytes[False] / (fp64_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+    }
 
-    cpu = None if args.no_cpu else cpu_baseline(args.cpu_seg, args.rtol)
+    accuracy = {"mixed_vs_fp64_gpu_rel": float(np.abs(res[True]["u"] - res[False]["u"]).max()
+                                               / np.abs(res[False]["u"]).max())}
+    cpu = None
+    if not args.no_cpu:
+        cpu, u_oracle = cpu_baseline(args.cpu_seg, args.rtol)
+        if args.cpu_seg == args.seg:
+            scale = np.abs(u_oracle).max()
+            accuracy["mixed_vs_fp64_oracle_rel"] = float(np.abs(res[True]["u"] - u_oracle).max() / scale)
+            accuracy["fp64_gpu_vs_fp64_oracle_rel"] = float(np.abs(res[False]["u"] - u_oracle).max() / scale)
+        accuracy["mixed_513_vs_dd128_oracle_rel"] = dd128_check()
 
     line = {
         "metric": "Munknowns/s to rel. residual 1e-10: FP32+FP64-refine vs FP64 MG; % HBM roofline",
@@ -158,17 +204,58 @@ def main():
         "data": "synthetic (seeded Gaussian bumps, SplitMix64)",
         "config": {"workload": f"poisson_unit_square_{args.seg + 1}x{args.seg + 1}_gaussian_bumps64_"
                                f"mixed_fp32_vcycles_fp64_refinement_to_rel_residual_{args.rtol:g}",
-                   "nodes": [p.seg_x + 1, p.seg_y + 1], "unknowns": p.unknowns, "cycle": "V(2,2) RB-GS, HW restriction",
-                   "parallelism": "single GPU"},
+                   "nodes": [p.seg_x + 1, p.seg_y + 1], "unknowns": p.unknowns,
+                   "cycle": "V(2,2) RB-GS, half-weighting restriction", "parallelism": "single GPU"},
         "fp64": {"value": round(p.unknowns / (fp64_ms * 1e-3) / 1e6, 2), "ms_per_step": round(fp64_ms, 4),
                  "cycles": res[False]["cycles"][0]},
         "mixed_cycles": res[True]["cycles"][0],
         "speedup_mixed_vs_fp64": round(fp64_ms / mixed_ms, 3),
         "roofline": roofline,
+        "solve_roofline": solve_roofline,
+        "accuracy": accuracy,
         "cpu_baseline": cpu,
         "kernels_top": kernels,
     }
     print(json.dumps(line), flush=True)
+
+
+def alg_bytes_per_operator(p, mixed, cycles, nu1=2, nu2=2):
+    """Bytes a solve moves if every operator is one streaming pass: per
+    non-coarsest level (3(nu1+nu2) + 6) s N_l + 3 s N_{l+1}, coarsest 3 s N_c;
+    mixed adds 40 B/node per refinement step (FP64 residual + cast, FP64
+    update) and 20 B/node for the first residual; FP64 adds a 16 B/node
+    residual check per cycle and one initial check."""
+    s = 4 if mixed else 8
+    nx, ny = p.seg_x, p.seg_y
+    levels = []
+    while True:
+        levels.append((nx + 1) * (ny + 1))
+        if nx % 2 or ny % 2 or min(nx, ny) <= 2:
+            break
+        nx //= 2
+        ny //= 2
+    L = min(int(np.log2(p.seg_x)), int(np.log2(p.seg_y)))
+    levels = levels[:L]
+    per_cycle = sum((3 * (nu1 + nu2) + 6) * s * levels[l] + 3 * s * levels[l + 1] for l in range(L - 1))
+    per_cycle += 3 * s * levels[-1]
+    n0 = levels[0]
+    if mixed:
+        return float(cycles * (per_cycle + 40 * n0) + 20 * n0)
+    return float(cycles * (per_cycle + 16 * n0) + 16 * n0)
+
+
+def measured_traffic(kernel_key):
+    """HBM bytes per launch of a kernel from the committed rocprofv3 PMC
+    summary (profiles/level0_kernels_pmc.md, lines 'TRAFFIC <key> <bytes>'),
+    or None when no measurement of that kernel is recorded."""
+    path = os.path.join(ROOT, "profiles", "level0_kernels_pmc.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        parts = line.split()
+        if len(parts) == 3 and parts[0] == "TRAFFIC" and parts[1] == kernel_key:
+            return float(parts[2])
+    return None
 
 
 if __name__ == "__main__":
