@@ -209,8 +209,30 @@ def test_mixed_full_size_8193():
     assert d <= 1e-7 * np.abs(out[False][0]).max()
 
 
+@pytest.mark.parametrize("seg,K", [(4096, 2), (8192, 2), (16384, 1)])
+def test_forward_cycles_bitwise_full_sizes(seg, K):
+    """FP64 V(2,2) cycles at the benchmark sizes (config 3 and config 5
+    shapes) through the fused kernels: the iterate after K cycles is the
+    oracle's bit for bit (the map the adjoint transposes)."""
+    from oracle import forward_cycles
+    p = problems.config(3, seg)
+    _say(f"forward cycles {seg + 1}^2 K={K}")
+    u0 = p.initial("zero")
+    ur = forward_cycles(p.seg_x, p.seg_y, p.hx, p.hy, p.f, u0, K, 2, 2, 3)
+    s = MultiGridSchemeHIP(p.seg_x, p.seg_y, p.hx, p.hy, p.f, options=Options.performance())
+    s.un0[:] = u0
+    s.init_after_boundaries_and_initial_iteration_inited()
+    s.forward_cycles(K)
+    ug = s.get_array()
+    d = np.abs(ug - ur)
+    bad = np.argwhere(d > 0)
+    assert bad.shape[0] == 0, (bad.shape[0], bad[:10].tolist(), float(d.max()))
+    s.cleanup()
+
+
 # ------------------------------------------------------------------ adjoint
-@pytest.mark.parametrize("seg_x,seg_y,hx,hy,K", [(16, 16, 1 / 16, 1 / 16, 2), (64, 32, 0.01, 0.02, 3)])
+@pytest.mark.parametrize("seg_x,seg_y,hx,hy,K", [(16, 16, 1 / 16, 1 / 16, 2), (64, 32, 0.01, 0.02, 3),
+                                                (1024, 1024, 1 / 1024, 1 / 1024, 2)])
 def test_adjoint_cycles_match_oracle(seg_x, seg_y, hx, hy, K):
     _say(f"adjoint cycles {seg_x}x{seg_y} K={K}")
     f, g = _random(seg_x, seg_y, 99)
