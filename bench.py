@@ -26,6 +26,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "poisson8193_level0_pmc.md")
+# launch classes that run on the FP32 hierarchy in a mixed-precision solve
+FP32_CLASSES_IN_MIXED = {"down_fused", "up_fused", "coarse_vcycle", "rb_half", "residual", "restrict",
+                         "prolong", "fill"}
 
 
 def log(msg):
@@ -125,7 +129,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              "(the multigrid path runs on one MI355X, see DESIGN.md for how it would shard)", file=sys.stderr)
+              "(the multigrid path runs on one MI355X)", file=sys.stderr)
         sys.exit(2)
 
     from vlp2d_amd import MultiGridSchemeHIP, Options, problems
@@ -162,9 +166,10 @@ def main():
         avg_s = ms * 1e-3 / launches
         achieved = per_launch_bytes / avg_s / 1e9
         key = f"{name}@level{lvl}"
+        prec = "f32" if name in FP32_CLASSES_IN_MIXED else "f64"
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(key),
-                    "kernel": key, "avg_launch_us": round(avg_s * 1e6, 3),
+                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(f"{key}/{prec}"),
+                    "kernel": f"{key}/{prec}", "avg_launch_us": round(avg_s * 1e6, 3),
                     "alg_bytes_per_launch": per_launch_bytes, "share_of_solve": round(ms / tot, 3)}
         kernels = sorted(([e[0], e[1], e[2], round(e[3], 4), e[4]] for e in kt), key=lambda e: -e[3])[:12]
     # whole-solve accounting with the per-operator byte model (unfused operators)
@@ -200,7 +205,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 V-cycles + f64 refinement",
+        "dtype": "f32+f64",
         "data": "synthetic (seeded Gaussian bumps, SplitMix64)",
         "config": {"workload": f"poisson_unit_square_{args.seg + 1}x{args.seg + 1}_gaussian_bumps64_"
                                f"mixed_fp32_vcycles_fp64_refinement_to_rel_residual_{args.rtol:g}",
@@ -246,9 +251,10 @@ def alg_bytes_per_operator(p, mixed, cycles, nu1=2, nu2=2):
 
 def measured_traffic(kernel_key):
     """HBM bytes per launch of a kernel from the committed rocprofv3 PMC
-    summary (profiles/level0_kernels_pmc.md, lines 'TRAFFIC <key> <bytes>'),
-    or None when no measurement of that kernel is recorded."""
-    path = os.path.join(ROOT, "profiles", "level0_kernels_pmc.md")
+    summary (lines 'TRAFFIC <class>@level0/<f32|f64> <bytes>' written by
+    vlp2d_amd.rocprof_summary), or None when no measurement of that kernel
+    is recorded."""
+    path = PMC_SUMMARY
     if not os.path.exists(path):
         return None
     for line in open(path):

@@ -1,0 +1,166 @@
+"""Summaries of rocprofv3 CSV output for the solver's kernels.
+
+    python -m vlp2d_amd.rocprof_summary trace <kernel_trace.csv> [<kernel_stats.csv>]
+    python -m vlp2d_amd.rocprof_summary pmc <fetch counter_collection.csv> \
+        <write counter_collection.csv> --copy-bytes B
+
+``trace`` groups kernel dispatches by kernel and grid size (a grid size
+identifies a multigrid level) and prints launches, average and total
+duration as a Markdown table.
+
+``pmc`` averages FETCH_SIZE and WRITE_SIZE (KiB per dispatch) per kernel and
+grid size, calibrates both against the runtime's device-to-device copy
+kernel of known size (``--copy-bytes``, the bytes the copy reads and also
+writes), and prints the calibrated HBM traffic per launch.  For the largest
+grid of each solver kernel (level 0) it adds a line
+
+    TRAFFIC <class>@level0/<f32|f64> <bytes>
+
+where <class> is the launch class the library reports in its kernel timings.
+"""
+import argparse
+import collections
+import csv
+import re
+import sys
+
+# kernel function name -> launch class reported by vlp2d_mg_kernel_times
+KERNEL_CLASS = {
+    "k_down": "down_fused",
+    "k_up": "up_fused",
+    "k_coarse": "coarse_vcycle",
+    "k_refine_stream": "update_mixed",
+    "k_refine": "update_mixed",
+    "k_update_mixed": "update_mixed",
+    "k_update_oop": "update_mixed",
+    "k_resnorm64": "resnorm64",
+    "k_reduce": "reduce",
+    "k_rb_half": "rb_half",
+    "k_residual": "residual",
+    "k_restrict": "restrict",
+    "k_prolong": "prolong",
+    "k_fill": "fill",
+    "k_copy_interior": "copy_interior",
+    "k_absdiff_max": "absdiff_max",
+}
+
+# classes whose largest grid is the level-0 (finest grid) launch
+LEVEL0_CLASSES = {"down_fused", "up_fused", "update_mixed", "resnorm64", "rb_half", "residual", "restrict",
+                  "prolong", "copy_interior", "absdiff_max"}
+
+COPY_PATTERN = re.compile(r"copyBuffer", re.I)
+
+
+def short_name(full):
+    """'void vlp2d::k_down<float, 2, true>(vlp2d::LevelView<float>, ...)' ->
+    'k_down<float, 2, true>'."""
+    name = full.split("(", 1)[0]
+    name = re.sub(r"^void\s+", "", name).strip()
+    name = name.replace("vlp2d::", "")
+    return name
+
+
+def base_and_precision(short):
+    base = short.split("<", 1)[0]
+    args = short[len(base):]
+    prec = "f32" if "float" in args.split(",", 1)[0] else "f64"
+    return base, prec
+
+
+def grid_of(row):
+    if "Grid_Size" in row and row["Grid_Size"]:
+        return int(row["Grid_Size"])
+    return int(row.get("Grid_Size_X", 1)) * int(row.get("Grid_Size_Y", 1)) * int(row.get("Grid_Size_Z", 1))
+
+
+def trace_table(path, top=40):
+    d = collections.defaultdict(list)
+    for r in csv.DictReader(open(path)):
+        key = (short_name(r["Kernel_Name"]), grid_of(r))
+        d[key].append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1000.0)
+    tot = sum(sum(v) for v in d.values())
+    out = ["| kernel | grid (threads) | launches | avg us | total us | share |",
+           "|---|---|---|---|---|---|"]
+    for (name, grid), v in sorted(d.items(), key=lambda kv: -sum(kv[1]))[:top]:
+        out.append(f"| `{name}` | {grid} | {len(v)} | {sum(v) / len(v):.1f} | {sum(v):.1f} | "
+                   f"{100 * sum(v) / tot:.1f}% |")
+    return "\n".join(out)
+
+
+def stats_table(path, top=20):
+    rows = list(csv.DictReader(open(path)))
+    out = ["| kernel | calls | avg us | total ms | share |", "|---|---|---|---|---|"]
+    for r in rows[:top]:
+        out.append(f"| `{short_name(r['Name'])}` | {r['Calls']} | {float(r['AverageNs']) / 1e3:.1f} | "
+                   f"{float(r['TotalDurationNs']) / 1e6:.3f} | {float(r['Percentage']):.1f}% |")
+    return "\n".join(out)
+
+
+def counter_means(path, counter):
+    d = collections.defaultdict(list)
+    for r in csv.DictReader(open(path)):
+        if r["Counter_Name"] != counter:
+            continue
+        d[(short_name(r["Kernel_Name"]), grid_of(r))].append(float(r["Counter_Value"]))
+    return {k: sum(v) / len(v) for k, v in d.items()}, {k: len(v) for k, v in d.items()}
+
+
+def pmc_report(fetch_csv, write_csv, copy_bytes):
+    fetch, nf = counter_means(fetch_csv, "FETCH_SIZE")
+    write, _ = counter_means(write_csv, "WRITE_SIZE")
+    copies = [k for k in fetch if COPY_PATTERN.search(k[0])]
+    if not copies:
+        raise SystemExit("no copy kernel in the FETCH_SIZE run: cannot calibrate")
+    ck = max(copies, key=lambda k: fetch[k])
+    cf = copy_bytes / (fetch[ck] * 1024.0)
+    cw = copy_bytes / (write[ck] * 1024.0) if ck in write else float("nan")
+    lines = [f"Calibration: `{ck[0]}` copying {copy_bytes} B reads FETCH_SIZE {fetch[ck]:.0f} KiB and "
+             f"WRITE_SIZE {write.get(ck, float('nan')):.0f} KiB per dispatch, so FETCH_SIZE is scaled by "
+             f"{cf:.3f} and WRITE_SIZE by {cw:.3f}.", "",
+             "| kernel | grid (threads) | dispatches | FETCH_SIZE KiB | WRITE_SIZE KiB | calibrated HBM bytes |",
+             "|---|---|---|---|---|---|"]
+    level0 = {}
+    for k in sorted(fetch, key=lambda k: -fetch[k]):
+        if k not in write:
+            continue
+        b = fetch[k] * 1024.0 * cf + write[k] * 1024.0 * cw
+        lines.append(f"| `{k[0]}` | {k[1]} | {nf[k]} | {fetch[k]:.0f} | {write[k]:.0f} | {b:.4g} |")
+        base, prec = base_and_precision(k[0])
+        cls = KERNEL_CLASS.get(base)
+        if cls not in LEVEL0_CLASSES:
+            continue
+        tag = f"{cls}@level0/{prec}"
+        if tag not in level0 or k[1] > level0[tag][0]:
+            level0[tag] = (k[1], b)
+    lines.append("")
+    lines.append("Level-0 traffic per launch (largest grid of each kernel class):")
+    lines.append("")
+    lines.append("```")
+    for tag, (_, b) in sorted(level0.items()):
+        lines.append(f"TRAFFIC {tag} {b:.0f}")
+    lines.append("```")
+    return "\n".join(lines)
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    sub = ap.add_subparsers(dest="mode", required=True)
+    t = sub.add_parser("trace")
+    t.add_argument("kernel_trace")
+    t.add_argument("kernel_stats", nargs="?")
+    p = sub.add_parser("pmc")
+    p.add_argument("fetch_csv")
+    p.add_argument("write_csv")
+    p.add_argument("--copy-bytes", type=float, required=True)
+    a = ap.parse_args(argv)
+    if a.mode == "trace":
+        print(trace_table(a.kernel_trace))
+        if a.kernel_stats:
+            print()
+            print(stats_table(a.kernel_stats))
+    else:
+        print(pmc_report(a.fetch_csv, a.write_csv, a.copy_bytes))
+
+
+if __name__ == "__main__":
+    sys.exit(main())
