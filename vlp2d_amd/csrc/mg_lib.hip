@@ -31,6 +31,7 @@
 #include "mg_tiles.hpp"
 #include "mg_coarse.hpp"
 #include "mg_refine.hpp"
+#include "mg_stream.hpp"
 
 #ifndef VLP2D_REFINE_STREAM
 #define VLP2D_REFINE_STREAM 1
@@ -318,61 +319,95 @@ template <typename T> LevelView<T> view(const Lvl<T> &v, const T *uin, T *uout, 
   return a;
 }
 
-template <typename T, int NS>
-void launch_down_ns(Ctx *c, const LevelView<T> &a, bool zi, Lvl<T> &g, int grid) {
-  if (zi)
-    hipLaunchKernelGGL((k_down<T, NS, true>), dim3(grid), dim3(TNT), 0, c->st, a, g.rhs, g.nx, g.ny, g.pitch);
-  else
-    hipLaunchKernelGGL((k_down<T, NS, false>), dim3(grid), dim3(TNT), 0, c->st, a, g.rhs, g.nx, g.ny, g.pitch);
-}
-
-template <typename T> void down(Ctx *c, Lvl<T> &f, Lvl<T> &g, int lvl, bool zi) {
-  LevelView<T> a = view(f, f.u, f.u2, down_halo(std::min(c->o.nu1, 3)));
-  const int grid = 8 * a.chunk;
-  const double s = sizeof(T);
-  launch(c, KC_DOWN, lvl, s * ((zi ? 2.0 : 3.0) * nodes(f) + nodes(g)), [&] {
-    switch (c->o.nu1) {
-      case 0: launch_down_ns<T, 0>(c, a, zi, g, grid); break;
-      case 1: launch_down_ns<T, 1>(c, a, zi, g, grid); break;
-      case 2: launch_down_ns<T, 2>(c, a, zi, g, grid); break;
-      default: launch_down_ns<T, 3>(c, a, zi, g, grid); break;
-    }
-  });
+// Streaming-kernel view: tile grid of k_sdown / k_sup with halo H and R
+// output rows per tile (64-thread workgroups, 8 * chunk of them).
+template <typename T> LevelView<T> sview(const Lvl<T> &v, const T *uin, T *uout, int H, int R) {
+  LevelView<T> a = view(v, uin, uout, H);
+  const STileGrid g = s_grid(v.nx, v.ny, R, H);
+  a.tiles_i = g.tiles_i;
+  a.tiles_j = g.tiles_j;
+  a.chunk = g.chunk;
+  return a;
 }
 
 template <typename T, int NS>
-void launch_up_ns(Ctx *c, const LevelView<T> &a, int epi, const Lvl<T> &g, int grid, double *part, const MixView &mx) {
-  if constexpr (sizeof(T) == 8) {
-    if (epi == EPI_RESNORM)
-      hipLaunchKernelGGL((k_up<T, NS, EPI_RESNORM>), dim3(grid), dim3(TNT), 0, c->st, a, g.u, g.pitch, part, mx);
-    else if (epi == EPI_METRIC)
-      hipLaunchKernelGGL((k_up<T, NS, EPI_METRIC>), dim3(grid), dim3(TNT), 0, c->st, a, g.u, g.pitch, part, mx);
-    else
-      hipLaunchKernelGGL((k_up<T, NS, EPI_NONE>), dim3(grid), dim3(TNT), 0, c->st, a, g.u, g.pitch, part, mx);
+void launch_down_ns(Ctx *c, Lvl<T> &f, Lvl<T> &g, bool zi) {
+  constexpr int H = s_down_halo(NS), R = s_rows(H, s_period(NS));
+  const LevelView<T> a = sview(f, f.u, f.u2, H, R);
+  const dim3 grid(8 * a.chunk), blk(64);
+  if (zi) {
+    if (a.eq) hipLaunchKernelGGL((k_sdown<T, NS, true, true>), grid, blk, 0, c->st, a, g.rhs, g.nx, g.ny, g.pitch);
+    else hipLaunchKernelGGL((k_sdown<T, NS, true, false>), grid, blk, 0, c->st, a, g.rhs, g.nx, g.ny, g.pitch);
   } else {
-    if (epi == EPI_MIXED)
-      hipLaunchKernelGGL((k_up<T, NS, EPI_MIXED>), dim3(grid), dim3(TNT), 0, c->st, a, g.u, g.pitch, part, mx);
-    else
-      hipLaunchKernelGGL((k_up<T, NS, EPI_NONE>), dim3(grid), dim3(TNT), 0, c->st, a, g.u, g.pitch, part, mx);
+    if (a.eq) hipLaunchKernelGGL((k_sdown<T, NS, false, true>), grid, blk, 0, c->st, a, g.rhs, g.nx, g.ny, g.pitch);
+    else hipLaunchKernelGGL((k_sdown<T, NS, false, false>), grid, blk, 0, c->st, a, g.rhs, g.nx, g.ny, g.pitch);
   }
 }
 
-// returns the launch grid (the number of partials an epilogue writes)
+template <typename T> void down(Ctx *c, Lvl<T> &f, Lvl<T> &g, int lvl, bool zi) {
+  const double s = sizeof(T);
+  launch(c, KC_DOWN, lvl, s * ((zi ? 2.0 : 3.0) * nodes(f) + nodes(g)), [&] {
+    switch (c->o.nu1) {
+      case 1: launch_down_ns<T, 1>(c, f, g, zi); break;
+      case 2: launch_down_ns<T, 2>(c, f, g, zi); break;
+      default: launch_down_ns<T, 3>(c, f, g, zi); break;
+    }
+  });
+}
+
+template <typename T, int NS, int EPI>
+int launch_sup(Ctx *c, Lvl<T> &f, const Lvl<T> &g) {
+  constexpr int H = s_up_halo(NS, EPI), R = s_rows(H, s_period(NS));
+  const LevelView<T> a = sview(f, f.u2, f.u, H, R);
+  const dim3 grid(8 * a.chunk), blk(64);
+  if (a.eq) hipLaunchKernelGGL((k_sup<T, NS, EPI, true>), grid, blk, 0, c->st, a, g.u, g.nx, g.ny, g.pitch, c->part);
+  else hipLaunchKernelGGL((k_sup<T, NS, EPI, false>), grid, blk, 0, c->st, a, g.u, g.nx, g.ny, g.pitch, c->part);
+  return a.tiles_i * a.tiles_j;
+}
+
+// Fused refinement epilogue (FP32 level 0 of a mixed step) on the LDS-tiled
+// ascent kernel.
+template <typename T, int NS>
+int launch_up_mixed(Ctx *c, Lvl<T> &f, const Lvl<T> &g, const MixView &mx) {
+  if constexpr (sizeof(T) == 4) {
+    const LevelView<T> a = view(f, f.u2, f.u, up_halo(NS, EPI_MIXED));
+    const int grid = 8 * a.chunk;
+    hipLaunchKernelGGL((k_up<T, NS, EPI_MIXED>), dim3(grid), dim3(TNT), 0, c->st, a, g.u, g.pitch, c->part, mx);
+    return grid;
+  } else {
+    return 0;
+  }
+}
+
+// FP64 levels run epilogues none / residual norm / update norm; FP32 levels
+// (the correction cycle of a mixed step) none / fused refinement.
+template <typename T, int NS> int launch_up_ns(Ctx *c, Lvl<T> &f, const Lvl<T> &g, int epi, const MixView &mx) {
+  if constexpr (sizeof(T) == 8) {
+    switch (epi) {
+      case EPI_RESNORM: return launch_sup<T, NS, EPI_RESNORM>(c, f, g);
+      case EPI_METRIC: return launch_sup<T, NS, EPI_METRIC>(c, f, g);
+      default: return launch_sup<T, NS, EPI_NONE>(c, f, g);
+    }
+  } else {
+    if (epi == EPI_MIXED) return launch_up_mixed<T, NS>(c, f, g, mx);
+    return launch_sup<T, NS, EPI_NONE>(c, f, g);
+  }
+}
+
+// returns the number of partials the epilogue writes
 template <typename T> int up(Ctx *c, Lvl<T> &f, Lvl<T> &g, int lvl, int epi, const MixView &mx) {
-  LevelView<T> a = view(f, f.u2, f.u, up_halo(std::min(c->o.nu2, 3), epi));
-  const int grid = 8 * a.chunk;
   const double s = sizeof(T);
   double bytes = s * (3.0 * nodes(f) + nodes(g));
   if (epi == EPI_MIXED) bytes = s * (2.0 * nodes(f) + nodes(g)) + 28.0 * nodes(f);
+  int np = 0;
   launch(c, KC_UP, lvl, bytes, [&] {
     switch (c->o.nu2) {
-      case 0: launch_up_ns<T, 0>(c, a, epi, g, grid, c->part, mx); break;
-      case 1: launch_up_ns<T, 1>(c, a, epi, g, grid, c->part, mx); break;
-      case 2: launch_up_ns<T, 2>(c, a, epi, g, grid, c->part, mx); break;
-      default: launch_up_ns<T, 3>(c, a, epi, g, grid, c->part, mx); break;
+      case 1: np = launch_up_ns<T, 1>(c, f, g, epi, mx); break;
+      case 2: np = launch_up_ns<T, 2>(c, f, g, epi, mx); break;
+      default: np = launch_up_ns<T, 3>(c, f, g, epi, mx); break;
     }
   });
-  return grid;
+  return np;
 }
 
 template <typename T> void coarse(Ctx *c, Hier<T> &H, bool zero_first, double *snap) {
@@ -406,8 +441,8 @@ template <typename T> int vcycle_fused(Ctx *c, Hier<T> &H, bool zero0, int epi, 
   coarse(c, H, lc > 0 || zero0, lc == 0 ? snap : nullptr);
   int np = 0;
   for (int l = lc - 1; l >= 0; --l) {
-    const int grid = up(c, H.lv[l], H.lv[l + 1], l, l == 0 ? epi : EPI_NONE, mx);
-    if (l == 0) np = grid;
+    const int n = up(c, H.lv[l], H.lv[l + 1], l, l == 0 ? epi : EPI_NONE, mx);
+    if (l == 0) np = n;
   }
   return np;
 }
@@ -499,12 +534,13 @@ void mixed_step(Ctx *c, Lvl<double> &P) {
   vcycle_fused(c, Z, true, EPI_NONE, mx, nullptr);  // correction e -> z.u
   int np = 0;
   if (VLP2D_REFINE_STREAM) {
-    const dim3 g((P.ny - 2 + 63) / 64, (P.nx - 2 + 4 * RR - 1) / (4 * RR));
-    np = (int)(g.x * g.y);
+    const int rows = refine_row_blocks(P.nx);
+    np = refine_strips(P.ny) * rows;
+    const int chunk = (np + 7) / 8;
     launch(c, KC_UPDATE, 0, 32.0 * nodes(P), [&] {
-      hipLaunchKernelGGL(k_refine_stream, g, dim3(256), 0, c->st, P.u, P.u2, z.u, P.rhs, z.rhs2, P.nx, P.ny,
-                         P.pitch, (double)P.hx2, (double)P.hy2, (double)P.ihx2, (double)P.ihy2, P.pw2 ? 1 : 0,
-                         c->part, c->part2);
+      hipLaunchKernelGGL(k_refine_stream, dim3(8 * chunk), dim3(256), 0, c->st, P.u, P.u2, z.u, P.rhs, z.rhs2, P.nx,
+                         P.ny, P.pitch, (double)P.hx2, (double)P.hy2, (double)P.ihx2, (double)P.ihy2, P.pw2 ? 1 : 0,
+                         rows, np, chunk, c->part, c->part2);
     });
   } else {
     const int ti = (P.nx - 2 + RF_TI - 1) / RF_TI, tj = (P.ny - 2 + RF_TJ - 1) / RF_TJ;
@@ -898,7 +934,7 @@ int vlp2d_mg_create(int segX, int segY, double hx, double hy, const double *f, d
   c->o = o;
   c->profile = (o.flags & VLP2D_FLAG_PROFILE) != 0;
   // fused kernels are instantiated for 0..3 sweeps per pass
-  c->simple = (o.flags & VLP2D_FLAG_SIMPLE) != 0 || o.nu1 > 3 || o.nu2 > 3;
+  c->simple = (o.flags & VLP2D_FLAG_SIMPLE) != 0 || o.nu1 < 1 || o.nu2 < 1 || o.nu1 > 3 || o.nu2 > 3;
   c->L = std::min((int)std::log2((double)segX + 0.5), (int)std::log2((double)segY + 0.5));
   auto bail = [&](int rc) {
     g_create_error = c->err;
@@ -925,10 +961,11 @@ int vlp2d_mg_create(int segX, int segY, double hx, double hy, const double *f, d
   fill(c, v.rhs, v.elems(), 0.0);
   {
     // partials of the largest launch grid: FP64 tiles of 32 rows, >= 100
-    // columns; refinement tiles of RF_TI x RF_TJ
+    // columns; refinement tiles of RF_TI x RF_TJ or of the streaming kernel
     long long tiles = (long long)((nx + 31) / 32) * ((ny + 99) / 100);
     long long rtiles = (long long)((nx + RF_TI - 1) / RF_TI) * ((ny + RF_TJ - 1) / RF_TJ);
-    c->part_cap = (int)std::max<long long>(1 << 16, std::max(tiles, rtiles) + 64);
+    long long stiles = (long long)refine_strips(ny) * refine_row_blocks(nx);
+    c->part_cap = (int)std::max<long long>(1 << 16, std::max(std::max(tiles, rtiles), stiles) + 64);
   }
   if ((rc = dalloc(c, &c->part, (size_t)c->part_cap)) != VLP2D_OK) return bail(rc);
   if ((rc = dalloc(c, &c->part2, (size_t)c->part_cap)) != VLP2D_OK) return bail(rc);

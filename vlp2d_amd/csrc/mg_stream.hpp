@@ -1,0 +1,540 @@
+// Register-streaming fused multigrid kernels for gfx950.
+//
+// One wavefront (one 64-thread workgroup) owns a tile of R output rows by
+// TJ = 128 - 2H output columns of one level.  It stages a strip of 128
+// columns (lane m holds the column pair cb + 2m, cb + 2m + 1) and walks it
+// top to bottom, one staged row per step, LR = R + 2H rows in all.  Every
+// row is loaded from HBM once per tile; all red-black Gauss-Seidel
+// half-sweeps of the launch are pipelined over the rows (temporal blocking in
+// registers): at step q the wavefront loads row q + 2 (prefetch), applies
+// half-sweep s to row q - s for s = 1..S, and runs the epilogue on the rows
+// the last half-sweep has finished.  Vertical neighbours are registers of the
+// same lane; horizontal neighbours of the other lane come through DPP
+// wavefront shifts (wave_shr:1 / wave_shl:1).  No LDS, no barriers.
+//
+// Red-black bookkeeping.  The tile origin (gi0, cb) has both coordinates
+// even, so node (gi0 + p, cb + 2m + e) has colour (p + e) & 1 (0 = red,
+// updated first, VisualLaplacePoisson2D/Model/CPU/MultiGridSlidingIteration.cs:
+// 53-57, GridIterator.cs:184-188).  Half-sweep s updates colour
+// c_s = (s - 1) & 1, one node per lane and row.  A Gauss-Seidel update reads
+// only the four neighbours, which are all of the other colour, so each
+// half-sweep level keeps just the value it updated: U[s][p] is the colour-c_s
+// node of row p after half-sweep s (element (p + c_s) & 1 of the pair), and
+// the full state of row p after half-sweep s is {U[s][p], U[s-1][p]}.
+// Levels -1 and 0 are the staged values (red and black).  A node that is not
+// updated (domain boundary) carries U[s-2][p].
+//
+// Within one colour the update is order independent, and every formula is
+// evaluated in the reference's operation order, so the result is bitwise the
+// reference's multigrid.  Rows and columns within distance s of the strip
+// edge hold partial results after half-sweep s; the halo H covers them.
+//
+// A tile whose staged nodes are all interior nodes of the domain (most of
+// them) runs a body without boundary predicates.
+//
+// Kernels:
+//   k_sdown : [zero or staged iterate] -> S = 2*NS half-sweeps -> residual
+//             and half-weighting restriction into the coarse right-hand side
+//             (MultiGridScheme.cs:134-169) -> write the tile.  H = S + 2.
+//   k_sup   : staged iterate + additive bilinear prolongation of the coarse
+//             correction (MultiGridScheme.cs:171-186) -> S half-sweeps ->
+//             write the tile, with epilogue EPI 0 (none), 1 (partial sums of
+//             the squared residual; H = S + 2) or 2 (partial max of the
+//             update |u - u_after_prolongation|, MultiGridScheme.cs:69,111).
+#pragma once
+#include "mg_tiles.hpp"
+
+#ifndef VLP2D_SROWS
+#define VLP2D_SROWS 64
+#endif
+static_assert(VLP2D_SROWS >= 32, "partial-sum buffers are sized for tiles of at least 32 rows");
+
+namespace vlp2d {
+
+constexpr int SCOLS = 128;  // staged columns per strip (64 lanes x 2)
+
+// Steps per unrolled loop body: a multiple of the colour period (2), the
+// level rings (6) and the right-hand-side ring (S + 4 rows live).
+__host__ __device__ constexpr int s_period(int ns) { return 2 * ns + 4 <= 6 ? 6 : 12; }
+__host__ __device__ constexpr int s_down_halo(int ns) { return 2 * ns + 2; }
+__host__ __device__ constexpr int s_up_halo(int ns, int epi) {
+  return epi == 1 ? 2 * ns + 2 : (2 * ns < 2 ? 2 : 2 * ns);
+}
+// Output rows per tile: about VLP2D_SROWS, chosen so that the staged rows
+// R + 2H are a whole number of loop bodies.
+__host__ __device__ constexpr int s_rows(int H, int P) {
+  return ((VLP2D_SROWS + 2 * H + P - 1) / P) * P - 2 * H;
+}
+
+struct STileGrid {
+  int tiles_i, tiles_j, tiles, chunk;
+};
+inline STileGrid s_grid(int nx, int ny, int R, int H) {
+  STileGrid g;
+  const int TJ = SCOLS - 2 * H;
+  g.tiles_i = (nx - 2) / R + 1;  // output rows [ti R, ti R + R) cover 1..nx-2
+  g.tiles_j = (ny - 2) / TJ + 1;
+  g.tiles = g.tiles_i * g.tiles_j;
+  g.chunk = (g.tiles + 7) / 8;
+  return g;
+}
+
+// ---- lane shifts (DPP) --------------------------------------------------
+// from_left(x): lane m receives lane m-1's x (lane 0 receives 0);
+// from_right(x): lane m receives lane m+1's x (lane 63 receives 0).
+// Called with all 64 lanes active.
+__device__ inline int dpp_shr1(int x) { return __builtin_amdgcn_update_dpp(0, x, 0x138, 0xF, 0xF, true); }
+__device__ inline int dpp_shl1(int x) { return __builtin_amdgcn_update_dpp(0, x, 0x130, 0xF, 0xF, true); }
+__device__ inline float from_left(float x) { return __int_as_float(dpp_shr1(__float_as_int(x))); }
+__device__ inline float from_right(float x) { return __int_as_float(dpp_shl1(__float_as_int(x))); }
+__device__ inline double from_left(double x) {
+  const long long b = __double_as_longlong(x);
+  const int lo = dpp_shr1((int)b), hi = dpp_shr1((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ inline double from_right(double x) {
+  const long long b = __double_as_longlong(x);
+  const int lo = dpp_shl1((int)b), hi = dpp_shl1((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+template <int N> __host__ __device__ constexpr int md(int x) { return ((x % N) + N) % N; }
+
+// compile-time loop: f(integral_constant<int, 0>) ... f(integral_constant<int, N-1>)
+template <int I, int N, typename F> __device__ __forceinline__ void static_for(F &&f) {
+  if constexpr (I < N) {
+    f(std::integral_constant<int, I>{});
+    static_for<I + 1, N>(f);
+  }
+}
+
+// Point update of the reference (MultiGridSlidingIteration.cs:53-57):
+// EQ (hx == hy): (uim + uip + ujm + ujp - f hx^2) / 4; otherwise
+// coef ((uim + uip)/hx^2 + (ujm + ujp)/hy^2 - f).  Laplace levels carry f = 0,
+// which leaves both forms bitwise unchanged (x - (+0) == x).
+template <typename T, bool EQ, bool PW2>
+__device__ __forceinline__ T s_gs(T uim, T uip, T ujm, T ujp, T f, const LevelView<T> &a) {
+  if constexpr (EQ) {
+    return (uim + uip + ujm + ujp - f * a.hx2) * T(0.25);
+  } else {
+    if constexpr (PW2) return a.coef * ((uim + uip) * a.ihx2 + (ujm + ujp) * a.ihy2 - f);
+    else return a.coef * ((uim + uip) / a.hx2 + (ujm + ujp) / a.hy2 - f);
+  }
+}
+
+// Residual rhs - Lap u of the reference (UtilsOpLap.cs:15-20).
+template <typename T, bool PW2>
+__device__ __forceinline__ T s_res(T u, T uim, T uip, T ujm, T ujp, T rhs, const LevelView<T> &a) {
+  const T u2 = u * T(2);
+  if constexpr (PW2) return rhs - ((uim + uip - u2) * a.ihx2 + (ujm + ujp - u2) * a.ihy2);
+  else return rhs - ((uim + uip - u2) / a.hx2 + (ujm + ujp - u2) / a.hy2);
+}
+
+template <typename T> struct Pair {
+  T x0, x1;
+};
+
+// Pair load / store at byte offset boff of a row (the row pointer is
+// wavefront-uniform, the offset per lane).
+template <typename T> __device__ __forceinline__ Pair<T> ld_pair(const T *row, unsigned boff) {
+  const char *p = reinterpret_cast<const char *>(row) + boff;
+  if constexpr (sizeof(T) == 8) {
+    const double2 v = *reinterpret_cast<const double2 *>(p);
+    return Pair<T>{v.x, v.y};
+  } else {
+    const float2 v = *reinterpret_cast<const float2 *>(p);
+    return Pair<T>{v.x, v.y};
+  }
+}
+template <typename T> __device__ __forceinline__ void st_pair(T *row, unsigned boff, T x0, T x1) {
+  char *p = reinterpret_cast<char *>(row) + boff;
+  if constexpr (sizeof(T) == 8) *reinterpret_cast<double2 *>(p) = make_double2(x0, x1);
+  else *reinterpret_cast<float2 *>(p) = make_float2(x0, x1);
+}
+
+// Per-lane constants of a strip.
+struct Lane {
+  int m;          // lane
+  int gj;         // column of element 0 (even); element 1 is gj + 1
+  bool ld;        // the pair lies inside the row allocation
+  bool in0, in1;  // interior columns
+  bool oc;        // output column pair
+  unsigned boff;  // byte offset of the pair within a row (0 when !ld)
+};
+
+template <typename T> __device__ inline Lane make_lane(int cb, int H, int TJ, int ny, int pitch) {
+  Lane L;
+  L.m = threadIdx.x & 63;
+  L.gj = cb + 2 * L.m;
+  L.ld = L.gj >= 0 && L.gj <= pitch - 2;
+  L.in0 = L.gj >= 1 && L.gj <= ny - 2;
+  L.in1 = L.gj + 1 >= 1 && L.gj + 1 <= ny - 2;
+  L.oc = L.m >= H / 2 && L.m < H / 2 + TJ / 2;
+  L.boff = L.ld ? (unsigned)L.gj * (unsigned)sizeof(T) : 0u;
+  return L;
+}
+
+template <typename T> __device__ __forceinline__ const T *row_of(const T *base, int gi, int pitch) {
+  return base + gi * pitch;
+}
+template <typename T> __device__ __forceinline__ T *row_of(T *base, int gi, int pitch) { return base + gi * pitch; }
+
+// Store the final pair of row gi where the nodes are output nodes (interior
+// nodes of the domain; every node of an interior tile is).
+template <typename T, bool EDGE>
+__device__ __forceinline__ void s_store(T *uout, int pitch, int gi, const Lane &L, T x0, T x1) {
+  T *row = row_of(uout, gi, pitch);
+  if constexpr (!EDGE) {
+    if (L.oc) st_pair(row, L.boff, x0, x1);
+  } else {
+    const bool o0 = L.oc && L.in0, o1 = L.oc && L.in1;
+    if (o0 && o1) st_pair(row, L.boff, x0, x1);
+    else if (o0) row[L.gj] = x0;
+    else if (o1) row[L.gj + 1] = x1;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Sweep state shared by both kernels.  Ring slots are compile-time: row p
+// lives in slot p mod 6 of each level ring (a level-s row is read up to
+// three steps after the next row of level s-1 is written), p mod P of the
+// right-hand-side ring, p mod 3 of the prefetch ring (staged row q = q0 + J
+// with q0 a multiple of P, itself a multiple of 6).  The rings start at zero;
+// rows above the first staged row then read zeros, and their results lie in
+// the halo.
+// ---------------------------------------------------------------------------
+template <typename T, int NS> struct SweepState {
+  static constexpr int S = 2 * NS;
+  static constexpr int P = s_period(NS);
+  T U[S + 2][6];  // U[s + 1][p % 6]: level s in -1..S
+  T F[P][2];      // right-hand side of row p (0 on Laplace levels and outside the domain)
+  T RU[3][2];     // prefetched staged rows (raw)
+  __device__ void zero() {
+#pragma unroll
+    for (int s = 0; s < S + 2; ++s)
+#pragma unroll
+      for (int k = 0; k < 6; ++k) U[s][k] = T(0);
+#pragma unroll
+    for (int k = 0; k < P; ++k) F[k][0] = F[k][1] = T(0);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) RU[k][0] = RU[k][1] = T(0);
+  }
+  // full value of element e of row p = q - d (J the step phase) after
+  // half-sweep s
+  template <int s, int J, int d, int e> __device__ __forceinline__ T full() {
+    constexpr int p = J - d;  // row index mod P (only parity and mod 6 matter)
+    constexpr int cs = md<2>(s - 1);
+    constexpr int es = md<2>(p + cs);
+    if constexpr (e == es) return U[s + 1][md<6>(p)];
+    else return U[s][md<6>(p)];
+  }
+};
+
+// Apply half-sweeps 1..S at step J (row q = q0 + J): half-sweep s updates
+// row q - s, using level s-1 rows q-s-1, q-s, q-s+1.  On edge tiles nodes
+// outside the domain interior keep their value.
+template <typename T, int NS, bool EQ, bool PW2, bool EDGE, int J>
+__device__ __forceinline__ void s_sweeps(SweepState<T, NS> &st, const LevelView<T> &a, const Lane &L, int gi0, int q) {
+  constexpr int S = 2 * NS, P = SweepState<T, NS>::P;
+  static_for<1, S + 1>([&](auto sc) {
+    constexpr int s = decltype(sc)::value;
+    constexpr int p = J - s;  // phase of the row
+    constexpr int cs = md<2>(s - 1);
+    constexpr int e = md<2>(p + cs);  // element holding colour cs in this row
+    const T above = st.U[s][md<6>(p - 1)];
+    const T below = st.U[s][md<6>(p + 1)];
+    const T mid = st.U[s][md<6>(p)];
+    T ujm, ujp;
+    if constexpr (e == 0) {
+      ujm = from_left(mid);
+      ujp = mid;
+    } else {
+      ujm = mid;
+      ujp = from_right(mid);
+    }
+    const T g = s_gs<T, EQ, PW2>(above, below, ujm, ujp, st.F[md<P>(p)][e], a);
+    if constexpr (EDGE) {
+      const int gi = gi0 + q - s;
+      const bool ok = gi >= 1 && gi <= a.nx - 2 && (e == 0 ? L.in0 : L.in1);
+      st.U[s + 1][md<6>(p)] = ok ? g : st.U[s - 1][md<6>(p)];
+    } else {
+      st.U[s + 1][md<6>(p)] = g;
+    }
+  });
+}
+
+// Load the raw pair of staged row q + 2 (u unless zero-initialised) and its
+// right-hand side.
+template <typename T, int NS, bool ZI, bool EDGE, int J>
+__device__ __forceinline__ void s_prefetch(SweepState<T, NS> &st, const LevelView<T> &a, const Lane &L, int gi0, int q,
+                                           int LR) {
+  constexpr int P = SweepState<T, NS>::P;
+  const int qq = q + 2, gi = gi0 + qq;
+  Pair<T> u{T(0), T(0)}, f{T(0), T(0)};
+  bool go = qq < LR;
+  if constexpr (EDGE) go = go && gi >= 0 && gi <= a.nx - 1 && L.ld;
+  if (go) {
+    if constexpr (!ZI) u = ld_pair(row_of(a.uin, gi, a.pitch), L.boff);
+    if (!a.lap) f = ld_pair(row_of(a.f, gi, a.pitch), L.boff);
+  }
+  st.RU[md<3>(J + 2)][0] = u.x0;
+  st.RU[md<3>(J + 2)][1] = u.x1;
+  st.F[md<P>(J + 2)][0] = f.x0;
+  st.F[md<P>(J + 2)][1] = f.x1;
+}
+
+// Tile of this workgroup: output rows [ti R, ti R + R), columns
+// [tj TJ, tj TJ + TJ); staged origin (gi0, cb) = (ti R - H, tj TJ - H).
+// An interior tile stages only interior nodes of the domain.
+struct STile {
+  int t, gi0, cb;
+  bool interior;
+};
+template <int R, int H> __device__ __forceinline__ bool s_tile(int tiles_i, int tiles_j, int chunk, int nx, int ny,
+                                                               STile &tl) {
+  constexpr int TJ = SCOLS - 2 * H, LR = R + 2 * H;
+  tl.t = tile_of_block(tiles_i * tiles_j, chunk);
+  if (tl.t < 0) return false;
+  int ti, tj;
+  tile_coords(tl.t, tiles_i, tiles_j, ti, tj);
+  tl.gi0 = ti * R - H;
+  tl.cb = tj * TJ - H;
+  tl.interior = tl.gi0 >= 1 && tl.gi0 + LR - 1 <= nx - 2 && tl.cb >= 1 && tl.cb + SCOLS - 1 <= ny - 2;
+  return true;
+}
+
+// ---------------------------------------------------------------------------
+// Descent
+// ---------------------------------------------------------------------------
+template <typename T, int NS, bool ZI, bool EQ, bool PW2, bool EDGE>
+__device__ __forceinline__ void sdown_body(const LevelView<T> &a, const STile &tl, T *__restrict__ rhsc, int ncx,
+                                           int ncy, int pc) {
+  constexpr int S = 2 * NS, H = s_down_halo(NS), P = s_period(NS), R = s_rows(H, P), TJ = SCOLS - 2 * H;
+  constexpr int LR = R + 2 * H;
+  static_assert(LR % P == 0, "staged rows must be whole loop bodies");
+  const int gi0 = tl.gi0, cb = tl.cb;
+  const Lane L = make_lane<T>(cb, H, TJ, a.ny, a.pitch);
+  SweepState<T, NS> st;
+  st.zero();
+  T RR[3][2];  // residual of rows (slot p % 3); element 1 only on even rows
+#pragma unroll
+  for (int k = 0; k < 3; ++k) RR[k][0] = RR[k][1] = T(0);
+  // rows 0 and 1 (the loop prefetches q + 2)
+  static_for<-2, 0>([&](auto jc) {
+    constexpr int J = decltype(jc)::value;
+    s_prefetch<T, NS, ZI, EDGE, J>(st, a, L, gi0, J, LR);
+  });
+  const int Jc = L.m + cb / 2;  // coarse column of element 0 (fine column 2 Jc)
+  const bool oc_c = EDGE ? L.oc && Jc >= 1 && Jc <= ncy - 2 : L.oc;
+  T *const rc = rhsc + Jc;
+#pragma unroll 1
+  for (int q0 = 0; q0 < LR; q0 += P) {
+    static_for<0, P>([&](auto jc) {
+      constexpr int J = decltype(jc)::value;
+      const int q = q0 + J;
+      s_prefetch<T, NS, ZI, EDGE, J>(st, a, L, gi0, q, LR);
+      // level 0 of row q: red element (J & 1), black element
+      constexpr int er = J & 1;
+      st.U[0][md<6>(J)] = st.RU[md<3>(J)][er];
+      st.U[1][md<6>(J)] = st.RU[md<3>(J)][1 - er];
+      s_sweeps<T, NS, EQ, PW2, EDGE, J>(st, a, L, gi0, q);
+      // store row q - S
+      {
+        const int p = q - S, gi = gi0 + p;
+        bool go = p >= H && p < H + R;
+        if constexpr (EDGE) go = go && gi >= 1 && gi <= a.nx - 2;
+        if (go) s_store<T, EDGE>(a.uout, a.pitch, gi, L, st.template full<S, J, S, 0>(), st.template full<S, J, S, 1>());
+      }
+      // residual of row q - S - 1
+      {
+        constexpr int d = S + 1;
+        constexpr int pr = J - d;
+        const T f0 = st.F[md<P>(pr)][0], f1 = st.F[md<P>(pr)][1];
+        const T x0 = st.template full<S, J, d, 0>(), x1 = st.template full<S, J, d, 1>();
+        const T a0 = st.template full<S, J, d + 1, 0>(), b0 = st.template full<S, J, d - 1, 0>();
+        RR[md<3>(pr)][0] = s_res<T, PW2>(x0, a0, b0, from_left(x1), x1, f0, a);
+        if constexpr (md<2>(pr) == 0) {
+          const T a1 = st.template full<S, J, d + 1, 1>(), b1 = st.template full<S, J, d - 1, 1>();
+          RR[md<3>(pr)][1] = s_res<T, PW2>(x1, a1, b1, x0, from_right(x0), f1, a);
+        }
+      }
+      // restriction at even row q - S - 2 (MultiGridScheme.cs:158-167)
+      {
+        constexpr int px = J - S - 2;
+        if constexpr (md<2>(px) == 0) {
+          const int p = q - S - 2;
+          const T r0 = RR[md<3>(px)][0], rjp = RR[md<3>(px)][1];
+          const T rjm = from_left(rjp);
+          const T rim = RR[md<3>(px - 1)][0], rip = RR[md<3>(px + 1)][0];
+          const int I = (gi0 + p) / 2;
+          bool go = p >= H && p < H + R;
+          if constexpr (EDGE) go = go && I >= 1 && I <= ncx - 2;
+          if (go && oc_c) rc[I * pc] = T(0.125) * (r0 * T(4) + (rip + rim + rjp + rjm));
+        }
+      }
+    });
+  }
+}
+
+template <typename T, int NS, bool ZI, bool EQ>
+__global__ void __launch_bounds__(64) k_sdown(LevelView<T> a, T *__restrict__ rhsc, int ncx, int ncy, int pc) {
+  constexpr int H = s_down_halo(NS), R = s_rows(H, s_period(NS));
+  STile tl;
+  if (!s_tile<R, H>(a.tiles_i, a.tiles_j, a.chunk, a.nx, a.ny, tl)) return;
+  if (a.pw2) {
+    if (tl.interior) sdown_body<T, NS, ZI, EQ, true, false>(a, tl, rhsc, ncx, ncy, pc);
+    else sdown_body<T, NS, ZI, EQ, true, true>(a, tl, rhsc, ncx, ncy, pc);
+  } else {
+    sdown_body<T, NS, ZI, EQ, false, true>(a, tl, rhsc, ncx, ncy, pc);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Ascent
+// ---------------------------------------------------------------------------
+template <typename T, int NS, int EPI, bool EQ, bool PW2, bool EDGE>
+__device__ __forceinline__ void sup_body(const LevelView<T> &a, const STile &tl, const T *__restrict__ uc, int ncx,
+                                         int ncy, int pc, double *__restrict__ part) {
+  constexpr int S = 2 * NS, H = s_up_halo(NS, EPI), P = s_period(NS), R = s_rows(H, P), TJ = SCOLS - 2 * H;
+  constexpr int LR = R + 2 * H;
+  static_assert(LR % P == 0, "staged rows must be whole loop bodies");
+  const int gi0 = tl.gi0, cb = tl.cb;
+  const Lane L = make_lane<T>(cb, H, TJ, a.ny, a.pitch);
+  SweepState<T, NS> st;
+  st.zero();
+  // coarse rows kk = 0, 1, ... of the strip: global row gi0/2 + kk, column
+  // Jc = cb/2 + m (fine column gj = 2 Jc); ring slot kk % 3.  The right
+  // coarse neighbour Jc + 1 comes from lane m + 1; lane 63 loads its own (CR),
+  // so the staged strip is exact after prolongation up to its last column.
+  T C[3], CR[3];
+  const int Jc = cb / 2 + L.m;
+  const bool last = L.m == 63;
+  const bool ldc = EDGE ? Jc >= 0 && Jc <= ncy - 1 : true;
+  const bool ldr = last && (EDGE ? Jc + 1 >= 0 && Jc + 1 <= ncy - 1 : true);
+  const T *const ucl = uc + (ldc ? Jc : 0);
+  const T *const ucr = uc + (ldr ? Jc + 1 : 0);
+  auto load_coarse = [&](int kk, T &v, T &vr) {
+    const int I = gi0 / 2 + kk;
+    v = T(0);
+    vr = T(0);
+    if (I >= 0 && I <= ncx - 1) {
+      if (ldc) v = ucl[I * pc];
+      if (ldr) vr = ucr[I * pc];
+    }
+  };
+  load_coarse(0, C[0], CR[0]);
+  load_coarse(1, C[1], CR[1]);
+  C[2] = CR[2] = T(0);
+  // EPI 2 keeps the staged (post-prolongation) pair of each row until its
+  // last half-sweep
+  constexpr int NV = EPI == 2 ? P : 1;
+  T V0[NV][2];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) V0[k][0] = V0[k][1] = T(0);
+  double acc = 0.0;
+  static_for<-2, 0>([&](auto jc) {
+    constexpr int J = decltype(jc)::value;
+    s_prefetch<T, NS, false, EDGE, J>(st, a, L, gi0, J, LR);
+  });
+#pragma unroll 1
+  for (int q0 = 0; q0 < LR; q0 += P) {
+    static_for<0, P>([&](auto jc) {
+      constexpr int J = decltype(jc)::value;
+      const int q = q0 + J;
+      s_prefetch<T, NS, false, EDGE, J>(st, a, L, gi0, q, LR);
+      constexpr int kk = J >> 1;  // coarse row of this fine row (phase)
+      if constexpr ((J & 1) == 0) load_coarse((q >> 1) + 2, C[md<3>(kk + 2)], CR[md<3>(kk + 2)]);
+      // staged row q + prolongation on interior nodes
+      {
+        const int gi = gi0 + q;
+        T v0 = st.RU[md<3>(J)][0], v1 = st.RU[md<3>(J)][1];
+        const T c00 = C[md<3>(kk)];
+        const T c01r = from_right(c00);
+        const T c01 = last ? CR[md<3>(kk)] : c01r;
+        T p0, p1;
+        if constexpr ((J & 1) == 0) {
+          p0 = c00;
+          p1 = (c00 + c01) * T(0.5);
+        } else {
+          const T c10 = C[md<3>(kk + 1)];
+          const T c11r = from_right(c10);
+          const T c11 = last ? CR[md<3>(kk + 1)] : c11r;
+          p0 = (c00 + c10) * T(0.5);
+          p1 = (c00 + c01 + c10 + c11) * T(0.25);
+        }
+        if constexpr (EDGE) {
+          const bool rin = gi >= 1 && gi <= a.nx - 2;
+          if (rin && L.in0) v0 = v0 + p0;
+          if (rin && L.in1) v1 = v1 + p1;
+        } else {
+          v0 = v0 + p0;
+          v1 = v1 + p1;
+        }
+        constexpr int er = J & 1;
+        st.U[0][md<6>(J)] = er == 0 ? v0 : v1;
+        st.U[1][md<6>(J)] = er == 0 ? v1 : v0;
+        if constexpr (EPI == 2) {
+          V0[md<NV>(J)][0] = v0;
+          V0[md<NV>(J)][1] = v1;
+        }
+      }
+      s_sweeps<T, NS, EQ, PW2, EDGE, J>(st, a, L, gi0, q);
+      // store row q - S (+ update metric)
+      {
+        const int p = q - S, gi = gi0 + p;
+        const T x0 = st.template full<S, J, S, 0>(), x1 = st.template full<S, J, S, 1>();
+        bool orow = p >= H && p < H + R;
+        if constexpr (EDGE) orow = orow && gi >= 1 && gi <= a.nx - 2;
+        if (orow) s_store<T, EDGE>(a.uout, a.pitch, gi, L, x0, x1);
+        if constexpr (EPI == 2) {
+          constexpr int pv = md<NV>(J - S);
+          if (orow && L.oc) {
+            if (!EDGE || L.in0) acc = nanmax(acc, fabs((double)(x0 - V0[pv][0])));
+            if (!EDGE || L.in1) acc = nanmax(acc, fabs((double)(x1 - V0[pv][1])));
+          }
+        }
+      }
+      // residual norm of row q - S - 1
+      if constexpr (EPI == 1) {
+        constexpr int d = S + 1;
+        constexpr int pr = J - d;
+        const int p = q - d, gi = gi0 + p;
+        const T x0 = st.template full<S, J, d, 0>(), x1 = st.template full<S, J, d, 1>();
+        const T a0 = st.template full<S, J, d + 1, 0>(), b0 = st.template full<S, J, d - 1, 0>();
+        const T a1 = st.template full<S, J, d + 1, 1>(), b1 = st.template full<S, J, d - 1, 1>();
+        const T xl = from_left(x1), xr = from_right(x0);
+        bool orow = p >= H && p < H + R;
+        if constexpr (EDGE) orow = orow && gi >= 1 && gi <= a.nx - 2;
+        if (orow && L.oc) {
+          const double r0 = (double)s_res<T, PW2>(x0, a0, b0, xl, x1, st.F[md<P>(pr)][0], a);
+          const double r1 = (double)s_res<T, PW2>(x1, a1, b1, x0, xr, st.F[md<P>(pr)][1], a);
+          if (!EDGE || L.in0) acc += r0 * r0;
+          if (!EDGE || L.in1) acc += r1 * r1;
+        }
+      }
+    });
+  }
+  if constexpr (EPI == 1) {
+    acc = wave_sum(acc);
+    if (L.m == 0) part[tl.t] = acc;
+  } else if constexpr (EPI == 2) {
+    acc = wave_max(acc);
+    if (L.m == 0) part[tl.t] = acc;
+  }
+}
+
+template <typename T, int NS, int EPI, bool EQ>
+__global__ void __launch_bounds__(64) k_sup(LevelView<T> a, const T *__restrict__ uc, int ncx, int ncy, int pc,
+                                            double *__restrict__ part) {
+  constexpr int H = s_up_halo(NS, EPI), R = s_rows(H, s_period(NS));
+  STile tl;
+  if (!s_tile<R, H>(a.tiles_i, a.tiles_j, a.chunk, a.nx, a.ny, tl)) return;
+  if (a.pw2) {
+    if (tl.interior) sup_body<T, NS, EPI, EQ, true, false>(a, tl, uc, ncx, ncy, pc, part);
+    else sup_body<T, NS, EPI, EQ, true, true>(a, tl, uc, ncx, ncy, pc, part);
+  } else {
+    sup_body<T, NS, EPI, EQ, false, true>(a, tl, uc, ncx, ncy, pc, part);
+  }
+}
+
+}  // namespace vlp2d
