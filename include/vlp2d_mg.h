@@ -48,7 +48,7 @@ extern "C" {
 #define VLP2D_FLAG_NO_GRAPH 1u   /* launch kernels directly, no hipGraph replay   */
 #define VLP2D_FLAG_PROFILE 2u    /* time every kernel class with hipEvents        */
 #define VLP2D_FLAG_SIMPLE 4u     /* one operator per launch (no fused kernels)    */
-#define VLP2D_FLAG_FUSED_REFINE 8u /* mixed: FP64 update inside the FP32 ascent  */
+#define VLP2D_FLAG_SEPARATE_REFINE 8u /* mixed: FP64 update as its own pass      */
 
 typedef struct vlp2d_mg_ctx *vlp2d_mg_handle;
 

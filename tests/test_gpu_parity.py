@@ -72,7 +72,7 @@ KERNEL_PATHS = {
     "one_workgroup_all": {"coarse_nodes": 1 << 30},  # whole cycle in k_coarse
     "one_op_per_launch": {"flags": _lib.FLAG_SIMPLE},
     "no_graph": {"flags": _lib.FLAG_NO_GRAPH},
-    "fused_refine": {"flags": _lib.FLAG_FUSED_REFINE},
+    "separate_refine": {"flags": _lib.FLAG_SEPARATE_REFINE},
 }
 
 

@@ -33,9 +33,6 @@
 #include "mg_refine.hpp"
 #include "mg_stream.hpp"
 
-#ifndef VLP2D_REFINE_STREAM
-#define VLP2D_REFINE_STREAM 1
-#endif
 
 using namespace vlp2d;
 
@@ -120,6 +117,8 @@ struct vlp2d_mg_ctx {
   std::vector<double *> ckpt;
   int K = 0;
   std::map<int, std::pair<hipGraph_t, hipGraphExec_t>> graphs;
+  int num_cu = 0;                    // compute units of the device
+  std::map<const void *, int> occ;   // resident 64-thread workgroups per GPU, per kernel
   // profiling
   bool profile = false;
   std::vector<PendingEv> pend;
@@ -294,9 +293,7 @@ template <typename T> void vcycle_simple(Ctx *c, Hier<T> &H, bool zero0, double 
 }
 
 // ---- fused kernels -------------------------------------------------------------
-// Operands of one fused launch; the tile grid depends on the kernel's halo H
-// (tiles of tile_rows<T>() x (128 - 2H) output nodes).  The launch grid is
-// 8 * chunk workgroups (see tile_of_block).
+// Operands of one fused launch (the tile grid is set by sview).
 template <typename T> LevelView<T> view(const Lvl<T> &v, const T *uin, T *uout, int H) {
   LevelView<T> a;
   a.uin = uin;
@@ -313,17 +310,36 @@ template <typename T> LevelView<T> view(const Lvl<T> &v, const T *uin, T *uout, 
   a.ihx2 = v.ihx2;
   a.ihy2 = v.ihy2;
   a.pw2 = v.pw2 ? 1 : 0;
-  a.tiles_i = (v.nx - 2 + tile_rows<T>() - 1) / tile_rows<T>();
-  a.tiles_j = (v.ny - 2 + tile_cols(H) - 1) / tile_cols(H);
-  a.chunk = (a.tiles_i * a.tiles_j + 7) / 8;
+  a.tiles_i = a.tiles_j = a.chunk = a.rows = 0;
   return a;
 }
 
-// Streaming-kernel view: tile grid of k_sdown / k_sup with halo H and R
-// output rows per tile (64-thread workgroups, 8 * chunk of them).
-template <typename T> LevelView<T> sview(const Lvl<T> &v, const T *uin, T *uout, int H, int R) {
+// Resident 64-thread workgroups (wavefronts) of a kernel on the whole GPU.
+int resident_waves(Ctx *c, const void *fn) {
+  auto it = c->occ.find(fn);
+  if (it != c->occ.end()) return it->second;
+  int nb = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, 64, 0) != hipSuccess || nb < 1) {
+    (void)hipGetLastError();
+    nb = 4;
+  }
+  const int q = nb * std::max(1, c->num_cu);
+  c->occ[fn] = q;
+  return q;
+}
+
+// Streaming-kernel view: tiles of R output rows by 128 - 2H columns, R
+// chosen so that the tiles fill the resident wavefront slots about once (one
+// wave of tiles: each staged halo row is re-read once per tile boundary, and
+// no tail of late tiles).  The staged rows R + 2H are a multiple of P.
+template <typename T>
+LevelView<T> sview(Ctx *c, const void *fn, const Lvl<T> &v, const T *uin, T *uout, int H, int P) {
   LevelView<T> a = view(v, uin, uout, H);
+  const int tj = (v.ny - 2) / (SCOLS - 2 * H) + 1;
+  const int ti = std::max(1, resident_waves(c, fn) / tj);
+  const int R = s_rows_at_least((v.nx - 2) / ti + 1, H, P);
   const STileGrid g = s_grid(v.nx, v.ny, R, H);
+  a.rows = R;
   a.tiles_i = g.tiles_i;
   a.tiles_j = g.tiles_j;
   a.chunk = g.chunk;
@@ -332,16 +348,12 @@ template <typename T> LevelView<T> sview(const Lvl<T> &v, const T *uin, T *uout,
 
 template <typename T, int NS>
 void launch_down_ns(Ctx *c, Lvl<T> &f, Lvl<T> &g, bool zi) {
-  constexpr int H = s_down_halo(NS), R = s_rows(H, s_period(NS));
-  const LevelView<T> a = sview(f, f.u, f.u2, H, R);
-  const dim3 grid(8 * a.chunk), blk(64);
-  if (zi) {
-    if (a.eq) hipLaunchKernelGGL((k_sdown<T, NS, true, true>), grid, blk, 0, c->st, a, g.rhs, g.nx, g.ny, g.pitch);
-    else hipLaunchKernelGGL((k_sdown<T, NS, true, false>), grid, blk, 0, c->st, a, g.rhs, g.nx, g.ny, g.pitch);
-  } else {
-    if (a.eq) hipLaunchKernelGGL((k_sdown<T, NS, false, true>), grid, blk, 0, c->st, a, g.rhs, g.nx, g.ny, g.pitch);
-    else hipLaunchKernelGGL((k_sdown<T, NS, false, false>), grid, blk, 0, c->st, a, g.rhs, g.nx, g.ny, g.pitch);
-  }
+  constexpr int H = s_down_halo(NS), P = s_period(NS);
+  using K = void (*)(LevelView<T>, T *, int, int, int);
+  const K k = zi ? (f.eq ? (K)k_sdown<T, NS, true, true> : (K)k_sdown<T, NS, true, false>)
+                 : (f.eq ? (K)k_sdown<T, NS, false, true> : (K)k_sdown<T, NS, false, false>);
+  const LevelView<T> a = sview(c, (const void *)k, f, f.u, f.u2, H, P);
+  hipLaunchKernelGGL(k, dim3(8 * a.chunk), dim3(64), 0, c->st, a, g.rhs, g.nx, g.ny, g.pitch);
 }
 
 template <typename T> void down(Ctx *c, Lvl<T> &f, Lvl<T> &g, int lvl, bool zi) {
@@ -356,27 +368,13 @@ template <typename T> void down(Ctx *c, Lvl<T> &f, Lvl<T> &g, int lvl, bool zi) 
 }
 
 template <typename T, int NS, int EPI>
-int launch_sup(Ctx *c, Lvl<T> &f, const Lvl<T> &g) {
-  constexpr int H = s_up_halo(NS, EPI), R = s_rows(H, s_period(NS));
-  const LevelView<T> a = sview(f, f.u2, f.u, H, R);
-  const dim3 grid(8 * a.chunk), blk(64);
-  if (a.eq) hipLaunchKernelGGL((k_sup<T, NS, EPI, true>), grid, blk, 0, c->st, a, g.u, g.nx, g.ny, g.pitch, c->part);
-  else hipLaunchKernelGGL((k_sup<T, NS, EPI, false>), grid, blk, 0, c->st, a, g.u, g.nx, g.ny, g.pitch, c->part);
+int launch_sup(Ctx *c, Lvl<T> &f, const Lvl<T> &g, const MixView &mx) {
+  constexpr int H = s_up_halo(NS, EPI), P = s_period(NS);
+  using K = void (*)(LevelView<T>, const T *, int, int, int, double *, MixView);
+  const K k = f.eq ? (K)k_sup<T, NS, EPI, true> : (K)k_sup<T, NS, EPI, false>;
+  const LevelView<T> a = sview(c, (const void *)k, f, f.u2, f.u, H, P);
+  hipLaunchKernelGGL(k, dim3(8 * a.chunk), dim3(64), 0, c->st, a, g.u, g.nx, g.ny, g.pitch, c->part, mx);
   return a.tiles_i * a.tiles_j;
-}
-
-// Fused refinement epilogue (FP32 level 0 of a mixed step) on the LDS-tiled
-// ascent kernel.
-template <typename T, int NS>
-int launch_up_mixed(Ctx *c, Lvl<T> &f, const Lvl<T> &g, const MixView &mx) {
-  if constexpr (sizeof(T) == 4) {
-    const LevelView<T> a = view(f, f.u2, f.u, up_halo(NS, EPI_MIXED));
-    const int grid = 8 * a.chunk;
-    hipLaunchKernelGGL((k_up<T, NS, EPI_MIXED>), dim3(grid), dim3(TNT), 0, c->st, a, g.u, g.pitch, c->part, mx);
-    return grid;
-  } else {
-    return 0;
-  }
 }
 
 // FP64 levels run epilogues none / residual norm / update norm; FP32 levels
@@ -384,13 +382,13 @@ int launch_up_mixed(Ctx *c, Lvl<T> &f, const Lvl<T> &g, const MixView &mx) {
 template <typename T, int NS> int launch_up_ns(Ctx *c, Lvl<T> &f, const Lvl<T> &g, int epi, const MixView &mx) {
   if constexpr (sizeof(T) == 8) {
     switch (epi) {
-      case EPI_RESNORM: return launch_sup<T, NS, EPI_RESNORM>(c, f, g);
-      case EPI_METRIC: return launch_sup<T, NS, EPI_METRIC>(c, f, g);
-      default: return launch_sup<T, NS, EPI_NONE>(c, f, g);
+      case EPI_RESNORM: return launch_sup<T, NS, EPI_RESNORM>(c, f, g, mx);
+      case EPI_METRIC: return launch_sup<T, NS, EPI_METRIC>(c, f, g, mx);
+      default: return launch_sup<T, NS, EPI_NONE>(c, f, g, mx);
     }
   } else {
-    if (epi == EPI_MIXED) return launch_up_mixed<T, NS>(c, f, g, mx);
-    return launch_sup<T, NS, EPI_NONE>(c, f, g);
+    if (epi == EPI_MIXED) return launch_sup<T, NS, EPI_MIXED>(c, f, g, mx);
+    return launch_sup<T, NS, EPI_NONE>(c, f, g, mx);
   }
 }
 
@@ -524,33 +522,24 @@ void mixed_step(Ctx *c, Lvl<double> &P) {
   mx.ihy2 = P.ihy2;
   mx.pw2 = P.pw2 ? 1 : 0;
   mx.part_max = c->part2;
-  if (c->o.flags & VLP2D_FLAG_FUSED_REFINE) {
-    // refinement update inside the level-0 ascent of the FP32 cycle
+  if (!(c->o.flags & VLP2D_FLAG_SEPARATE_REFINE) && z.pw2 == P.pw2 && z.pitch == P.pitch) {
+    // refinement update inside the level-0 ascent of the FP32 cycle (the
+    // kernel evaluates both precisions' divisions the same way, as
+    // multiplications when the steps are powers of two)
     const int np = vcycle_fused(c, Z, true, EPI_MIXED, mx, nullptr);
     reduce_part(c, c->part2, np, 0, true);
     reduce_part(c, c->part, np, 1, false);
     return;
   }
   vcycle_fused(c, Z, true, EPI_NONE, mx, nullptr);  // correction e -> z.u
-  int np = 0;
-  if (VLP2D_REFINE_STREAM) {
-    const int rows = refine_row_blocks(P.nx);
-    np = refine_strips(P.ny) * rows;
-    const int chunk = (np + 7) / 8;
-    launch(c, KC_UPDATE, 0, 32.0 * nodes(P), [&] {
-      hipLaunchKernelGGL(k_refine_stream, dim3(8 * chunk), dim3(256), 0, c->st, P.u, P.u2, z.u, P.rhs, z.rhs2, P.nx,
-                         P.ny, P.pitch, (double)P.hx2, (double)P.hy2, (double)P.ihx2, (double)P.ihy2, P.pw2 ? 1 : 0,
-                         rows, np, chunk, c->part, c->part2);
-    });
-  } else {
-    const int ti = (P.nx - 2 + RF_TI - 1) / RF_TI, tj = (P.ny - 2 + RF_TJ - 1) / RF_TJ;
-    np = ti * tj;
-    launch(c, KC_UPDATE, 0, 32.0 * nodes(P), [&] {
-      hipLaunchKernelGGL(k_refine, dim3(np), dim3(TNT), 0, c->st, P.u, P.u2, z.u, P.rhs, z.rhs2, P.nx, P.ny, P.pitch,
-                         (double)P.hx2, (double)P.hy2, (double)P.ihx2, (double)P.ihy2, P.pw2 ? 1 : 0, ti, c->part,
-                         c->part2);
-    });
-  }
+  const int rows = refine_row_blocks(P.nx);
+  const int np = refine_strips(P.ny) * rows;
+  const int chunk = (np + 7) / 8;
+  launch(c, KC_UPDATE, 0, 32.0 * nodes(P), [&] {
+    hipLaunchKernelGGL(k_refine_stream, dim3(8 * chunk), dim3(256), 0, c->st, P.u, P.u2, z.u, P.rhs, z.rhs2, P.nx,
+                       P.ny, P.pitch, (double)P.hx2, (double)P.hy2, (double)P.ihx2, (double)P.ihy2, P.pw2 ? 1 : 0, rows,
+                       np, chunk, c->part, c->part2);
+  });
   reduce_part(c, c->part2, np, 0, true);
   reduce_part(c, c->part, np, 1, false);
 }
@@ -625,9 +614,12 @@ template <typename T> void level_params(Lvl<T> &v, T hx0, T hy0, int segX, int s
   }
 }
 
+// Levels at or below this size run in the one-workgroup coarse kernel; larger
+// ones in the streaming kernels (measured on MI355X at 8193^2: 33^2 is the
+// fastest split, a streaming launch on a small level costs ~10 us).
 int coarse_threshold(const Ctx *c) {
   if (c->o.coarse_nodes > 0) return c->o.coarse_nodes;
-  return 129 * 129;
+  return 33 * 33;
 }
 
 template <typename T> void shape_hier(Ctx *c, Hier<T> &H, T hx0, T hy0) {
@@ -933,7 +925,7 @@ int vlp2d_mg_create(int segX, int segY, double hx, double hy, const double *f, d
   c->laplace = (f == nullptr);
   c->o = o;
   c->profile = (o.flags & VLP2D_FLAG_PROFILE) != 0;
-  // fused kernels are instantiated for 0..3 sweeps per pass
+  // fused kernels are instantiated for 1..3 sweeps per pass
   c->simple = (o.flags & VLP2D_FLAG_SIMPLE) != 0 || o.nu1 < 1 || o.nu2 < 1 || o.nu1 > 3 || o.nu2 > 3;
   c->L = std::min((int)std::log2((double)segX + 0.5), (int)std::log2((double)segY + 0.5));
   auto bail = [&](int rc) {
@@ -949,6 +941,10 @@ int vlp2d_mg_create(int segX, int segY, double hx, double hy, const double *f, d
     c->err = "hipStreamCreate failed";
     return bail(VLP2D_E_DEVICE);
   }
+  if (hipDeviceGetAttribute(&c->num_cu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) {
+    (void)hipGetLastError();
+    c->num_cu = 0;
+  }
   shape_hier(c, c->h64, hx, hy);
   Lvl<double> &v = c->h64.lv[0];
   v.lap = c->laplace;
@@ -960,12 +956,13 @@ int vlp2d_mg_create(int segX, int segY, double hx, double hy, const double *f, d
   fill(c, v.u2, v.elems(), 0.0);
   fill(c, v.rhs, v.elems(), 0.0);
   {
-    // partials of the largest launch grid: FP64 tiles of 32 rows, >= 100
-    // columns; refinement tiles of RF_TI x RF_TJ or of the streaming kernel
-    long long tiles = (long long)((nx + 31) / 32) * ((ny + 99) / 100);
-    long long rtiles = (long long)((nx + RF_TI - 1) / RF_TI) * ((ny + RF_TJ - 1) / RF_TJ);
-    long long stiles = (long long)refine_strips(ny) * refine_row_blocks(nx);
-    c->part_cap = (int)std::max<long long>(1 << 16, std::max(std::max(tiles, rtiles), stiles) + 64);
+    // partials: one per tile of the largest launch.  Streaming tiles number
+    // at most max(resident wavefronts, column strips); refinement tiles are
+    // 512 x 64.
+    const long long strips = (ny - 2) / 112 + 1;
+    long long stiles = std::max<long long>(strips, 64LL * std::max(1, c->num_cu));
+    long long rtiles = (long long)refine_strips(ny) * refine_row_blocks(nx);
+    c->part_cap = (int)std::max<long long>(1 << 16, std::max(stiles, rtiles) + 64);
   }
   if ((rc = dalloc(c, &c->part, (size_t)c->part_cap)) != VLP2D_OK) return bail(rc);
   if ((rc = dalloc(c, &c->part2, (size_t)c->part_cap)) != VLP2D_OK) return bail(rc);

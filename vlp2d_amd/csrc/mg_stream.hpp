@@ -1,7 +1,8 @@
 // Register-streaming fused multigrid kernels for gfx950.
 //
 // One wavefront (one 64-thread workgroup) owns a tile of R output rows by
-// TJ = 128 - 2H output columns of one level.  It stages a strip of 128
+// TJ = 128 - 2H output columns of one level (R is chosen per launch so that
+// the tiles fill the GPU's resident wavefront slots about once).  It stages a strip of 128
 // columns (lane m holds the column pair cb + 2m, cb + 2m + 1) and walks it
 // top to bottom, one staged row per step, LR = R + 2H rows in all.  Every
 // row is loaded from HBM once per tile; all red-black Gauss-Seidel
@@ -39,15 +40,18 @@
 //   k_sup   : staged iterate + additive bilinear prolongation of the coarse
 //             correction (MultiGridScheme.cs:171-186) -> S half-sweeps ->
 //             write the tile, with epilogue EPI 0 (none), 1 (partial sums of
-//             the squared residual; H = S + 2) or 2 (partial max of the
-//             update |u - u_after_prolongation|, MultiGridScheme.cs:69,111).
+//             the squared residual; H = S + 2), 2 (partial max of the update
+//             |u - u_after_prolongation|, MultiGridScheme.cs:69,111) or 3
+//             (FP32 correction e on level 0 of a mixed-precision step: the
+//             FP64 refinement u64 += e, r32 = (float)(fn - Lap u64), partial
+//             sums of r^2 and partial max |e|; e itself is not written;
+//             H = S + 2).
 #pragma once
+#include <algorithm>
+#include <type_traits>
+
 #include "mg_tiles.hpp"
 
-#ifndef VLP2D_SROWS
-#define VLP2D_SROWS 64
-#endif
-static_assert(VLP2D_SROWS >= 32, "partial-sum buffers are sized for tiles of at least 32 rows");
 
 namespace vlp2d {
 
@@ -58,12 +62,13 @@ constexpr int SCOLS = 128;  // staged columns per strip (64 lanes x 2)
 __host__ __device__ constexpr int s_period(int ns) { return 2 * ns + 4 <= 6 ? 6 : 12; }
 __host__ __device__ constexpr int s_down_halo(int ns) { return 2 * ns + 2; }
 __host__ __device__ constexpr int s_up_halo(int ns, int epi) {
-  return epi == 1 ? 2 * ns + 2 : (2 * ns < 2 ? 2 : 2 * ns);
+  return (epi == 1 || epi == 3) ? 2 * ns + 2 : (2 * ns < 2 ? 2 : 2 * ns);
 }
-// Output rows per tile: about VLP2D_SROWS, chosen so that the staged rows
-// R + 2H are a whole number of loop bodies.
-__host__ __device__ constexpr int s_rows(int H, int P) {
-  return ((VLP2D_SROWS + 2 * H + P - 1) / P) * P - 2 * H;
+// Output rows per tile R (a launch parameter): the staged rows R + 2H must
+// be a whole number of loop bodies; the smallest such R >= want.
+inline int s_rows_at_least(int want, int H, int P) {
+  const int lr = ((std::max(want, 2) + 2 * H + P - 1) / P) * P;
+  return lr - 2 * H;
 }
 
 struct STileGrid {
@@ -290,9 +295,10 @@ struct STile {
   int t, gi0, cb;
   bool interior;
 };
-template <int R, int H> __device__ __forceinline__ bool s_tile(int tiles_i, int tiles_j, int chunk, int nx, int ny,
-                                                               STile &tl) {
-  constexpr int TJ = SCOLS - 2 * H, LR = R + 2 * H;
+template <int H> __device__ __forceinline__ bool s_tile(int R, int tiles_i, int tiles_j, int chunk, int nx, int ny,
+                                                        STile &tl) {
+  constexpr int TJ = SCOLS - 2 * H;
+  const int LR = R + 2 * H;
   tl.t = tile_of_block(tiles_i * tiles_j, chunk);
   if (tl.t < 0) return false;
   int ti, tj;
@@ -309,9 +315,8 @@ template <int R, int H> __device__ __forceinline__ bool s_tile(int tiles_i, int 
 template <typename T, int NS, bool ZI, bool EQ, bool PW2, bool EDGE>
 __device__ __forceinline__ void sdown_body(const LevelView<T> &a, const STile &tl, T *__restrict__ rhsc, int ncx,
                                            int ncy, int pc) {
-  constexpr int S = 2 * NS, H = s_down_halo(NS), P = s_period(NS), R = s_rows(H, P), TJ = SCOLS - 2 * H;
-  constexpr int LR = R + 2 * H;
-  static_assert(LR % P == 0, "staged rows must be whole loop bodies");
+  constexpr int S = 2 * NS, H = s_down_halo(NS), P = s_period(NS), TJ = SCOLS - 2 * H;
+  const int R = a.rows, LR = R + 2 * H;  // LR is a multiple of P
   const int gi0 = tl.gi0, cb = tl.cb;
   const Lane L = make_lane<T>(cb, H, TJ, a.ny, a.pitch);
   SweepState<T, NS> st;
@@ -378,9 +383,9 @@ __device__ __forceinline__ void sdown_body(const LevelView<T> &a, const STile &t
 
 template <typename T, int NS, bool ZI, bool EQ>
 __global__ void __launch_bounds__(64) k_sdown(LevelView<T> a, T *__restrict__ rhsc, int ncx, int ncy, int pc) {
-  constexpr int H = s_down_halo(NS), R = s_rows(H, s_period(NS));
+  constexpr int H = s_down_halo(NS);
   STile tl;
-  if (!s_tile<R, H>(a.tiles_i, a.tiles_j, a.chunk, a.nx, a.ny, tl)) return;
+  if (!s_tile<H>(a.rows, a.tiles_i, a.tiles_j, a.chunk, a.nx, a.ny, tl)) return;
   if (a.pw2) {
     if (tl.interior) sdown_body<T, NS, ZI, EQ, true, false>(a, tl, rhsc, ncx, ncy, pc);
     else sdown_body<T, NS, ZI, EQ, true, true>(a, tl, rhsc, ncx, ncy, pc);
@@ -394,10 +399,9 @@ __global__ void __launch_bounds__(64) k_sdown(LevelView<T> a, T *__restrict__ rh
 // ---------------------------------------------------------------------------
 template <typename T, int NS, int EPI, bool EQ, bool PW2, bool EDGE>
 __device__ __forceinline__ void sup_body(const LevelView<T> &a, const STile &tl, const T *__restrict__ uc, int ncx,
-                                         int ncy, int pc, double *__restrict__ part) {
-  constexpr int S = 2 * NS, H = s_up_halo(NS, EPI), P = s_period(NS), R = s_rows(H, P), TJ = SCOLS - 2 * H;
-  constexpr int LR = R + 2 * H;
-  static_assert(LR % P == 0, "staged rows must be whole loop bodies");
+                                         int ncy, int pc, double *__restrict__ part, const MixView &mx) {
+  constexpr int S = 2 * NS, H = s_up_halo(NS, EPI), P = s_period(NS), TJ = SCOLS - 2 * H;
+  const int R = a.rows, LR = R + 2 * H;  // LR is a multiple of P
   const int gi0 = tl.gi0, cb = tl.cb;
   const Lane L = make_lane<T>(cb, H, TJ, a.ny, a.pitch);
   SweepState<T, NS> st;
@@ -431,7 +435,16 @@ __device__ __forceinline__ void sup_body(const LevelView<T> &a, const STile &tl,
   T V0[NV][2];
 #pragma unroll
   for (int k = 0; k < NV; ++k) V0[k][0] = V0[k][1] = T(0);
-  double acc = 0.0;
+  double acc = 0.0, emax = 0.0;
+  static_assert(EPI != 3 || sizeof(T) == 4, "the refinement epilogue runs on the FP32 correction");
+  // EPI 3: FP64 iterate rows (raw, then refined x = u64 + e) and FP64
+  // right-hand side; row r is loaded at step r + S - 2, refined at step
+  // r + S (when e of row r is final), used by the residuals of rows r-1..r+1.
+  constexpr int N3 = EPI == 3 ? 3 : 1;
+  double X64[N3][2], XR[N3][2], FN[N3][2];
+#pragma unroll
+  for (int k = 0; k < N3; ++k) X64[k][0] = X64[k][1] = XR[k][0] = XR[k][1] = FN[k][0] = FN[k][1] = 0.0;
+  const unsigned boff8 = L.ld ? (unsigned)L.gj * 8u : 0u;
   static_for<-2, 0>([&](auto jc) {
     constexpr int J = decltype(jc)::value;
     s_prefetch<T, NS, false, EDGE, J>(st, a, L, gi0, J, LR);
@@ -478,6 +491,22 @@ __device__ __forceinline__ void sup_body(const LevelView<T> &a, const STile &tl,
           V0[md<NV>(J)][1] = v1;
         }
       }
+      if constexpr (EPI == 3) {
+        // prefetch FP64 rows: iterate row q - S + 2, right-hand side row q - S + 1
+        const int r = q - S + 2, gr = gi0 + r;
+        Pair<double> x{0.0, 0.0}, f{0.0, 0.0};
+        bool go = r >= 0 && r < LR;
+        if constexpr (EDGE) go = go && gr >= 0 && gr <= a.nx - 1 && L.ld;
+        if (go) x = ld_pair(row_of(mx.u64in, gr, a.pitch), boff8);
+        const int rf = q - S + 1, gf = gi0 + rf;
+        bool gof = rf >= 0 && rf < LR;
+        if constexpr (EDGE) gof = gof && gf >= 0 && gf <= a.nx - 1 && L.ld;
+        if (gof) f = ld_pair(row_of(mx.fn64, gf, a.pitch), boff8);
+        X64[md<3>(J - S + 2)][0] = x.x0;
+        X64[md<3>(J - S + 2)][1] = x.x1;
+        FN[md<3>(J - S + 1)][0] = f.x0;
+        FN[md<3>(J - S + 1)][1] = f.x1;
+      }
       s_sweeps<T, NS, EQ, PW2, EDGE, J>(st, a, L, gi0, q);
       // store row q - S (+ update metric)
       {
@@ -485,12 +514,74 @@ __device__ __forceinline__ void sup_body(const LevelView<T> &a, const STile &tl,
         const T x0 = st.template full<S, J, S, 0>(), x1 = st.template full<S, J, S, 1>();
         bool orow = p >= H && p < H + R;
         if constexpr (EDGE) orow = orow && gi >= 1 && gi <= a.nx - 2;
-        if (orow) s_store<T, EDGE>(a.uout, a.pitch, gi, L, x0, x1);
+        if constexpr (EPI != 3) {
+          if (orow) s_store<T, EDGE>(a.uout, a.pitch, gi, L, x0, x1);
+        } else {
+          // refined FP64 row q - S: u64 + e on interior nodes
+          constexpr int sx = md<3>(J - S);
+          double r0 = X64[sx][0], r1 = X64[sx][1];
+          if constexpr (EDGE) {
+            const bool rin = gi >= 1 && gi <= a.nx - 2;
+            if (rin && L.in0) r0 = r0 + (double)x0;
+            if (rin && L.in1) r1 = r1 + (double)x1;
+          } else {
+            r0 = r0 + (double)x0;
+            r1 = r1 + (double)x1;
+          }
+          XR[sx][0] = r0;
+          XR[sx][1] = r1;
+        }
         if constexpr (EPI == 2) {
           constexpr int pv = md<NV>(J - S);
           if (orow && L.oc) {
             if (!EDGE || L.in0) acc = nanmax(acc, fabs((double)(x0 - V0[pv][0])));
             if (!EDGE || L.in1) acc = nanmax(acc, fabs((double)(x1 - V0[pv][1])));
+          }
+        }
+      }
+      // refinement of row q - S - 1: u64out, r32out = (float)(fn - Lap u64out)
+      // (UtilsOpLap.cs:15-20 order, FP64), partial sum r^2 and max |e|
+      if constexpr (EPI == 3) {
+        constexpr int d = S + 1;
+        constexpr int pr = J - d;
+        const int p = q - d, gi = gi0 + p;
+        const double x0 = XR[md<3>(pr)][0], x1 = XR[md<3>(pr)][1];
+        const double a0 = XR[md<3>(pr - 1)][0], a1 = XR[md<3>(pr - 1)][1];
+        const double b0 = XR[md<3>(pr + 1)][0], b1 = XR[md<3>(pr + 1)][1];
+        const double xl = from_left(x1), xr = from_right(x0);
+        const T e0 = st.template full<S, J, d, 0>(), e1 = st.template full<S, J, d, 1>();
+        bool orow = p >= H && p < H + R;
+        if constexpr (EDGE) orow = orow && gi >= 1 && gi <= a.nx - 2;
+        if (orow && L.oc) {
+          const double u20 = x0 * 2.0, u21 = x1 * 2.0;
+          double res0, res1;
+          if constexpr (PW2) {
+            res0 = FN[md<3>(pr)][0] - ((a0 + b0 - u20) * mx.ihx2 + (xl + x1 - u20) * mx.ihy2);
+            res1 = FN[md<3>(pr)][1] - ((a1 + b1 - u21) * mx.ihx2 + (x0 + xr - u21) * mx.ihy2);
+          } else {
+            res0 = FN[md<3>(pr)][0] - ((a0 + b0 - u20) / mx.hx2 + (xl + x1 - u20) / mx.hy2);
+            res1 = FN[md<3>(pr)][1] - ((a1 + b1 - u21) / mx.hx2 + (x0 + xr - u21) / mx.hy2);
+          }
+          const bool o0 = !EDGE || L.in0, o1 = !EDGE || L.in1;
+          double *urow = row_of(mx.u64out, gi, a.pitch);
+          float *rrow = row_of(mx.r32out, gi, a.pitch);
+          if (o0 && o1) {
+            st_pair(urow, boff8, x0, x1);
+            st_pair(rrow, L.boff, (float)res0, (float)res1);
+          } else if (o0) {
+            urow[L.gj] = x0;
+            rrow[L.gj] = (float)res0;
+          } else if (o1) {
+            urow[L.gj + 1] = x1;
+            rrow[L.gj + 1] = (float)res1;
+          }
+          if (o0) {
+            acc += res0 * res0;
+            emax = nanmax(emax, fabs((double)e0));
+          }
+          if (o1) {
+            acc += res1 * res1;
+            emax = nanmax(emax, fabs((double)e1));
           }
         }
       }
@@ -514,26 +605,32 @@ __device__ __forceinline__ void sup_body(const LevelView<T> &a, const STile &tl,
       }
     });
   }
-  if constexpr (EPI == 1) {
+  if constexpr (EPI == 1 || EPI == 3) {
     acc = wave_sum(acc);
     if (L.m == 0) part[tl.t] = acc;
   } else if constexpr (EPI == 2) {
     acc = wave_max(acc);
     if (L.m == 0) part[tl.t] = acc;
   }
+  if constexpr (EPI == 3) {
+    emax = wave_max(emax);
+    if (L.m == 0) mx.part_max[tl.t] = emax;
+  }
 }
 
+// EPI 3 requires the FP64 steps of mx to be powers of two exactly when the
+// FP32 steps of a are (the host checks).
 template <typename T, int NS, int EPI, bool EQ>
 __global__ void __launch_bounds__(64) k_sup(LevelView<T> a, const T *__restrict__ uc, int ncx, int ncy, int pc,
-                                            double *__restrict__ part) {
-  constexpr int H = s_up_halo(NS, EPI), R = s_rows(H, s_period(NS));
+                                            double *__restrict__ part, MixView mx) {
+  constexpr int H = s_up_halo(NS, EPI);
   STile tl;
-  if (!s_tile<R, H>(a.tiles_i, a.tiles_j, a.chunk, a.nx, a.ny, tl)) return;
+  if (!s_tile<H>(a.rows, a.tiles_i, a.tiles_j, a.chunk, a.nx, a.ny, tl)) return;
   if (a.pw2) {
-    if (tl.interior) sup_body<T, NS, EPI, EQ, true, false>(a, tl, uc, ncx, ncy, pc, part);
-    else sup_body<T, NS, EPI, EQ, true, true>(a, tl, uc, ncx, ncy, pc, part);
+    if (tl.interior) sup_body<T, NS, EPI, EQ, true, false>(a, tl, uc, ncx, ncy, pc, part, mx);
+    else sup_body<T, NS, EPI, EQ, true, true>(a, tl, uc, ncx, ncy, pc, part, mx);
   } else {
-    sup_body<T, NS, EPI, EQ, false, true>(a, tl, uc, ncx, ncy, pc, part);
+    sup_body<T, NS, EPI, EQ, false, true>(a, tl, uc, ncx, ncy, pc, part, mx);
   }
 }
 
