@@ -6,8 +6,9 @@ boundary.  One step = one complete solve from the zero interior guess until
 ||r||_2 <= 1e-10 ||r_0||_2, with the problem already resident in HBM (the
 iterate is restored on the device between steps, outside the timed region).
 
-Reported: the mixed-precision solver (FP32 V(2,2) cycles inside FP64
-iterative refinement) as ``value``; the plain FP64 V(2,2) multigrid on the same
+Reported: the mixed-precision solver (FP32 V(1,3) cycles inside FP64
+iterative refinement; --cycle picks another V(nu1,nu2)) as ``value``; the
+plain FP64 multigrid with the same cycle on the same
 problem beside it (``fp64``); the dominant kernel's roofline from hipEvent
 timings of a profiled solve; and the oracle's CPU multigrid on a bounded
 sample of the same workload.
@@ -52,12 +53,13 @@ def solve_times(scheme, steps, warmup, rtol):
     return wall, dev, cycles, rel
 
 
-def profile_solve(p, mixed, rtol):
+def profile_solve(p, mixed, rtol, nu):
     """One solve with every launch bracketed by hipEvents; returns per
     (kernel, level) launches / ms / algorithmic bytes."""
     from vlp2d_amd import MultiGridSchemeHIP, Options
     from vlp2d_amd._lib import FLAG_PROFILE
-    s = MultiGridSchemeHIP(p.seg_x, p.seg_y, p.hx, p.hy, p.f, options=Options.performance(mixed, flags=FLAG_PROFILE))
+    s = MultiGridSchemeHIP(p.seg_x, p.seg_y, p.hx, p.hy, p.f,
+                           options=Options.performance(mixed, nu1=nu[0], nu2=nu[1], flags=FLAG_PROFILE))
     s.un0[:] = p.initial("zero")
     s.init_after_boundaries_and_initial_iteration_inited()
     s.save_iterate()
@@ -70,11 +72,11 @@ def profile_solve(p, mixed, rtol):
     return kt
 
 
-def cpu_baseline(seg, rtol):
+def cpu_baseline(seg, rtol, nu):
     """Oracle (C++ restatement of the reference CPU multigrid, OpenMP over
-    rows) on the same formulas: FP64 V(2,2) to rtol.  Returns the baseline
-    record and the oracle's solution (the FP64 oracle the GPU result is
-    compared with)."""
+    rows) on the same formulas: FP64 V(nu1,nu2) to rtol.  Returns the
+    baseline record and the oracle's solution (the FP64 oracle the GPU result
+    is compared with)."""
     from oracle import Oracle
     from vlp2d_amd import problems
     p = problems.config(3, seg)
@@ -83,17 +85,17 @@ def cpu_baseline(seg, rtol):
     o = Oracle(p.seg_x, p.seg_y, p.hx, p.hy, p.f, "double", nthreads=cores)
     o.set_u(p.initial("zero"))
     t0 = time.perf_counter()
-    k, hist = o.plain_solve(2, 2, 3, rtol, 100)
+    k, hist = o.plain_solve(nu[0], nu[1], 3, rtol, 100)
     dt = time.perf_counter() - t0
     u = o.get_u()
     o.close()
     rec = {"value": round(p.unknowns / dt / 1e6, 2), "unit": "Munknowns/s", "cores": cores, "kind": "port",
-           "sample": f"oracle FP64 V(2,2) RB-GS multigrid on the {seg + 1}x{seg + 1} workload, "
+           "sample": f"oracle FP64 V({nu[0]},{nu[1]}) RB-GS multigrid on the {seg + 1}x{seg + 1} workload, "
                      f"{k} cycles to rel. residual {hist[-1]:.2e} in {dt:.2f} s"}
     return rec, u
 
 
-def dd128_check(seg=512, rtol=1e-10):
+def dd128_check(nu, seg=512, rtol=1e-10):
     """Error of the GPU mixed-precision solution of the 513^2 replica against
     the oracle's double-double multigrid converged to its floor."""
     import numpy as np
@@ -107,7 +109,7 @@ def dd128_check(seg=512, rtol=1e-10):
             break
     hi, lo = o.get_u(with_lo=True)
     o.close()
-    s = MultiGridSchemeHIP(p.seg_x, p.seg_y, p.hx, p.hy, p.f, options=Options.performance(True))
+    s = MultiGridSchemeHIP(p.seg_x, p.seg_y, p.hx, p.hy, p.f, options=Options.performance(True, nu1=nu[0], nu2=nu[1]))
     s.un0[:] = p.initial("zero")
     s.init_after_boundaries_and_initial_iteration_inited()
     s.solve(rtol, 100)
@@ -126,7 +128,10 @@ def main():
     ap.add_argument("--cpu-seg", type=int, default=8192)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--cycle", default="1,3",
+                    help="nu1,nu2 of the V-cycle (V(1,3): fewest cycles to 1e-10 on this workload, measured)")
     args = ap.parse_args()
+    nu = tuple(int(x) for x in args.cycle.split(","))
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               "(the multigrid path runs on one MI355X)", file=sys.stderr)
@@ -140,7 +145,8 @@ def main():
 
     res = {}
     for mixed in (True, False):
-        s = MultiGridSchemeHIP(p.seg_x, p.seg_y, p.hx, p.hy, p.f, options=Options.performance(mixed))
+        s = MultiGridSchemeHIP(p.seg_x, p.seg_y, p.hx, p.hy, p.f,
+                               options=Options.performance(mixed, nu1=nu[0], nu2=nu[1]))
         s.un0[:] = p.initial("zero")
         s.init_after_boundaries_and_initial_iteration_inited()
         s.save_iterate()
@@ -158,7 +164,7 @@ def main():
     roofline = None
     kernels = None
     if not args.no_profile:
-        kt = profile_solve(p, True, args.rtol)
+        kt = profile_solve(p, True, args.rtol, nu)
         tot = sum(e[3] for e in kt)
         dom = max(kt, key=lambda e: e[3])
         name, lvl, launches, ms, nbytes = dom
@@ -173,10 +179,10 @@ def main():
                     "alg_bytes_per_launch": per_launch_bytes, "share_of_solve": round(ms / tot, 3)}
         kernels = sorted(([e[0], e[1], e[2], round(e[3], 4), e[4]] for e in kt), key=lambda e: -e[3])[:12]
     # whole-solve accounting with the per-operator byte model (unfused operators)
-    solve_bytes = {m: alg_bytes_per_operator(p, m, res[m]["cycles"][0]) for m in (True, False)}
+    solve_bytes = {m: alg_bytes_per_operator(p, m, res[m]["cycles"][0], nu[0], nu[1]) for m in (True, False)}
     solve_roofline = {
-        "model": "per-operator algorithmic bytes (each operator one streaming pass; V(2,2): 25 s B/node/cycle, "
-                 "refinement 40 B/node/step, residual check 16 B/node)",
+        "model": "per-operator algorithmic bytes (each operator one streaming pass; V(nu1,nu2): "
+                 "(3(nu1+nu2)+6+3/4) s B/node/cycle, refinement 40 B/node/step, residual check 16 B/node)",
         "mixed": {"bytes": solve_bytes[True], "effective_GBps": round(solve_bytes[True] / (mixed_ms * 1e-3) / 1e9, 1),
                   "frac": round(solve_bytes[True] / (mixed_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
         "fp64": {"bytes": solve_bytes[False], "effective_GBps": round(solve_bytes[False] / (fp64_ms * 1e-3) / 1e9, 1),
@@ -187,12 +193,12 @@ def main():
                                                / np.abs(res[False]["u"]).max())}
     cpu = None
     if not args.no_cpu:
-        cpu, u_oracle = cpu_baseline(args.cpu_seg, args.rtol)
+        cpu, u_oracle = cpu_baseline(args.cpu_seg, args.rtol, nu)
         if args.cpu_seg == args.seg:
             scale = np.abs(u_oracle).max()
             accuracy["mixed_vs_fp64_oracle_rel"] = float(np.abs(res[True]["u"] - u_oracle).max() / scale)
             accuracy["fp64_gpu_vs_fp64_oracle_rel"] = float(np.abs(res[False]["u"] - u_oracle).max() / scale)
-        accuracy["mixed_513_vs_dd128_oracle_rel"] = dd128_check()
+        accuracy["mixed_513_vs_dd128_oracle_rel"] = dd128_check(nu)
 
     line = {
         "metric": "Munknowns/s to rel. residual 1e-10: FP32+FP64-refine vs FP64 MG; % HBM roofline",
@@ -210,7 +216,7 @@ def main():
         "config": {"workload": f"poisson_unit_square_{args.seg + 1}x{args.seg + 1}_gaussian_bumps64_"
                                f"mixed_fp32_vcycles_fp64_refinement_to_rel_residual_{args.rtol:g}",
                    "nodes": [p.seg_x + 1, p.seg_y + 1], "unknowns": p.unknowns,
-                   "cycle": "V(2,2) RB-GS, half-weighting restriction", "parallelism": "single GPU"},
+                   "cycle": f"V({nu[0]},{nu[1]}) RB-GS, half-weighting restriction", "parallelism": "single GPU"},
         "fp64": {"value": round(p.unknowns / (fp64_ms * 1e-3) / 1e6, 2), "ms_per_step": round(fp64_ms, 4),
                  "cycles": res[False]["cycles"][0]},
         "mixed_cycles": res[True]["cycles"][0],

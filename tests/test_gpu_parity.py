@@ -139,20 +139,23 @@ def test_config1_reference_stop_rule_cycle_count():
     s.cleanup()
 
 
+@pytest.mark.parametrize("nu", [(2, 2), (1, 3)])
 @pytest.mark.parametrize("mixed", [False, True])
-@pytest.mark.parametrize("cfg,seg", [(2, 512), (3, 256), (1, 128)])
-def test_performance_mode_matches_oracle(cfg, seg, mixed):
-    """V(2,2) to relative residual 1e-10 from a zero interior, FP64 or FP32
-    cycles in FP64 refinement: same number of cycles as the oracle and the
-    same iterate (bitwise), which is the DST-I solution to solver accuracy."""
+@pytest.mark.parametrize("cfg,seg", [(2, 512), (3, 256), (1, 128), (3, 1024)])
+def test_performance_mode_matches_oracle(cfg, seg, mixed, nu):
+    """V(nu1,nu2) to relative residual 1e-10 from a zero interior (V(2,2), and
+    the benchmark's V(1,3)), FP64 or FP32 cycles in FP64 refinement: same
+    number of cycles as the oracle and the same iterate (bitwise), which is
+    the DST-I solution to solver accuracy."""
     p = problems.config(cfg, seg)
-    _say(f"perf cfg{cfg} seg{seg} mixed={mixed}")
+    _say(f"perf cfg{cfg} seg{seg} mixed={mixed} nu={nu}")
     u0 = p.initial("zero")
     if mixed:
-        ur, kr, hr = mixed_solve(p.seg_x, p.seg_y, p.hx, p.hy, p.f, u0, 2, 2, 3, 1e-10, 60)
+        ur, kr, hr = mixed_solve(p.seg_x, p.seg_y, p.hx, p.hy, p.f, u0, nu[0], nu[1], 3, 1e-10, 60)
     else:
-        ur, kr, hr = plain_solve(p.seg_x, p.seg_y, p.hx, p.hy, p.f, u0, 2, 2, 3, 1e-10, 60)
-    s = MultiGridSchemeHIP(p.seg_x, p.seg_y, p.hx, p.hy, p.f, options=Options.performance(mixed))
+        ur, kr, hr = plain_solve(p.seg_x, p.seg_y, p.hx, p.hy, p.f, u0, nu[0], nu[1], 3, 1e-10, 60)
+    s = MultiGridSchemeHIP(p.seg_x, p.seg_y, p.hx, p.hy, p.f,
+                           options=Options.performance(mixed, nu1=nu[0], nu2=nu[1]))
     s.un0[:] = u0
     s.init_after_boundaries_and_initial_iteration_inited()
     st = s.solve(1e-10, 60)

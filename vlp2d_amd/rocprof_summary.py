@@ -4,15 +4,16 @@
     python -m vlp2d_amd.rocprof_summary pmc <fetch counter_collection.csv> \
         <write counter_collection.csv> --copy-bytes B
 
-``trace`` groups kernel dispatches by kernel and grid size (a grid size
-identifies a multigrid level) and prints launches, average and total
-duration as a Markdown table.
+``trace`` groups kernel dispatches by kernel and grid size (launches on
+different multigrid levels differ in grid size) and prints launches, average
+and total duration as a Markdown table.
 
 ``pmc`` averages FETCH_SIZE and WRITE_SIZE (KiB per dispatch) per kernel and
 grid size, calibrates both against the runtime's device-to-device copy
 kernel of known size (``--copy-bytes``, the bytes the copy reads and also
-writes), and prints the calibrated HBM traffic per launch.  For the largest
-grid of each solver kernel (level 0) it adds a line
+writes), and prints the calibrated HBM traffic per launch.  For the launch
+of each kernel class that moves the most bytes (level 0, whose grid holds four
+times the nodes of the next level) it adds a line
 
     TRAFFIC <class>@level0/<f32|f64> <bytes>
 
@@ -26,8 +27,8 @@ import sys
 
 # kernel function name -> launch class reported by vlp2d_mg_kernel_times
 KERNEL_CLASS = {
-    "k_down": "down_fused",
-    "k_up": "up_fused",
+    "k_sdown": "down_fused",
+    "k_sup": "up_fused",
     "k_coarse": "coarse_vcycle",
     "k_refine_stream": "update_mixed",
     "k_refine": "update_mixed",
@@ -130,10 +131,10 @@ def pmc_report(fetch_csv, write_csv, copy_bytes):
         if cls not in LEVEL0_CLASSES:
             continue
         tag = f"{cls}@level0/{prec}"
-        if tag not in level0 or k[1] > level0[tag][0]:
+        if tag not in level0 or b > level0[tag][1]:
             level0[tag] = (k[1], b)
     lines.append("")
-    lines.append("Level-0 traffic per launch (largest grid of each kernel class):")
+    lines.append("Level-0 traffic per launch (the most traffic of each kernel class):")
     lines.append("")
     lines.append("```")
     for tag, (_, b) in sorted(level0.items()):

@@ -33,8 +33,11 @@ class Options:
 
     @staticmethod
     def performance(mixed=False, **kw):
-        """V(2,2) cycles (relative-residual stop), FP64 or FP32-in-FP64."""
-        return Options(nu1=2, nu2=2, nu_coarse=3, precision=_lib.PREC_MIXED if mixed else _lib.PREC_FP64, **kw)
+        """V(2,2) cycles unless nu1/nu2 are given (relative-residual stop),
+        FP64 or FP32-in-FP64."""
+        base = dict(nu1=2, nu2=2, nu_coarse=3, precision=_lib.PREC_MIXED if mixed else _lib.PREC_FP64)
+        base.update(kw)
+        return Options(**base)
 
     def to_c(self):
         return _lib.Opts(self.nu1, self.nu2, self.nu_coarse, self.precision, self.flags, self.coarse_nodes)
