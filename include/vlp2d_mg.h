@@ -168,8 +168,66 @@ void vlp2d_mg_cancel(vlp2d_mg_handle h);
  * (MultiGridSchemeCU.cs:272-285).  NULL is accepted. */
 void vlp2d_mg_destroy(vlp2d_mg_handle h);
 
+/* ---- problem set-up on the device --------------------------------------
+ * Expressions of x and y in the syntax of the reference's input files
+ * (VLP2D_Data/InputData/<name>.txt, read by ViewModel/InputParser.cs:50-152):
+ * + - * / ^ (power binds tightest and is left-associative, as in
+ * Substitutor/OperationsRegistry.cs:64-84), unary -, c ? a : b, numbers,
+ * x, y, sin cos tan asin acos atan sinh cosh tanh exp ln log log10 sqrt abs,
+ * and pi(a) = pi*a (Common/Utils.cs:352-364).  The reference evaluates them
+ * with the binary-only ELW library through managed delegates; here they are
+ * compiled once and evaluated on the GPU.  Node coordinates are x = hx*i,
+ * y = hy*j. */
+
+/* Constructor with the right-hand side as an expression: fn = -f(hx*i, hy*j)
+ * is sampled on the GPU (no host array, no PCIe copy).  f_expr NULL or
+ * blank = Laplace.  Replaces the constructor's host sampling through fKsi
+ * (MultiGridSchemeCU.cs:41-45).  A syntax error returns VLP2D_E_ARG, a
+ * non-finite f VLP2D_E_NONFINITE (message in vlp2d_mg_last_error(NULL)). */
+int vlp2d_mg_create_expr(int segX, int segY, double hx, double hy, const char *f_expr, double eps, int device,
+                         const vlp2d_mg_opts *opts, vlp2d_mg_handle *out);
+
+/* Samples the Dirichlet data into the level-0 iterate on the GPU: bottom
+ * (j = 0) and top (j = ny-1) edges from the edge functions of x, or the
+ * boundary function at (x, 0) and (x, yMax); then left (i = 0) and right
+ * (i = nx-1) from the edge functions of y, or the boundary function at (0, y)
+ * and (xMax, y), overwriting the corners.  An edge function may name its
+ * variable x or y.  NULL or blank = absent; the boundary function is required
+ * unless all four edge functions are given (InputParser.cs:190-203).
+ * *vmin / *vmax (may be NULL) receive the min / max of every value written.
+ * Replaces initTopBottomBorders + initLeftRightBorders (UtilsBorders.cs:
+ * 115-144, 210-239; order VLPRectangleModel.cs:541-542). */
+int vlp2d_mg_set_boundary_expr(vlp2d_mg_handle h, const char *bottom, const char *top, const char *left,
+                               const char *right, const char *boundary, double *vmin, double *vmax);
+
+#define VLP2D_INIT_MEAN 0            /* interior = value (NaN: (min+max)/2 of the sampled boundary) */
+#define VLP2D_INIT_ARITHMETIC_MEAN 1 /* mean of the four edge values of the node's row and column  */
+#define VLP2D_INIT_LINEAR 2          /* mean of the linear interpolations along x and along y       */
+#define VLP2D_INIT_WEIGHT_LINEAR 3   /* the two interpolations weighted by distance to the edges    */
+
+/* Initial guess of the interior on the GPU from the boundary values in place.
+ * Replaces initInitialIteration{Mean, ArithmeticMean, LinearInterpolation,
+ * WeightLinearInterpolation} (UtilsII.cs:15-105; weights computed in FP32 as
+ * there; Mean value VLPRectangleModel.cs:543-545). */
+int vlp2d_mg_init_guess(vlp2d_mg_handle h, int kind, double value);
+
+/* min / max of the level-0 iterate over all nodes (NaN ignored).  Replaces
+ * pointsMinMax (UtilsDiff.cs:19-40). */
+int vlp2d_mg_points_min_max(vlp2d_mg_handle h, double *vmin, double *vmax);
+
+/* u - analytic(x, y) on nodes i >= 1, j >= 1: its min / max (NaN u skipped)
+ * and, when diff is not NULL, the whole array (nx*ny, row 0 and column 0 set
+ * to 0).  Replaces calculateDifference (UtilsDiff.cs:42-88). */
+int vlp2d_mg_difference(vlp2d_mg_handle h, const char *analytic, double *dmin, double *dmax, double *diff);
+
+/* Host evaluation of an expression at n points (for hosts without the ELW
+ * evaluator; the GPU evaluates the same compiled program).  Errors in
+ * vlp2d_mg_last_error(NULL). */
+int vlp2d_expr_eval(const char *expr, const double *x, const double *y, long long n, double *out);
+
 /* Message of the last failure on h; with h == NULL, of the last failed
- * vlp2d_mg_create on the calling thread.  Never NULL. */
+ * vlp2d_mg_create / vlp2d_mg_create_expr / vlp2d_expr_eval on the calling
+ * thread.  Never NULL. */
 const char *vlp2d_mg_last_error(vlp2d_mg_handle h);
 
 /* ABI version, (major << 16) | minor. */

@@ -15,7 +15,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def _declared():
     txt = open(os.path.join(ROOT, "include", "vlp2d_mg.h")).read()
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
-    return sorted(set(re.findall(r"\b(vlp2d_mg_\w+)\s*\(", txt)))
+    return sorted(set(re.findall(r"\b(vlp2d_(?:mg|expr)_\w+)\s*\(", txt)))
 
 
 def test_header_matches_binding_table():

@@ -25,8 +25,10 @@ EXPORTS = [
     "vlp2d_mg_download", "vlp2d_mg_residual_norm", "vlp2d_mg_solve", "vlp2d_mg_adjoint_exact",
     "vlp2d_mg_forward_cycles", "vlp2d_mg_adjoint_cycles", "vlp2d_mg_get_info", "vlp2d_mg_kernel_times",
     "vlp2d_mg_reset_kernel_times", "vlp2d_mg_save_iterate", "vlp2d_mg_restore_iterate", "vlp2d_mg_cancel",
-    "vlp2d_mg_destroy", "vlp2d_mg_last_error",
+    "vlp2d_mg_destroy", "vlp2d_mg_last_error", "vlp2d_mg_create_expr", "vlp2d_mg_set_boundary_expr",
+    "vlp2d_mg_init_guess", "vlp2d_mg_points_min_max", "vlp2d_mg_difference", "vlp2d_expr_eval",
 ]
+INIT_MEAN, INIT_ARITHMETIC_MEAN, INIT_LINEAR, INIT_WEIGHT_LINEAR = 0, 1, 2, 3
 
 
 class VLP2DError(RuntimeError):
@@ -89,6 +91,15 @@ def load_library(path=LIB_PATH):
         "vlp2d_mg_cancel": (None, [_h]),
         "vlp2d_mg_destroy": (None, [_h]),
         "vlp2d_mg_last_error": (ctypes.c_char_p, [_h]),
+        "vlp2d_mg_create_expr": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                                ctypes.c_char_p, ctypes.c_double, ctypes.c_int, ctypes.POINTER(Opts),
+                                                ctypes.POINTER(_h)]),
+        "vlp2d_mg_set_boundary_expr": (ctypes.c_int, [_h, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p,
+                                                      ctypes.c_char_p, ctypes.c_char_p, _dp, _dp]),
+        "vlp2d_mg_init_guess": (ctypes.c_int, [_h, ctypes.c_int, ctypes.c_double]),
+        "vlp2d_mg_points_min_max": (ctypes.c_int, [_h, _dp, _dp]),
+        "vlp2d_mg_difference": (ctypes.c_int, [_h, ctypes.c_char_p, _dp, _dp, _dp]),
+        "vlp2d_expr_eval": (ctypes.c_int, [ctypes.c_char_p, _dp, _dp, ctypes.c_longlong, _dp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

@@ -32,6 +32,7 @@
 #include "mg_coarse.hpp"
 #include "mg_refine.hpp"
 #include "mg_stream.hpp"
+#include "mg_setup.hpp"
 
 
 using namespace vlp2d;
@@ -55,11 +56,12 @@ enum KClass {
   KC_DOWN,
   KC_UP,
   KC_COARSE,
+  KC_SETUP,
   KC_COUNT
 };
 const char *kKClassName[KC_COUNT] = {"rb_half",   "residual",     "restrict", "fill",    "prolong",
                                      "copy_interior", "absdiff_max", "resnorm64", "reduce", "update_mixed",
-                                     "adjoint",   "down_fused",   "up_fused", "coarse_vcycle"};
+                                     "adjoint",   "down_fused",   "up_fused", "coarse_vcycle", "setup"};
 
 // Epilogues of the level-0 ascent.
 enum Epi { EPI_NONE = 0, EPI_RESNORM = 1, EPI_METRIC = 2, EPI_MIXED = 3 };
@@ -118,6 +120,8 @@ struct vlp2d_mg_ctx {
   int K = 0;
   std::map<int, std::pair<hipGraph_t, hipGraphExec_t>> graphs;
   int num_cu = 0;                    // compute units of the device
+  ExprOp *dprog = nullptr;           // device copies of compiled expressions (kProgSlots programs)
+  double bmin = NAN, bmax = NAN;     // min / max of the last boundary sampled on the device
   std::map<const void *, int> occ;   // resident 64-thread workgroups per GPU, per kernel
   // profiling
   bool profile = false;
@@ -863,6 +867,75 @@ void cycle_T(Ctx *c, int l) {
 // =============================================================================
 // C ABI
 // =============================================================================
+// ---- problem set-up on the GPU (expressions, boundary, initial guess) ----
+constexpr int kProgSlots = 6;
+
+int compile_expr(Ctx *c, const char *text, const char *what, Expr &e) {
+  std::string err;
+  if (!expr_compile(text ? text : "", e, err)) return fail(c, VLP2D_E_ARG, std::string(what) + ": " + err);
+  return VLP2D_OK;
+}
+
+// copies program e into device slot k
+int upload_prog(Ctx *c, int k, const Expr &e, DevProg &d) {
+  if (!c->dprog) RCHK(dalloc(c, &c->dprog, (size_t)kProgSlots * kExprMaxOps));
+  d.ops = c->dprog + (size_t)k * kExprMaxOps;
+  d.n = (int)e.ops.size();
+  if (d.n) HIPCHK(c, hipMemcpyAsync((void *)d.ops, e.ops.data(), e.ops.size() * sizeof(ExprOp), hipMemcpyHostToDevice,
+                                    c->st));
+  return VLP2D_OK;
+}
+
+// 2-D set-up grid: x over columns, y over rows, at most SETUP_MAX_BLOCKS blocks
+inline dim3 setup_grid(int nx, int ny) {
+  const int gx = std::max(1, std::min((ny + SETUP_THREADS - 1) / SETUP_THREADS, 32));
+  const int gy = std::max(1, std::min(nx, SETUP_MAX_BLOCKS / gx));
+  return dim3(gx, gy);
+}
+
+// min / max of np partials -> *mn, *mx (synchronises)
+int read_minmax(Ctx *c, int np, double *mn, double *mx) {
+  launch(c, KC_SETUP, 0, 16.0 * np, [&] {
+    hipLaunchKernelGGL(k_mm_final, dim3(1), dim3(SETUP_THREADS), 0, c->st, c->part, c->part2, np, c->dscal + 4);
+  });
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipMemcpyAsync(c->hscal + 4, c->dscal + 4, 2 * sizeof(double), hipMemcpyDeviceToHost, c->st));
+  HIPCHK(c, hipStreamSynchronize(c->st));
+  if (c->profile) collect_profile(c);
+  if (mn) *mn = c->hscal[4];
+  if (mx) *mx = c->hscal[5];
+  return VLP2D_OK;
+}
+
+// fn = -f(hx i, hy j) on level 0 from a compiled program; fails on a
+// non-finite value
+int sample_rhs(Ctx *c, const Expr &fe) {
+  Lvl<double> &P = c->h64.lv[0];
+  DevProg d;
+  RCHK(upload_prog(c, 0, fe, d));
+  launch(c, KC_SETUP, 0, 8.0 * nodes(P), [&] {
+    hipLaunchKernelGGL(k_expr_rhs, setup_grid(P.nx, P.ny), dim3(SETUP_THREADS), 0, c->st, d, P.rhs, P.nx, P.ny,
+                       P.pitch, c->hx, c->hy);
+  });
+  HIPCHK(c, hipGetLastError());
+  const dim3 g = setup_grid(P.nx, P.ny);
+  launch(c, KC_SETUP, 0, 8.0 * nodes(P), [&] {
+    hipLaunchKernelGGL(k_minmax, g, dim3(SETUP_THREADS), 0, c->st, P.rhs, P.nx, P.ny, P.pitch, c->part, c->part2);
+  });
+  double mn, mx;
+  RCHK(read_minmax(c, (int)(g.x * g.y), &mn, &mx));
+  if (!std::isfinite(mn) || !std::isfinite(mx)) return fail(c, VLP2D_E_NONFINITE, "f evaluates to NaN or Inf");
+  return VLP2D_OK;
+}
+
+// state after the level-0 iterate changed on the device (as after upload)
+int iterate_replaced(Ctx *c) {
+  c->r32_valid = false;
+  c->K = 0;
+  c->cancel.store(0);
+  return mirror_level0(c, c->h64.lv[0]);
+}
+
 extern "C" {
 
 void vlp2d_mg_default_opts(vlp2d_mg_opts *o) {
@@ -877,9 +950,8 @@ void vlp2d_mg_default_opts(vlp2d_mg_opts *o) {
 
 int vlp2d_mg_version(void) { return (1 << 16) | 1; }
 
-int vlp2d_mg_create(int segX, int segY, double hx, double hy, const double *f, double eps, int device,
-                    const vlp2d_mg_opts *opts, vlp2d_mg_handle *out) {
-  g_create_error.clear();
+static int create_impl(int segX, int segY, double hx, double hy, const double *f, const Expr *fe, double eps,
+                       int device, const vlp2d_mg_opts *opts, vlp2d_mg_handle *out) {
   if (!out) {
     g_create_error = "out handle pointer is NULL";
     return VLP2D_E_ARG;
@@ -922,7 +994,7 @@ int vlp2d_mg_create(int segX, int segY, double hx, double hy, const double *f, d
   c->hx = hx;
   c->hy = hy;
   c->eps = eps;
-  c->laplace = (f == nullptr);
+  c->laplace = fe ? fe->empty() : (f == nullptr);
   c->o = o;
   c->profile = (o.flags & VLP2D_FLAG_PROFILE) != 0;
   // fused kernels are instantiated for 1..3 sweeps per pass
@@ -978,6 +1050,9 @@ int vlp2d_mg_create(int segX, int segY, double hx, double hy, const double *f, d
     for (int i = 1; i < nx - 1; ++i)
       for (int j = 1; j < ny - 1; ++j) fn[(size_t)i * ny + j] = -f[(size_t)i * ny + j];
     if ((rc = upload_grid(c, v.rhs, v.pitch, fn.data())) != VLP2D_OK) return bail(rc);
+  }
+  if (fe && !fe->empty()) {
+    if ((rc = sample_rhs(c, *fe)) != VLP2D_OK) return bail(rc);
   }
   if (hipStreamSynchronize(c->st) != hipSuccess) {
     c->err = "stream synchronisation failed";
@@ -1252,9 +1327,138 @@ void vlp2d_mg_destroy(vlp2d_mg_handle c) {
   dfree(c, c->part, (size_t)c->part_cap);
   dfree(c, c->part2, (size_t)c->part_cap);
   dfree(c, c->dscal, 8);
+  dfree(c, c->dprog, (size_t)kProgSlots * kExprMaxOps);
   if (c->hscal) (void)hipHostFree(c->hscal);
   if (c->st) (void)hipStreamDestroy(c->st);
   delete c;
+}
+
+int vlp2d_mg_create(int segX, int segY, double hx, double hy, const double *f, double eps, int device,
+                    const vlp2d_mg_opts *opts, vlp2d_mg_handle *out) {
+  g_create_error.clear();
+  return create_impl(segX, segY, hx, hy, f, nullptr, eps, device, opts, out);
+}
+
+int vlp2d_mg_create_expr(int segX, int segY, double hx, double hy, const char *f_expr, double eps, int device,
+                         const vlp2d_mg_opts *opts, vlp2d_mg_handle *out) {
+  g_create_error.clear();
+  Expr fe;
+  std::string err;
+  if (!expr_compile(f_expr ? f_expr : "", fe, err)) {
+    g_create_error = "f: " + err;
+    if (out) *out = nullptr;
+    return VLP2D_E_ARG;
+  }
+  return create_impl(segX, segY, hx, hy, nullptr, &fe, eps, device, opts, out);
+}
+
+int vlp2d_expr_eval(const char *expr, const double *x, const double *y, long long n, double *out) {
+  g_create_error.clear();
+  if (!expr || (n > 0 && (!x || !y || !out)) || n < 0) {
+    g_create_error = "expr, x, y and out must be non-NULL and n >= 0";
+    return VLP2D_E_ARG;
+  }
+  Expr e;
+  std::string err;
+  if (!expr_compile(expr, e, err)) {
+    g_create_error = err;
+    return VLP2D_E_ARG;
+  }
+  if (e.empty()) {
+    g_create_error = "empty expression";
+    return VLP2D_E_ARG;
+  }
+  for (long long k = 0; k < n; ++k) out[k] = e.eval(x[k], y[k]);
+  return VLP2D_OK;
+}
+
+int vlp2d_mg_set_boundary_expr(vlp2d_mg_handle c, const char *bottom, const char *top, const char *left,
+                               const char *right, const char *boundary, double *vmin, double *vmax) {
+  if (!c) return VLP2D_E_ARG;
+  Expr eb, et, el, er, eg;
+  RCHK(compile_expr(c, bottom, "bottom", eb));
+  RCHK(compile_expr(c, top, "top", et));
+  RCHK(compile_expr(c, left, "left", el));
+  RCHK(compile_expr(c, right, "right", er));
+  RCHK(compile_expr(c, boundary, "boundary", eg));
+  if (eg.empty() && (eb.empty() || et.empty() || el.empty() || er.empty()))
+    return fail(c, VLP2D_E_ARG, "the boundary needs the boundary function or all four edge functions");
+  HIPCHK(c, hipSetDevice(c->dev));
+  Lvl<double> &P = c->h64.lv[0];
+  DevProg db, dt, dl, dr, dg;
+  RCHK(upload_prog(c, 1, eb, db));
+  RCHK(upload_prog(c, 2, et, dt));
+  RCHK(upload_prog(c, 3, el, dl));
+  RCHK(upload_prog(c, 4, er, dr));
+  RCHK(upload_prog(c, 5, eg, dg));
+  const int nb1 = std::max(1, std::min((P.nx + SETUP_THREADS - 1) / SETUP_THREADS, SETUP_MAX_BLOCKS / 2));
+  const int nb2 = std::max(1, std::min((P.ny + SETUP_THREADS - 1) / SETUP_THREADS, SETUP_MAX_BLOCKS / 2));
+  launch(c, KC_SETUP, 0, 16.0 * (P.nx + P.ny), [&] {
+    hipLaunchKernelGGL(k_border_rows, dim3(nb1), dim3(SETUP_THREADS), 0, c->st, db, dt, dg, P.u, P.nx, P.ny, P.pitch,
+                       c->hx, c->hy, c->part, c->part2);
+    hipLaunchKernelGGL(k_border_cols, dim3(nb2), dim3(SETUP_THREADS), 0, c->st, dl, dr, dg, P.u, P.nx, P.ny, P.pitch,
+                       c->hx, c->hy, c->part + nb1, c->part2 + nb1);
+  });
+  HIPCHK(c, hipGetLastError());
+  double mn, mx;
+  RCHK(read_minmax(c, nb1 + nb2, &mn, &mx));
+  c->bmin = mn;
+  c->bmax = mx;
+  if (vmin) *vmin = mn;
+  if (vmax) *vmax = mx;
+  return iterate_replaced(c);
+}
+
+int vlp2d_mg_init_guess(vlp2d_mg_handle c, int kind, double value) {
+  if (!c) return VLP2D_E_ARG;
+  if (kind < VLP2D_INIT_MEAN || kind > VLP2D_INIT_WEIGHT_LINEAR) return fail(c, VLP2D_E_ARG, "unknown initial guess");
+  if (kind == VLP2D_INIT_MEAN && std::isnan(value)) {
+    if (std::isnan(c->bmin)) return fail(c, VLP2D_E_STATE, "no boundary sampled on the device: pass the mean value");
+    value = (c->bmin + c->bmax) / 2.0;  // VLPRectangleModel.cs:545
+  }
+  HIPCHK(c, hipSetDevice(c->dev));
+  Lvl<double> &P = c->h64.lv[0];
+  launch(c, KC_SETUP, 0, 8.0 * nodes(P), [&] {
+    hipLaunchKernelGGL(k_init_guess, setup_grid(P.nx, P.ny), dim3(SETUP_THREADS), 0, c->st, P.u, P.nx, P.ny, P.pitch,
+                       kind, value);
+  });
+  HIPCHK(c, hipGetLastError());
+  return iterate_replaced(c);
+}
+
+int vlp2d_mg_points_min_max(vlp2d_mg_handle c, double *vmin, double *vmax) {
+  if (!c) return VLP2D_E_ARG;
+  HIPCHK(c, hipSetDevice(c->dev));
+  Lvl<double> &P = c->h64.lv[0];
+  const dim3 g = setup_grid(P.nx, P.ny);
+  launch(c, KC_SETUP, 0, 8.0 * nodes(P), [&] {
+    hipLaunchKernelGGL(k_minmax, g, dim3(SETUP_THREADS), 0, c->st, P.u, P.nx, P.ny, P.pitch, c->part, c->part2);
+  });
+  HIPCHK(c, hipGetLastError());
+  return read_minmax(c, (int)(g.x * g.y), vmin, vmax);
+}
+
+int vlp2d_mg_difference(vlp2d_mg_handle c, const char *analytic, double *dmin, double *dmax, double *diff) {
+  if (!c) return VLP2D_E_ARG;
+  Expr ea;
+  RCHK(compile_expr(c, analytic, "analytic", ea));
+  if (ea.empty()) return fail(c, VLP2D_E_ARG, "analytic: empty expression");
+  HIPCHK(c, hipSetDevice(c->dev));
+  Lvl<double> &P = c->h64.lv[0];
+  DevProg d;
+  RCHK(upload_prog(c, 0, ea, d));
+  double *dd = nullptr;
+  if (diff) RCHK(dalloc(c, &dd, P.elems()));
+  const dim3 g = setup_grid(P.nx, P.ny);
+  launch(c, KC_SETUP, 0, (diff ? 16.0 : 8.0) * nodes(P), [&] {
+    hipLaunchKernelGGL(k_difference, g, dim3(SETUP_THREADS), 0, c->st, P.u, d, P.nx, P.ny, P.pitch, c->hx, c->hy, dd,
+                       c->part, c->part2);
+  });
+  int rc = (hipGetLastError() == hipSuccess) ? VLP2D_OK : fail(c, VLP2D_E_DEVICE, "difference kernel failed");
+  if (rc == VLP2D_OK) rc = read_minmax(c, (int)(g.x * g.y), dmin, dmax);
+  if (rc == VLP2D_OK && diff) rc = download_grid(c, diff, dd, P.pitch);
+  if (dd) dfree(c, dd, P.elems());
+  return rc;
 }
 
 const char *vlp2d_mg_last_error(vlp2d_mg_handle c) {

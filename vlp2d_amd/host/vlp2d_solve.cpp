@@ -1,0 +1,73 @@
+// vlp2d_solve: runs the MultiGrid method on one of the reference's input
+// files on the GPU and prints the run's figures as one JSON line.
+//
+//   vlp2d_solve <input.txt> [--init mean|amean|linear|wlinear] [--device N]
+//               [--mixed] [--cycle nu1,nu2] [--max-cycles K]
+//
+// Without --cycle the scheme runs the reference's V(3,3) cycle with its update
+// norm stop (epsilonDouble of the file).
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+
+#include "vlp2d_host.hpp"
+
+int main(int argc, char **argv) {
+  if (argc < 2) {
+    std::fprintf(stderr, "usage: %s <input.txt> [--init mean|amean|linear|wlinear] [--device N] [--mixed] "
+                         "[--cycle nu1,nu2] [--max-cycles K]\n", argv[0]);
+    return 2;
+  }
+  int init = VLP2D_INIT_MEAN, device = 0, max_cycles = 0;
+  vlp2d_mg_opts o;
+  vlp2d_mg_default_opts(&o);
+  for (int k = 2; k < argc; ++k) {
+    const std::string a = argv[k];
+    auto next = [&]() -> const char * {
+      if (k + 1 >= argc) {
+        std::fprintf(stderr, "%s needs a value\n", a.c_str());
+        std::exit(2);
+      }
+      return argv[++k];
+    };
+    if (a == "--init") {
+      const std::string v = next();
+      init = v == "mean" ? VLP2D_INIT_MEAN : v == "amean" ? VLP2D_INIT_ARITHMETIC_MEAN
+           : v == "linear" ? VLP2D_INIT_LINEAR : v == "wlinear" ? VLP2D_INIT_WEIGHT_LINEAR : -1;
+      if (init < 0) {
+        std::fprintf(stderr, "unknown initial guess %s\n", v.c_str());
+        return 2;
+      }
+    } else if (a == "--device") {
+      device = std::atoi(next());
+    } else if (a == "--mixed") {
+      o.precision = VLP2D_PREC_MIXED;
+    } else if (a == "--cycle") {
+      if (std::sscanf(next(), "%d,%d", &o.nu1, &o.nu2) != 2) {
+        std::fprintf(stderr, "--cycle needs nu1,nu2\n");
+        return 2;
+      }
+    } else if (a == "--max-cycles") {
+      max_cycles = std::atoi(next());
+    } else {
+      std::fprintf(stderr, "unknown option %s\n", a.c_str());
+      return 2;
+    }
+  }
+  try {
+    const vlp2d_input in = vlp2d::parse_input_file(argv[1]);
+    const vlp2d_run_result r = vlp2d::run_model(in, init, device, &o, max_cycles);
+    std::printf("{\"input\": \"%s\", \"segments\": [%d, %d], \"steps\": [%.17g, %.17g], \"eps\": %.3g, "
+                "\"cycles\": %d, \"elapsed_iters\": %d, \"budget\": %d, \"converged\": %s, \"last_metric\": %.6g, "
+                "\"init_s\": %.6f, \"solve_s\": %.6f, \"min\": %.17g, \"max\": %.17g, \"diff_min\": %.6g, "
+                "\"diff_max\": %.6g}\n",
+                in.name, r.seg_x, r.seg_y, r.hx, r.hy, r.eps, r.cycles, r.elapsed_iters, in.mg_iters[init],
+                r.converged ? "true" : "false", r.last_metric, r.init_seconds, r.solve_seconds, r.vmin, r.vmax,
+                r.dmin, r.dmax);
+    return r.converged ? 0 : 1;
+  } catch (const std::exception &e) {
+    std::fprintf(stderr, "%s\n", e.what());
+    return 1;
+  }
+}

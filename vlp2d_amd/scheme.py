@@ -66,24 +66,68 @@ class MultiGridSchemeHIP:
     sampled at x = hx*i, y = hy*j, an (nx, ny) array, or None (Laplace).
     """
 
-    def __init__(self, seg_x, seg_y, step_x, step_y, f_ksi=None, eps=1e-13, device=0, options=None):
+    def __init__(self, seg_x, seg_y, step_x, step_y, f_ksi=None, eps=1e-13, device=0, options=None, f_expr=None):
+        """``f_expr`` (a text in the input files' syntax, vlp2d_mg.h) instead of
+        ``f_ksi`` samples f on the GPU."""
         self.lib = load_library()
         self.seg_x, self.seg_y = int(seg_x), int(seg_y)
         self.dim_x, self.dim_y = self.seg_x + 1, self.seg_y + 1
         self.step_x, self.step_y = float(step_x), float(step_y)
         self.eps = float(eps)
         self.options = options or Options()
-        f = _sample_rhs(f_ksi, self.dim_x, self.dim_y, self.step_x, self.step_y)
         opts = self.options.to_c()
         h = ctypes.c_void_p()
-        rc = self.lib.vlp2d_mg_create(self.seg_x, self.seg_y, self.step_x, self.step_y, ptr(f), self.eps, int(device),
-                                      ctypes.byref(opts), ctypes.byref(h))
+        if f_expr is not None:
+            if f_ksi is not None:
+                raise ValueError("give f_ksi or f_expr, not both")
+            rc = self.lib.vlp2d_mg_create_expr(self.seg_x, self.seg_y, self.step_x, self.step_y, f_expr.encode(),
+                                               self.eps, int(device), ctypes.byref(opts), ctypes.byref(h))
+        else:
+            f = _sample_rhs(f_ksi, self.dim_x, self.dim_y, self.step_x, self.step_y)
+            rc = self.lib.vlp2d_mg_create(self.seg_x, self.seg_y, self.step_x, self.step_y, ptr(f), self.eps,
+                                          int(device), ctypes.byref(opts), ctypes.byref(h))
         if rc != _lib.OK:
             raise VLP2DError(rc, self.lib.vlp2d_mg_last_error(None).decode())
         self.h = h
         self.un0 = np.zeros((self.dim_x, self.dim_y), dtype=np.float64)
         self._un0_changed = False
         self.last_metric = None
+
+    # ------------------------------------------------- device-side set-up
+    def set_boundary_expr(self, bottom="", top="", left="", right="", boundary=""):
+        """Samples the boundary on the GPU into the iterate (vlp2d_mg_set_boundary_expr);
+        returns (min, max) of the values written.  The host mirror ``un0`` is
+        not involved; ``get_array`` downloads the device grid."""
+        lo, hi = ctypes.c_double(), ctypes.c_double()
+        check(self.lib, self.h, self.lib.vlp2d_mg_set_boundary_expr(
+            self.h, bottom.encode(), top.encode(), left.encode(), right.encode(), boundary.encode(),
+            ctypes.byref(lo), ctypes.byref(hi)))
+        self._un0_changed = False
+        return lo.value, hi.value
+
+    def init_guess(self, kind, value=float("nan")):
+        """Initial guess of the interior on the GPU (vlp2d_mg_init_guess, _lib.INIT_*)."""
+        check(self.lib, self.h, self.lib.vlp2d_mg_init_guess(self.h, int(kind), float(value)))
+        self._un0_changed = False
+
+    def points_min_max_device(self):
+        lo, hi = ctypes.c_double(), ctypes.c_double()
+        check(self.lib, self.h, self.lib.vlp2d_mg_points_min_max(self.h, ctypes.byref(lo), ctypes.byref(hi)))
+        return lo.value, hi.value
+
+    def difference(self, analytic, with_array=False):
+        """(min, max[, diff]) of u - analytic on nodes i, j >= 1 (vlp2d_mg_difference)."""
+        lo, hi = ctypes.c_double(), ctypes.c_double()
+        d = np.zeros((self.dim_x, self.dim_y)) if with_array else None
+        check(self.lib, self.h, self.lib.vlp2d_mg_difference(self.h, analytic.encode(), ctypes.byref(lo),
+                                                             ctypes.byref(hi), ptr(d)))
+        return (lo.value, hi.value, d) if with_array else (lo.value, hi.value)
+
+    def download(self):
+        """The device iterate (nx, ny), without touching the host mirror."""
+        u = np.empty((self.dim_x, self.dim_y))
+        check(self.lib, self.h, self.lib.vlp2d_mg_download(self.h, ptr(u)))
+        return u
 
     # ----------------------------------------------------------- ISchemeInit
     def get_array_dimensions(self):
