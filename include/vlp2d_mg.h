@@ -49,6 +49,8 @@ extern "C" {
 #define VLP2D_FLAG_PROFILE 2u    /* time every kernel class with hipEvents        */
 #define VLP2D_FLAG_SIMPLE 4u     /* one operator per launch (no fused kernels)    */
 #define VLP2D_FLAG_SEPARATE_REFINE 8u /* mixed: FP64 update as its own pass      */
+#define VLP2D_FLAG_COARSE_L2 16u /* small levels' one-workgroup V-cycle in global
+                                     memory (L2) instead of LDS             */
 
 typedef struct vlp2d_mg_ctx *vlp2d_mg_handle;
 

@@ -69,7 +69,9 @@ def test_parity_mode_cycles_bitwise(seg_x, seg_y, hx, hy, laplace):
 KERNEL_PATHS = {
     "default": {},
     "fused_all_levels": {"coarse_nodes": -1},        # tiles down to the coarsest level
-    "one_workgroup_all": {"coarse_nodes": 1 << 30},  # whole cycle in k_coarse
+    "one_workgroup_all": {"coarse_nodes": 1 << 30},  # whole cycle in k_coarse (too big for LDS: L2)
+    "small_levels_in_l2": {"flags": _lib.FLAG_COARSE_L2},  # k_coarse instead of k_coarse_lds
+    "small_levels_lds_65": {"coarse_nodes": 65 * 65},   # k_coarse_lds over > 64 KB of LDS in FP64
     "one_op_per_launch": {"flags": _lib.FLAG_SIMPLE},
     "no_graph": {"flags": _lib.FLAG_NO_GRAPH},
     "separate_refine": {"flags": _lib.FLAG_SEPARATE_REFINE},
@@ -81,10 +83,10 @@ KERNEL_PATHS = {
                                                        (128, 256, np.e / 128, np.pi / 256, False),
                                                        (512, 64, 1 / 512, 1 / 512, False)])
 def test_kernel_paths_bitwise(path, seg_x, seg_y, hx, hy, laplace):
-    """Every kernel path (fused LDS tiles, one-workgroup small levels, one
-    operator per launch, graph or direct launches) gives the oracle's iterate
-    bit for bit, in parity mode (V(3,3) + update norm) and in the mixed
-    V(2,2) solve."""
+    """Every kernel path (streaming tiles, one-workgroup small levels in LDS
+    or in L2, one operator per launch, graph or direct launches) gives the
+    oracle's iterate bit for bit, in parity mode (V(3,3) + update norm) and
+    in the mixed V(2,2) solve."""
     _say(f"paths {path} {seg_x}x{seg_y}")
     f, g = _random(seg_x, seg_y, 17 + seg_x, laplace)
     u0 = _mean_init(g)

@@ -17,7 +17,7 @@ ERROR_NAMES = {
     E_STATE: "VLP2D_E_STATE",
 }
 PREC_FP64, PREC_MIXED = 0, 1
-FLAG_NO_GRAPH, FLAG_PROFILE, FLAG_SIMPLE, FLAG_SEPARATE_REFINE = 1, 2, 4, 8
+FLAG_NO_GRAPH, FLAG_PROFILE, FLAG_SIMPLE, FLAG_SEPARATE_REFINE, FLAG_COARSE_L2 = 1, 2, 4, 8, 16
 
 # Every entry point declared in include/vlp2d_mg.h.
 EXPORTS = [

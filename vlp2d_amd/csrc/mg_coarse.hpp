@@ -113,4 +113,133 @@ __global__ void __launch_bounds__(CNT) k_coarse(CoarseArgs<T> a) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// The same V-cycle with every level resident in LDS: per level the iterate
+// and the right-hand side, unpadded (row pitch ny), in level order.  Level l0
+// is loaded from global memory (its iterate unless zero-initialised, its
+// right-hand side, restricted by the launch above it), all coarser levels are
+// built in LDS, and only level l0's interior iterate is written back (the
+// next ascent prolongates from it).  Same formulas and order as k_coarse, so
+// the same bits; each phase costs an LDS round trip instead of an L2 one.
+// Launched with the LDS footprint sum_l 2 nx_l ny_l sizeof(T) as dynamic
+// shared memory (at most 160 KB on gfx950).
+// ---------------------------------------------------------------------------
+template <typename T> struct LdsLevel {
+  T *u, *f;
+  int nx, ny;
+};
+
+// LDS bytes k_coarse_lds needs for levels [l0, L) of the given sizes.
+inline size_t coarse_lds_bytes(const int *nx, const int *ny, int l0, int L, size_t elem) {
+  size_t n = 0;
+  for (int l = l0; l < L; ++l) n += 2 * (size_t)nx[l] * (size_t)ny[l];
+  return n * elem;
+}
+
+// Level l's arrays in LDS: levels l0 .. l-1 precede it (the offsets are
+// uniform scalar arithmetic on the kernel arguments).
+template <typename T> __device__ inline LdsLevel<T> cl_level(const CoarseArgs<T> &a, T *sm, int l) {
+  int o = 0;
+  for (int k = a.l0; k < l; ++k) o += 2 * a.lv[k].nx * a.lv[k].ny;
+  const int n = a.lv[l].nx * a.lv[l].ny;
+  return LdsLevel<T>{sm + o, sm + o + n, a.lv[l].nx, a.lv[l].ny};
+}
+
+template <typename T> __device__ inline void cl_smooth(const LdsLevel<T> &v, const CoarseLevel<T> &g, int n) {
+  const int wc = (v.ny - 1) / 2;  // items per row and colour (ny odd)
+  const int items = (v.nx - 2) * wc;
+  const int P = v.ny;
+  for (int s = 0; s < n; ++s)
+    for (int color = 0; color < 2; ++color) {
+      for (int p = threadIdx.x; p < items; p += CNT) {
+        const int i = 1 + p / wc, m = p - (p / wc) * wc;
+        const int j = 2 * m + 1 + ((i + color + 1) & 1);
+        if (j > v.ny - 2) continue;
+        const int c = i * P + j;
+        T *u = v.u;
+        T r;
+        if (g.eq) {
+          if (g.lap) r = (u[c - P] + u[c + P] + u[c - 1] + u[c + 1]) * T(0.25);
+          else r = (u[c - P] + u[c + P] + u[c - 1] + u[c + 1] - v.f[c] * g.hx2) * T(0.25);
+        } else {
+          const T sx = divh(u[c - P] + u[c + P], g.hx2, g.ihx2, g.pw2);
+          const T sy = divh(u[c - 1] + u[c + 1], g.hy2, g.ihy2, g.pw2);
+          if (g.lap) r = g.coef * (sx + sy);
+          else r = g.coef * (sx + sy - v.f[c]);
+        }
+        u[c] = r;
+      }
+      __syncthreads();
+    }
+}
+
+template <typename T> __device__ inline T cl_resid(const LdsLevel<T> &v, const CoarseLevel<T> &g, int c) {
+  const T *u = v.u;
+  const int P = v.ny;
+  const T u2 = u[c] * T(2);
+  return v.f[c] - (divh(u[c - P] + u[c + P] - u2, g.hx2, g.ihx2, g.pw2) +
+                   divh(u[c - 1] + u[c + 1] - u2, g.hy2, g.ihy2, g.pw2));
+}
+
+template <typename T>
+__global__ void __launch_bounds__(CNT) k_coarse_lds(CoarseArgs<T> a) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T *sm = reinterpret_cast<T *>(smem_raw);
+  {
+    // level l0: the whole iterate (its boundary feeds the stencils; interior
+    // zeroed when zero_first) and the right-hand side (0 on Laplace levels)
+    const CoarseLevel<T> &g = a.lv[a.l0];
+    const LdsLevel<T> v = cl_level(a, sm, a.l0);
+    for (int p = threadIdx.x; p < v.nx * v.ny; p += CNT) {
+      const int i = p / v.ny, j = p - i * v.ny;
+      const bool in = i > 0 && i < v.nx - 1 && j > 0 && j < v.ny - 1;
+      v.u[p] = a.zero_first && in ? T(0) : g.u[i * g.pitch + j];
+      v.f[p] = g.lap ? T(0) : g.f[i * g.pitch + j];
+    }
+    __syncthreads();
+  }
+  for (int l = a.l0; l <= a.L - 2; ++l) {
+    const LdsLevel<T> f = cl_level(a, sm, l), g = cl_level(a, sm, l + 1);
+    const CoarseLevel<T> &fg = a.lv[l];
+    cl_smooth(f, fg, a.nu1);
+    const int n = (g.nx - 2) * (g.ny - 2);
+    for (int p = threadIdx.x; p < n; p += CNT) {
+      const int I = 1 + p / (g.ny - 2), J = 1 + p - (p / (g.ny - 2)) * (g.ny - 2);
+      const int c = 2 * I * f.ny + 2 * J;
+      const T r0 = cl_resid(f, fg, c), rip = cl_resid(f, fg, c + f.ny), rim = cl_resid(f, fg, c - f.ny);
+      const T rjp = cl_resid(f, fg, c + 1), rjm = cl_resid(f, fg, c - 1);
+      g.f[I * g.ny + J] = T(0.125) * (r0 * T(4) + (rip + rim + rjp + rjm));
+    }
+    // the coarse iterate, boundary included, starts at 0 (the coarse
+    // right-hand side's boundary is never read)
+    for (int p = threadIdx.x; p < g.nx * g.ny; p += CNT) g.u[p] = T(0);
+    __syncthreads();
+  }
+  cl_smooth(cl_level(a, sm, a.L - 1), a.lv[a.L - 1], a.nuc);
+  for (int l = a.L - 2; l >= a.l0; --l) {
+    const LdsLevel<T> f = cl_level(a, sm, l), g = cl_level(a, sm, l + 1);
+    const int n = (f.nx - 2) * (f.ny - 2);
+    for (int p = threadIdx.x; p < n; p += CNT) {
+      const int i = 1 + p / (f.ny - 2), j = 1 + p - (p / (f.ny - 2)) * (f.ny - 2);
+      const int c = i * f.ny + j;
+      const T v = f.u[c] + prolong_corr(g.u, g.ny, i, j);
+      f.u[c] = v;
+      if (l == 0 && a.snap) a.snap[i * a.lv[0].pitch + j] = (double)v;
+    }
+    __syncthreads();
+    cl_smooth(f, a.lv[l], a.nu2);
+  }
+  {
+    // level l0's interior iterate back to global memory; the coarser levels
+    // live and die in LDS
+    const CoarseLevel<T> &g = a.lv[a.l0];
+    const LdsLevel<T> v = cl_level(a, sm, a.l0);
+    const int n = (v.nx - 2) * (v.ny - 2);
+    for (int p = threadIdx.x; p < n; p += CNT) {
+      const int i = 1 + p / (v.ny - 2), j = 1 + p - (p / (v.ny - 2)) * (v.ny - 2);
+      g.u[i * g.pitch + j] = v.u[i * v.ny + j];
+    }
+  }
+}
+
 }  // namespace vlp2d

@@ -120,6 +120,7 @@ struct vlp2d_mg_ctx {
   int K = 0;
   std::map<int, std::pair<hipGraph_t, hipGraphExec_t>> graphs;
   int num_cu = 0;                    // compute units of the device
+  int max_lds = 65536;               // dynamic LDS bytes one workgroup may use
   ExprOp *dprog = nullptr;           // device copies of compiled expressions (kProgSlots programs)
   double bmin = NAN, bmax = NAN;     // min / max of the last boundary sampled on the device
   std::map<const void *, int> occ;   // resident 64-thread workgroups per GPU, per kernel
@@ -430,6 +431,18 @@ template <typename T> void coarse(Ctx *c, Hier<T> &H, bool zero_first, double *s
   a.nuc = c->o.nu_coarse;
   a.zero_first = zero_first ? 1 : 0;
   a.snap = snap;
+  // every level in LDS when it fits in one workgroup's share, else in global
+  // memory (the working set then sits in L2)
+  int nx[kMaxLevels], ny[kMaxLevels];
+  for (int l = H.lc; l < L; ++l) nx[l] = H.lv[l].nx, ny[l] = H.lv[l].ny;
+  const size_t lds = coarse_lds_bytes(nx, ny, H.lc, L, sizeof(T));
+  if (!(c->o.flags & VLP2D_FLAG_COARSE_L2) && lds <= (size_t)c->max_lds) {
+    if (lds > 65536) (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_coarse_lds<T>),
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    launch(c, KC_COARSE, H.lc, bytes,
+           [&] { hipLaunchKernelGGL(k_coarse_lds<T>, dim3(1), dim3(CNT), lds, c->st, a); });
+    return;
+  }
   launch(c, KC_COARSE, H.lc, bytes, [&] { hipLaunchKernelGGL(k_coarse<T>, dim3(1), dim3(CNT), 0, c->st, a); });
 }
 
@@ -1012,6 +1025,10 @@ static int create_impl(int segX, int segY, double hx, double hy, const double *f
   if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) {
     c->err = "hipStreamCreate failed";
     return bail(VLP2D_E_DEVICE);
+  }
+  if (hipDeviceGetAttribute(&c->max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess) {
+    (void)hipGetLastError();
+    c->max_lds = 65536;
   }
   if (hipDeviceGetAttribute(&c->num_cu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) {
     (void)hipGetLastError();
