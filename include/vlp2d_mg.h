@@ -51,6 +51,10 @@ extern "C" {
 #define VLP2D_FLAG_SEPARATE_REFINE 8u /* mixed: FP64 update as its own pass      */
 #define VLP2D_FLAG_COARSE_L2 16u /* small levels' one-workgroup V-cycle in global
                                      memory (L2) instead of LDS             */
+#define VLP2D_FLAG_STORE_PRESMOOTH 32u /* zero-initialised levels with one pre-
+                                     smoothing sweep: the descent stores the
+                                     iterate and the ascent reads it back
+                                     (default: the ascent recomputes it)    */
 
 typedef struct vlp2d_mg_ctx *vlp2d_mg_handle;
 

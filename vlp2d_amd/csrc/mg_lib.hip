@@ -351,63 +351,78 @@ LevelView<T> sview(Ctx *c, const void *fn, const Lvl<T> &v, const T *uin, T *uou
   return a;
 }
 
+// A zero-initialised level pre-smoothed with one sweep is not stored by the
+// descent; the ascent recomputes it from the right-hand side (k_sup N1 = 1).
+// Not with the update metric, which needs the iterate before post-smoothing.
+bool recompute_presmooth(const Ctx *c, bool zi, int epi) {
+  return zi && c->o.nu1 == 1 && epi != EPI_METRIC && !(c->o.flags & VLP2D_FLAG_STORE_PRESMOOTH);
+}
+
 template <typename T, int NS>
-void launch_down_ns(Ctx *c, Lvl<T> &f, Lvl<T> &g, bool zi) {
+void launch_down_ns(Ctx *c, Lvl<T> &f, Lvl<T> &g, bool zi, bool store) {
   constexpr int H = s_down_halo(NS), P = s_period(NS);
   using K = void (*)(LevelView<T>, T *, int, int, int);
-  const K k = zi ? (f.eq ? (K)k_sdown<T, NS, true, true> : (K)k_sdown<T, NS, true, false>)
-                 : (f.eq ? (K)k_sdown<T, NS, false, true> : (K)k_sdown<T, NS, false, false>);
+  K k;
+  if (!zi) k = f.eq ? (K)k_sdown<T, NS, false, true> : (K)k_sdown<T, NS, false, false>;
+  else if (store) k = f.eq ? (K)k_sdown<T, NS, true, true> : (K)k_sdown<T, NS, true, false>;
+  else k = f.eq ? (K)k_sdown<T, NS, true, true, false> : (K)k_sdown<T, NS, true, false, false>;
   const LevelView<T> a = sview(c, (const void *)k, f, f.u, f.u2, H, P);
   hipLaunchKernelGGL(k, dim3(8 * a.chunk), dim3(64), 0, c->st, a, g.rhs, g.nx, g.ny, g.pitch);
 }
 
-template <typename T> void down(Ctx *c, Lvl<T> &f, Lvl<T> &g, int lvl, bool zi) {
+template <typename T> void down(Ctx *c, Lvl<T> &f, Lvl<T> &g, int lvl, bool zi, bool store) {
   const double s = sizeof(T);
-  launch(c, KC_DOWN, lvl, s * ((zi ? 2.0 : 3.0) * nodes(f) + nodes(g)), [&] {
+  const double rw = (zi ? 1.0 : 2.0) + (store ? 1.0 : 0.0);  // [u] + f + [u out]
+  launch(c, KC_DOWN, lvl, s * (rw * nodes(f) + nodes(g)), [&] {
     switch (c->o.nu1) {
-      case 1: launch_down_ns<T, 1>(c, f, g, zi); break;
-      case 2: launch_down_ns<T, 2>(c, f, g, zi); break;
-      default: launch_down_ns<T, 3>(c, f, g, zi); break;
+      case 1: launch_down_ns<T, 1>(c, f, g, zi, store); break;
+      case 2: launch_down_ns<T, 2>(c, f, g, zi, store); break;
+      default: launch_down_ns<T, 3>(c, f, g, zi, store); break;
     }
   });
 }
 
-template <typename T, int NS, int EPI>
+template <typename T, int NS, int EPI, int N1>
 int launch_sup(Ctx *c, Lvl<T> &f, const Lvl<T> &g, const MixView &mx) {
-  constexpr int H = s_up_halo(NS, EPI), P = s_period(NS);
+  constexpr int H = s_up_halo(N1 + NS, EPI), P = s_period(N1 + NS);
   using K = void (*)(LevelView<T>, const T *, int, int, int, double *, MixView);
-  const K k = f.eq ? (K)k_sup<T, NS, EPI, true> : (K)k_sup<T, NS, EPI, false>;
+  const K k = f.eq ? (K)k_sup<T, NS, EPI, true, N1> : (K)k_sup<T, NS, EPI, false, N1>;
   const LevelView<T> a = sview(c, (const void *)k, f, f.u2, f.u, H, P);
   hipLaunchKernelGGL(k, dim3(8 * a.chunk), dim3(64), 0, c->st, a, g.u, g.nx, g.ny, g.pitch, c->part, mx);
   return a.tiles_i * a.tiles_j;
 }
 
 // FP64 levels run epilogues none / residual norm / update norm; FP32 levels
-// (the correction cycle of a mixed step) none / fused refinement.
-template <typename T, int NS> int launch_up_ns(Ctx *c, Lvl<T> &f, const Lvl<T> &g, int epi, const MixView &mx) {
+// (the correction cycle of a mixed step) none / fused refinement.  rz: the
+// level's pre-smoothed iterate is recomputed (one pre-smoothing sweep).
+template <typename T, int NS>
+int launch_up_ns(Ctx *c, Lvl<T> &f, const Lvl<T> &g, int epi, bool rz, const MixView &mx) {
   if constexpr (sizeof(T) == 8) {
     switch (epi) {
-      case EPI_RESNORM: return launch_sup<T, NS, EPI_RESNORM>(c, f, g, mx);
-      case EPI_METRIC: return launch_sup<T, NS, EPI_METRIC>(c, f, g, mx);
-      default: return launch_sup<T, NS, EPI_NONE>(c, f, g, mx);
+      case EPI_RESNORM:
+        return rz ? launch_sup<T, NS, EPI_RESNORM, 1>(c, f, g, mx) : launch_sup<T, NS, EPI_RESNORM, 0>(c, f, g, mx);
+      case EPI_METRIC: return launch_sup<T, NS, EPI_METRIC, 0>(c, f, g, mx);
+      default: return rz ? launch_sup<T, NS, EPI_NONE, 1>(c, f, g, mx) : launch_sup<T, NS, EPI_NONE, 0>(c, f, g, mx);
     }
   } else {
-    if (epi == EPI_MIXED) return launch_sup<T, NS, EPI_MIXED>(c, f, g, mx);
-    return launch_sup<T, NS, EPI_NONE>(c, f, g, mx);
+    if (epi == EPI_MIXED)
+      return rz ? launch_sup<T, NS, EPI_MIXED, 1>(c, f, g, mx) : launch_sup<T, NS, EPI_MIXED, 0>(c, f, g, mx);
+    return rz ? launch_sup<T, NS, EPI_NONE, 1>(c, f, g, mx) : launch_sup<T, NS, EPI_NONE, 0>(c, f, g, mx);
   }
 }
 
 // returns the number of partials the epilogue writes
-template <typename T> int up(Ctx *c, Lvl<T> &f, Lvl<T> &g, int lvl, int epi, const MixView &mx) {
+template <typename T> int up(Ctx *c, Lvl<T> &f, Lvl<T> &g, int lvl, int epi, bool rz, const MixView &mx) {
   const double s = sizeof(T);
-  double bytes = s * (3.0 * nodes(f) + nodes(g));
-  if (epi == EPI_MIXED) bytes = s * (2.0 * nodes(f) + nodes(g)) + 28.0 * nodes(f);
+  const double rin = rz ? 0.0 : 1.0;  // the staged iterate, unless recomputed
+  double bytes = s * ((rin + 2.0) * nodes(f) + nodes(g));
+  if (epi == EPI_MIXED) bytes = s * ((rin + 1.0) * nodes(f) + nodes(g)) + 28.0 * nodes(f);
   int np = 0;
   launch(c, KC_UP, lvl, bytes, [&] {
     switch (c->o.nu2) {
-      case 1: np = launch_up_ns<T, 1>(c, f, g, epi, mx); break;
-      case 2: np = launch_up_ns<T, 2>(c, f, g, epi, mx); break;
-      default: np = launch_up_ns<T, 3>(c, f, g, epi, mx); break;
+      case 1: np = launch_up_ns<T, 1>(c, f, g, epi, rz, mx); break;
+      case 2: np = launch_up_ns<T, 2>(c, f, g, epi, rz, mx); break;
+      default: np = launch_up_ns<T, 3>(c, f, g, epi, rz, mx); break;
     }
   });
   return np;
@@ -452,11 +467,13 @@ template <typename T> void coarse(Ctx *c, Hier<T> &H, bool zero_first, double *s
 // returns the number of partials it writes (0 when lc == 0).
 template <typename T> int vcycle_fused(Ctx *c, Hier<T> &H, bool zero0, int epi, const MixView &mx, double *snap) {
   const int lc = H.lc;
-  for (int l = 0; l < lc; ++l) down(c, H.lv[l], H.lv[l + 1], l, l > 0 || zero0);
+  auto zi = [&](int l) { return l > 0 || zero0; };
+  auto ep = [&](int l) { return l == 0 ? epi : (int)EPI_NONE; };
+  for (int l = 0; l < lc; ++l) down(c, H.lv[l], H.lv[l + 1], l, zi(l), !recompute_presmooth(c, zi(l), ep(l)));
   coarse(c, H, lc > 0 || zero0, lc == 0 ? snap : nullptr);
   int np = 0;
   for (int l = lc - 1; l >= 0; --l) {
-    const int n = up(c, H.lv[l], H.lv[l + 1], l, l == 0 ? epi : EPI_NONE, mx);
+    const int n = up(c, H.lv[l], H.lv[l + 1], l, ep(l), recompute_presmooth(c, zi(l), ep(l)), mx);
     if (l == 0) np = n;
   }
   return np;
@@ -1046,9 +1063,11 @@ static int create_impl(int segX, int segY, double hx, double hy, const double *f
   fill(c, v.rhs, v.elems(), 0.0);
   {
     // partials: one per tile of the largest launch.  Streaming tiles number
-    // at most max(resident wavefronts, column strips); refinement tiles are
-    // 512 x 64.
-    const long long strips = (ny - 2) / 112 + 1;
+    // at most max(resident wavefronts, column strips), the strips being
+    // narrowest for the widest halo (ascent with a recomputed pre-sweep, 3
+    // post-sweeps and a residual epilogue); refinement tiles are 512 x 64.
+    constexpr int kMinStripCols = SCOLS - 2 * s_up_halo(4, 3);
+    const long long strips = (ny - 2) / kMinStripCols + 1;
     long long stiles = std::max<long long>(strips, 64LL * std::max(1, c->num_cu));
     long long rtiles = (long long)refine_strips(ny) * refine_row_blocks(nx);
     c->part_cap = (int)std::max<long long>(1 << 16, std::max(stiles, rtiles) + 64);

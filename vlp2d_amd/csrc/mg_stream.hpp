@@ -46,6 +46,12 @@
 //             FP64 refinement u64 += e, r32 = (float)(fn - Lap u64), partial
 //             sums of r^2 and partial max |e|; e itself is not written;
 //             H = S + 2).
+// On a zero-initialised level the pre-smoothed iterate is a local function
+// of the right-hand side (radius 2 NS1 for NS1 pre-sweeps).  There the
+// descent may skip writing it (k_sdown STORE = false) and the ascent
+// recomputes it in front of its own pipeline (k_sup N1 = NS1): 2 N1 more
+// half-sweeps on the right-hand-side rows it streams anyway, H + 2 N1, and
+// one write plus one read of the level less per cycle.
 #pragma once
 #include <algorithm>
 #include <type_traits>
@@ -211,6 +217,7 @@ __device__ __forceinline__ void s_store(T *uout, int pitch, int gi, const Lane &
 template <typename T, int NS> struct SweepState {
   static constexpr int S = 2 * NS;
   static constexpr int P = s_period(NS);
+  static_assert(S + 4 <= P, "right-hand-side ring too short for the pipeline");
   T U[S + 2][6];  // U[s + 1][p % 6]: level s in -1..S
   T F[P][2];      // right-hand side of row p (0 on Laplace levels and outside the domain)
   T RU[3][2];     // prefetched staged rows (raw)
@@ -235,13 +242,13 @@ template <typename T, int NS> struct SweepState {
   }
 };
 
-// Apply half-sweeps 1..S at step J (row q = q0 + J): half-sweep s updates
-// row q - s, using level s-1 rows q-s-1, q-s, q-s+1.  On edge tiles nodes
-// outside the domain interior keep their value.
-template <typename T, int NS, bool EQ, bool PW2, bool EDGE, int J>
+// Apply half-sweeps SA..SB (default 1..S) at step J (row q = q0 + J):
+// half-sweep s updates row q - s, using level s-1 rows q-s-1, q-s, q-s+1.
+// On edge tiles nodes outside the domain interior keep their value.
+template <typename T, int NS, bool EQ, bool PW2, bool EDGE, int J, int SA = 1, int SB = 2 * NS>
 __device__ __forceinline__ void s_sweeps(SweepState<T, NS> &st, const LevelView<T> &a, const Lane &L, int gi0, int q) {
-  constexpr int S = 2 * NS, P = SweepState<T, NS>::P;
-  static_for<1, S + 1>([&](auto sc) {
+  constexpr int P = SweepState<T, NS>::P;
+  static_for<SA, SB + 1>([&](auto sc) {
     constexpr int s = decltype(sc)::value;
     constexpr int p = J - s;  // phase of the row
     constexpr int cs = md<2>(s - 1);
@@ -312,7 +319,7 @@ template <int H> __device__ __forceinline__ bool s_tile(int R, int tiles_i, int 
 // ---------------------------------------------------------------------------
 // Descent
 // ---------------------------------------------------------------------------
-template <typename T, int NS, bool ZI, bool EQ, bool PW2, bool EDGE>
+template <typename T, int NS, bool ZI, bool STORE, bool EQ, bool PW2, bool EDGE>
 __device__ __forceinline__ void sdown_body(const LevelView<T> &a, const STile &tl, T *__restrict__ rhsc, int ncx,
                                            int ncy, int pc) {
   constexpr int S = 2 * NS, H = s_down_halo(NS), P = s_period(NS), TJ = SCOLS - 2 * H;
@@ -343,8 +350,8 @@ __device__ __forceinline__ void sdown_body(const LevelView<T> &a, const STile &t
       st.U[0][md<6>(J)] = st.RU[md<3>(J)][er];
       st.U[1][md<6>(J)] = st.RU[md<3>(J)][1 - er];
       s_sweeps<T, NS, EQ, PW2, EDGE, J>(st, a, L, gi0, q);
-      // store row q - S
-      {
+      // store row q - S (unless the ascent recomputes it)
+      if constexpr (STORE) {
         const int p = q - S, gi = gi0 + p;
         bool go = p >= H && p < H + R;
         if constexpr (EDGE) go = go && gi >= 1 && gi <= a.nx - 2;
@@ -381,30 +388,40 @@ __device__ __forceinline__ void sdown_body(const LevelView<T> &a, const STile &t
   }
 }
 
-template <typename T, int NS, bool ZI, bool EQ>
+// STORE = false: the smoothed iterate is not written (zero-initialised
+// levels whose ascent recomputes it, k_sup with N1 = NS).
+template <typename T, int NS, bool ZI, bool EQ, bool STORE = true>
 __global__ void __launch_bounds__(64) k_sdown(LevelView<T> a, T *__restrict__ rhsc, int ncx, int ncy, int pc) {
+  static_assert(STORE || ZI, "only a zero-initialised descent can be recomputed");
   constexpr int H = s_down_halo(NS);
   STile tl;
   if (!s_tile<H>(a.rows, a.tiles_i, a.tiles_j, a.chunk, a.nx, a.ny, tl)) return;
   if (a.pw2) {
-    if (tl.interior) sdown_body<T, NS, ZI, EQ, true, false>(a, tl, rhsc, ncx, ncy, pc);
-    else sdown_body<T, NS, ZI, EQ, true, true>(a, tl, rhsc, ncx, ncy, pc);
+    if (tl.interior) sdown_body<T, NS, ZI, STORE, EQ, true, false>(a, tl, rhsc, ncx, ncy, pc);
+    else sdown_body<T, NS, ZI, STORE, EQ, true, true>(a, tl, rhsc, ncx, ncy, pc);
   } else {
-    sdown_body<T, NS, ZI, EQ, false, true>(a, tl, rhsc, ncx, ncy, pc);
+    sdown_body<T, NS, ZI, STORE, EQ, false, true>(a, tl, rhsc, ncx, ncy, pc);
   }
 }
 
 // ---------------------------------------------------------------------------
 // Ascent
 // ---------------------------------------------------------------------------
-template <typename T, int NS, int EPI, bool EQ, bool PW2, bool EDGE>
+template <typename T, int NS, int N1, int EPI, bool EQ, bool PW2, bool EDGE>
 __device__ __forceinline__ void sup_body(const LevelView<T> &a, const STile &tl, const T *__restrict__ uc, int ncx,
                                          int ncy, int pc, double *__restrict__ part, const MixView &mx) {
-  constexpr int S = 2 * NS, H = s_up_halo(NS, EPI), P = s_period(NS), TJ = SCOLS - 2 * H;
+  // N1 > 0: the level was zero-initialised and pre-smoothed with N1 sweeps
+  // by the descent, which did not store it; the pipeline recomputes it from
+  // the right-hand side (half-sweeps 1..S1 on staged zeros, the same
+  // operations in the same order, hence the same bits) and adds the
+  // prolongation to level S1 of each row.
+  constexpr int S1 = 2 * N1, ST = S1 + 2 * NS;
+  constexpr int H = s_up_halo(N1 + NS, EPI), P = s_period(N1 + NS), TJ = SCOLS - 2 * H;
+  static_assert(N1 == 0 || EPI != 2, "the update metric needs the staged iterate");
   const int R = a.rows, LR = R + 2 * H;  // LR is a multiple of P
   const int gi0 = tl.gi0, cb = tl.cb;
   const Lane L = make_lane<T>(cb, H, TJ, a.ny, a.pitch);
-  SweepState<T, NS> st;
+  SweepState<T, N1 + NS> st;
   st.zero();
   // coarse rows kk = 0, 1, ... of the strip: global row gi0/2 + kk, column
   // Jc = cb/2 + m (fine column gj = 2 Jc); ring slot kk % 3.  The right
@@ -438,8 +455,8 @@ __device__ __forceinline__ void sup_body(const LevelView<T> &a, const STile &tl,
   double acc = 0.0, emax = 0.0;
   static_assert(EPI != 3 || sizeof(T) == 4, "the refinement epilogue runs on the FP32 correction");
   // EPI 3: FP64 iterate rows (raw, then refined x = u64 + e) and FP64
-  // right-hand side; row r is loaded at step r + S - 2, refined at step
-  // r + S (when e of row r is final), used by the residuals of rows r-1..r+1.
+  // right-hand side; row r is loaded at step r + ST - 2, refined at step
+  // r + ST (when e of row r is final), used by the residuals of rows r-1..r+1.
   constexpr int N3 = EPI == 3 ? 3 : 1;
   double X64[N3][2], XR[N3][2], FN[N3][2];
 #pragma unroll
@@ -447,25 +464,32 @@ __device__ __forceinline__ void sup_body(const LevelView<T> &a, const STile &tl,
   const unsigned boff8 = L.ld ? (unsigned)L.gj * 8u : 0u;
   static_for<-2, 0>([&](auto jc) {
     constexpr int J = decltype(jc)::value;
-    s_prefetch<T, NS, false, EDGE, J>(st, a, L, gi0, J, LR);
+    s_prefetch<T, N1 + NS, (N1 > 0), EDGE, J>(st, a, L, gi0, J, LR);
   });
 #pragma unroll 1
   for (int q0 = 0; q0 < LR; q0 += P) {
     static_for<0, P>([&](auto jc) {
       constexpr int J = decltype(jc)::value;
       const int q = q0 + J;
-      s_prefetch<T, NS, false, EDGE, J>(st, a, L, gi0, q, LR);
-      constexpr int kk = J >> 1;  // coarse row of this fine row (phase)
-      if constexpr ((J & 1) == 0) load_coarse((q >> 1) + 2, C[md<3>(kk + 2)], CR[md<3>(kk + 2)]);
-      // staged row q + prolongation on interior nodes
+      s_prefetch<T, N1 + NS, (N1 > 0), EDGE, J>(st, a, L, gi0, q, LR);
+      // staged row q (level -1: red element J & 1, level 0: black element)
+      constexpr int er = J & 1;
+      if constexpr (N1 > 0) {
+        st.U[0][md<6>(J)] = st.RU[md<3>(J)][er];
+        st.U[1][md<6>(J)] = st.RU[md<3>(J)][1 - er];
+        s_sweeps<T, N1 + NS, EQ, PW2, EDGE, J, 1, S1>(st, a, L, gi0, q);
+      }
+      // prolongation of row pr = q - S1, whose level S1 is now complete
       {
-        const int gi = gi0 + q;
-        T v0 = st.RU[md<3>(J)][0], v1 = st.RU[md<3>(J)][1];
+        constexpr int JP = md<P>(J - S1);  // phase of row pr (P is a multiple of 6)
+        constexpr int kk = JP >> 1;          // coarse row of row pr, mod 3
+        const int pr = q - S1, gi = gi0 + pr;
+        if constexpr ((JP & 1) == 0) load_coarse(pr / 2 + 2, C[md<3>(kk + 2)], CR[md<3>(kk + 2)]);
         const T c00 = C[md<3>(kk)];
         const T c01r = from_right(c00);
         const T c01 = last ? CR[md<3>(kk)] : c01r;
         T p0, p1;
-        if constexpr ((J & 1) == 0) {
+        if constexpr ((JP & 1) == 0) {
           p0 = c00;
           p1 = (c00 + c01) * T(0.5);
         } else {
@@ -475,50 +499,63 @@ __device__ __forceinline__ void sup_body(const LevelView<T> &a, const STile &tl,
           p0 = (c00 + c10) * T(0.5);
           p1 = (c00 + c01 + c10 + c11) * T(0.25);
         }
+        bool a0 = true, a1 = true;
         if constexpr (EDGE) {
           const bool rin = gi >= 1 && gi <= a.nx - 2;
-          if (rin && L.in0) v0 = v0 + p0;
-          if (rin && L.in1) v1 = v1 + p1;
-        } else {
-          v0 = v0 + p0;
-          v1 = v1 + p1;
+          a0 = rin && L.in0;
+          a1 = rin && L.in1;
         }
-        constexpr int er = J & 1;
-        st.U[0][md<6>(J)] = er == 0 ? v0 : v1;
-        st.U[1][md<6>(J)] = er == 0 ? v1 : v0;
-        if constexpr (EPI == 2) {
-          V0[md<NV>(J)][0] = v0;
-          V0[md<NV>(J)][1] = v1;
+        if constexpr (N1 == 0) {
+          T v0 = st.RU[md<3>(J)][0], v1 = st.RU[md<3>(J)][1];
+          if (a0) v0 = v0 + p0;
+          if (a1) v1 = v1 + p1;
+          st.U[0][md<6>(J)] = er == 0 ? v0 : v1;
+          st.U[1][md<6>(J)] = er == 0 ? v1 : v0;
+          if constexpr (EPI == 2) {
+            V0[md<NV>(J)][0] = v0;
+            V0[md<NV>(J)][1] = v1;
+          }
+        } else {
+          // level S1 holds colour 1 (black): element (pr + 1) & 1; the
+          // red element is overwritten by half-sweep S1 + 1 without being
+          // read (nodes that sweep leaves alone get no prolongation either)
+          constexpr int eb = md<2>(JP + 1);
+          T &v = st.U[S1 + 1][md<6>(JP)];
+          if constexpr (eb == 0) {
+            if (a0) v = v + p0;
+          } else {
+            if (a1) v = v + p1;
+          }
         }
       }
       if constexpr (EPI == 3) {
-        // prefetch FP64 rows: iterate row q - S + 2, right-hand side row q - S + 1
-        const int r = q - S + 2, gr = gi0 + r;
+        // prefetch FP64 rows: iterate row q - ST + 2, right-hand side row q - ST + 1
+        const int r = q - ST + 2, gr = gi0 + r;
         Pair<double> x{0.0, 0.0}, f{0.0, 0.0};
         bool go = r >= 0 && r < LR;
         if constexpr (EDGE) go = go && gr >= 0 && gr <= a.nx - 1 && L.ld;
         if (go) x = ld_pair(row_of(mx.u64in, gr, a.pitch), boff8);
-        const int rf = q - S + 1, gf = gi0 + rf;
+        const int rf = q - ST + 1, gf = gi0 + rf;
         bool gof = rf >= 0 && rf < LR;
         if constexpr (EDGE) gof = gof && gf >= 0 && gf <= a.nx - 1 && L.ld;
         if (gof) f = ld_pair(row_of(mx.fn64, gf, a.pitch), boff8);
-        X64[md<3>(J - S + 2)][0] = x.x0;
-        X64[md<3>(J - S + 2)][1] = x.x1;
-        FN[md<3>(J - S + 1)][0] = f.x0;
-        FN[md<3>(J - S + 1)][1] = f.x1;
+        X64[md<3>(J - ST + 2)][0] = x.x0;
+        X64[md<3>(J - ST + 2)][1] = x.x1;
+        FN[md<3>(J - ST + 1)][0] = f.x0;
+        FN[md<3>(J - ST + 1)][1] = f.x1;
       }
-      s_sweeps<T, NS, EQ, PW2, EDGE, J>(st, a, L, gi0, q);
-      // store row q - S (+ update metric)
+      s_sweeps<T, N1 + NS, EQ, PW2, EDGE, J, S1 + 1, ST>(st, a, L, gi0, q);
+      // store row q - ST (+ update metric)
       {
-        const int p = q - S, gi = gi0 + p;
-        const T x0 = st.template full<S, J, S, 0>(), x1 = st.template full<S, J, S, 1>();
+        const int p = q - ST, gi = gi0 + p;
+        const T x0 = st.template full<ST, J, ST, 0>(), x1 = st.template full<ST, J, ST, 1>();
         bool orow = p >= H && p < H + R;
         if constexpr (EDGE) orow = orow && gi >= 1 && gi <= a.nx - 2;
         if constexpr (EPI != 3) {
           if (orow) s_store<T, EDGE>(a.uout, a.pitch, gi, L, x0, x1);
         } else {
-          // refined FP64 row q - S: u64 + e on interior nodes
-          constexpr int sx = md<3>(J - S);
+          // refined FP64 row q - ST: u64 + e on interior nodes
+          constexpr int sx = md<3>(J - ST);
           double r0 = X64[sx][0], r1 = X64[sx][1];
           if constexpr (EDGE) {
             const bool rin = gi >= 1 && gi <= a.nx - 2;
@@ -532,24 +569,24 @@ __device__ __forceinline__ void sup_body(const LevelView<T> &a, const STile &tl,
           XR[sx][1] = r1;
         }
         if constexpr (EPI == 2) {
-          constexpr int pv = md<NV>(J - S);
+          constexpr int pv = md<NV>(J - ST);
           if (orow && L.oc) {
             if (!EDGE || L.in0) acc = nanmax(acc, fabs((double)(x0 - V0[pv][0])));
             if (!EDGE || L.in1) acc = nanmax(acc, fabs((double)(x1 - V0[pv][1])));
           }
         }
       }
-      // refinement of row q - S - 1: u64out, r32out = (float)(fn - Lap u64out)
+      // refinement of row q - ST - 1: u64out, r32out = (float)(fn - Lap u64out)
       // (UtilsOpLap.cs:15-20 order, FP64), partial sum r^2 and max |e|
       if constexpr (EPI == 3) {
-        constexpr int d = S + 1;
+        constexpr int d = ST + 1;
         constexpr int pr = J - d;
         const int p = q - d, gi = gi0 + p;
         const double x0 = XR[md<3>(pr)][0], x1 = XR[md<3>(pr)][1];
         const double a0 = XR[md<3>(pr - 1)][0], a1 = XR[md<3>(pr - 1)][1];
         const double b0 = XR[md<3>(pr + 1)][0], b1 = XR[md<3>(pr + 1)][1];
         const double xl = from_left(x1), xr = from_right(x0);
-        const T e0 = st.template full<S, J, d, 0>(), e1 = st.template full<S, J, d, 1>();
+        const T e0 = st.template full<ST, J, d, 0>(), e1 = st.template full<ST, J, d, 1>();
         bool orow = p >= H && p < H + R;
         if constexpr (EDGE) orow = orow && gi >= 1 && gi <= a.nx - 2;
         if (orow && L.oc) {
@@ -585,14 +622,14 @@ __device__ __forceinline__ void sup_body(const LevelView<T> &a, const STile &tl,
           }
         }
       }
-      // residual norm of row q - S - 1
+      // residual norm of row q - ST - 1
       if constexpr (EPI == 1) {
-        constexpr int d = S + 1;
+        constexpr int d = ST + 1;
         constexpr int pr = J - d;
         const int p = q - d, gi = gi0 + p;
-        const T x0 = st.template full<S, J, d, 0>(), x1 = st.template full<S, J, d, 1>();
-        const T a0 = st.template full<S, J, d + 1, 0>(), b0 = st.template full<S, J, d - 1, 0>();
-        const T a1 = st.template full<S, J, d + 1, 1>(), b1 = st.template full<S, J, d - 1, 1>();
+        const T x0 = st.template full<ST, J, d, 0>(), x1 = st.template full<ST, J, d, 1>();
+        const T a0 = st.template full<ST, J, d + 1, 0>(), b0 = st.template full<ST, J, d - 1, 0>();
+        const T a1 = st.template full<ST, J, d + 1, 1>(), b1 = st.template full<ST, J, d - 1, 1>();
         const T xl = from_left(x1), xr = from_right(x0);
         bool orow = p >= H && p < H + R;
         if constexpr (EDGE) orow = orow && gi >= 1 && gi <= a.nx - 2;
@@ -620,17 +657,18 @@ __device__ __forceinline__ void sup_body(const LevelView<T> &a, const STile &tl,
 
 // EPI 3 requires the FP64 steps of mx to be powers of two exactly when the
 // FP32 steps of a are (the host checks).
-template <typename T, int NS, int EPI, bool EQ>
+// N1 > 0 pairs with k_sdown<T, N1, true, EQ, false> on the same level.
+template <typename T, int NS, int EPI, bool EQ, int N1 = 0>
 __global__ void __launch_bounds__(64) k_sup(LevelView<T> a, const T *__restrict__ uc, int ncx, int ncy, int pc,
                                             double *__restrict__ part, MixView mx) {
-  constexpr int H = s_up_halo(NS, EPI);
+  constexpr int H = s_up_halo(N1 + NS, EPI);
   STile tl;
   if (!s_tile<H>(a.rows, a.tiles_i, a.tiles_j, a.chunk, a.nx, a.ny, tl)) return;
   if (a.pw2) {
-    if (tl.interior) sup_body<T, NS, EPI, EQ, true, false>(a, tl, uc, ncx, ncy, pc, part, mx);
-    else sup_body<T, NS, EPI, EQ, true, true>(a, tl, uc, ncx, ncy, pc, part, mx);
+    if (tl.interior) sup_body<T, NS, N1, EPI, EQ, true, false>(a, tl, uc, ncx, ncy, pc, part, mx);
+    else sup_body<T, NS, N1, EPI, EQ, true, true>(a, tl, uc, ncx, ncy, pc, part, mx);
   } else {
-    sup_body<T, NS, EPI, EQ, false, true>(a, tl, uc, ncx, ncy, pc, part, mx);
+    sup_body<T, NS, N1, EPI, EQ, false, true>(a, tl, uc, ncx, ncy, pc, part, mx);
   }
 }
 
