@@ -127,6 +127,7 @@ __global__ void __launch_bounds__(CNT) k_coarse(CoarseArgs<T> a) {
 template <typename T> struct LdsLevel {
   T *u, *f;
   int nx, ny;
+  int ly;  // ny = 2^ly + 1 (every level of a power-of-two grid)
 };
 
 // LDS bytes k_coarse_lds needs for levels [l0, L) of the given sizes.
@@ -136,23 +137,28 @@ inline size_t coarse_lds_bytes(const int *nx, const int *ny, int l0, int L, size
   return n * elem;
 }
 
+__device__ __forceinline__ int ilog2(int x) { return 31 - __builtin_clz(x); }
+
 // Level l's arrays in LDS: levels l0 .. l-1 precede it (the offsets are
 // uniform scalar arithmetic on the kernel arguments).
 template <typename T> __device__ inline LdsLevel<T> cl_level(const CoarseArgs<T> &a, T *sm, int l) {
   int o = 0;
   for (int k = a.l0; k < l; ++k) o += 2 * a.lv[k].nx * a.lv[k].ny;
   const int n = a.lv[l].nx * a.lv[l].ny;
-  return LdsLevel<T>{sm + o, sm + o + n, a.lv[l].nx, a.lv[l].ny};
+  return LdsLevel<T>{sm + o, sm + o + n, a.lv[l].nx, a.lv[l].ny, ilog2(a.lv[l].ny - 1)};
 }
 
+// Work items are mapped to nodes with shifts and masks only (row lengths are
+// powers of two plus one): one item per node of one colour in a half-sweep,
+// one per interior node (column 0 of each row idle) elsewhere.
 template <typename T> __device__ inline void cl_smooth(const LdsLevel<T> &v, const CoarseLevel<T> &g, int n) {
-  const int wc = (v.ny - 1) / 2;  // items per row and colour (ny odd)
-  const int items = (v.nx - 2) * wc;
+  const int lw = v.ly - 1;  // 2^lw items per row and colour
+  const int items = (v.nx - 2) << lw;
   const int P = v.ny;
   for (int s = 0; s < n; ++s)
     for (int color = 0; color < 2; ++color) {
       for (int p = threadIdx.x; p < items; p += CNT) {
-        const int i = 1 + p / wc, m = p - (p / wc) * wc;
+        const int i = 1 + (p >> lw), m = p & ((1 << lw) - 1);
         const int j = 2 * m + 1 + ((i + color + 1) & 1);
         if (j > v.ny - 2) continue;
         const int c = i * P + j;
@@ -190,11 +196,13 @@ __global__ void __launch_bounds__(CNT) k_coarse_lds(CoarseArgs<T> a) {
     // zeroed when zero_first) and the right-hand side (0 on Laplace levels)
     const CoarseLevel<T> &g = a.lv[a.l0];
     const LdsLevel<T> v = cl_level(a, sm, a.l0);
-    for (int p = threadIdx.x; p < v.nx * v.ny; p += CNT) {
-      const int i = p / v.ny, j = p - i * v.ny;
+    const int lr = v.ly + 1;  // 2^lr >= ny items per row
+    for (int p = threadIdx.x; p < v.nx << lr; p += CNT) {
+      const int i = p >> lr, j = p & ((1 << lr) - 1);
+      if (j >= v.ny) continue;
       const bool in = i > 0 && i < v.nx - 1 && j > 0 && j < v.ny - 1;
-      v.u[p] = a.zero_first && in ? T(0) : g.u[i * g.pitch + j];
-      v.f[p] = g.lap ? T(0) : g.f[i * g.pitch + j];
+      v.u[i * v.ny + j] = a.zero_first && in ? T(0) : g.u[i * g.pitch + j];
+      v.f[i * v.ny + j] = g.lap ? T(0) : g.f[i * g.pitch + j];
     }
     __syncthreads();
   }
@@ -202,9 +210,9 @@ __global__ void __launch_bounds__(CNT) k_coarse_lds(CoarseArgs<T> a) {
     const LdsLevel<T> f = cl_level(a, sm, l), g = cl_level(a, sm, l + 1);
     const CoarseLevel<T> &fg = a.lv[l];
     cl_smooth(f, fg, a.nu1);
-    const int n = (g.nx - 2) * (g.ny - 2);
-    for (int p = threadIdx.x; p < n; p += CNT) {
-      const int I = 1 + p / (g.ny - 2), J = 1 + p - (p / (g.ny - 2)) * (g.ny - 2);
+    for (int p = threadIdx.x; p < (g.nx - 2) << g.ly; p += CNT) {
+      const int I = 1 + (p >> g.ly), J = p & ((1 << g.ly) - 1);
+      if (J == 0) continue;
       const int c = 2 * I * f.ny + 2 * J;
       const T r0 = cl_resid(f, fg, c), rip = cl_resid(f, fg, c + f.ny), rim = cl_resid(f, fg, c - f.ny);
       const T rjp = cl_resid(f, fg, c + 1), rjm = cl_resid(f, fg, c - 1);
@@ -218,9 +226,9 @@ __global__ void __launch_bounds__(CNT) k_coarse_lds(CoarseArgs<T> a) {
   cl_smooth(cl_level(a, sm, a.L - 1), a.lv[a.L - 1], a.nuc);
   for (int l = a.L - 2; l >= a.l0; --l) {
     const LdsLevel<T> f = cl_level(a, sm, l), g = cl_level(a, sm, l + 1);
-    const int n = (f.nx - 2) * (f.ny - 2);
-    for (int p = threadIdx.x; p < n; p += CNT) {
-      const int i = 1 + p / (f.ny - 2), j = 1 + p - (p / (f.ny - 2)) * (f.ny - 2);
+    for (int p = threadIdx.x; p < (f.nx - 2) << f.ly; p += CNT) {
+      const int i = 1 + (p >> f.ly), j = p & ((1 << f.ly) - 1);
+      if (j == 0) continue;
       const int c = i * f.ny + j;
       const T v = f.u[c] + prolong_corr(g.u, g.ny, i, j);
       f.u[c] = v;
@@ -234,9 +242,9 @@ __global__ void __launch_bounds__(CNT) k_coarse_lds(CoarseArgs<T> a) {
     // live and die in LDS
     const CoarseLevel<T> &g = a.lv[a.l0];
     const LdsLevel<T> v = cl_level(a, sm, a.l0);
-    const int n = (v.nx - 2) * (v.ny - 2);
-    for (int p = threadIdx.x; p < n; p += CNT) {
-      const int i = 1 + p / (v.ny - 2), j = 1 + p - (p / (v.ny - 2)) * (v.ny - 2);
+    for (int p = threadIdx.x; p < (v.nx - 2) << v.ly; p += CNT) {
+      const int i = 1 + (p >> v.ly), j = p & ((1 << v.ly) - 1);
+      if (j == 0) continue;
       g.u[i * g.pitch + j] = v.u[i * v.ny + j];
     }
   }

@@ -182,26 +182,6 @@ __global__ void __launch_bounds__(BX *BY) k_absdiff_max(const T *__restrict__ a,
   if (tid == 0) part[blockIdx.y * gridDim.x + blockIdx.x] = m;
 }
 
-// FP64 residual of the level-0 iterate; partial sums of r^2 per block.
-// If r32 != nullptr also stores (float) r there (the FP32 V-cycle's rhs).
-__global__ void __launch_bounds__(BX *BY) k_resnorm64(const double *__restrict__ u, const double *__restrict__ rhs,
-                                                      int nx, int ny, int pitch, double hx2, double hy2,
-                                                      double *__restrict__ part, float *__restrict__ r32) {
-  __shared__ double sm[BX * BY / 64];
-  const int j = blockIdx.x * BX + threadIdx.x + 1;
-  double s = 0.0;
-  if (j <= ny - 2)
-    for (int i = blockIdx.y * BY + threadIdx.y + 1; i <= nx - 2; i += gridDim.y * BY) {
-      const int c = i * pitch + j;
-      double r = resid_point(u, rhs, c, pitch, hx2, hy2);
-      if (r32) r32[c] = (float)r;
-      s += r * r;
-    }
-  const int tid = threadIdx.y * BX + threadIdx.x;
-  s = block_reduce<false>(s, sm, tid, BX * BY);
-  if (tid == 0) part[blockIdx.y * gridDim.x + blockIdx.x] = s;
-}
-
 // u64 += (double) e32 on interior nodes; partial max |e|.
 __global__ void __launch_bounds__(BX *BY) k_update_mixed(double *__restrict__ u, const float *__restrict__ e, int nx,
                                                          int ny, int pitch, double *__restrict__ part) {

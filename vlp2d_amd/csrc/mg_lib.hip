@@ -21,6 +21,7 @@
 #include <atomic>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <string>
@@ -319,16 +320,18 @@ template <typename T> LevelView<T> view(const Lvl<T> &v, const T *uin, T *uout, 
   return a;
 }
 
+bool is_pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
+
 // Resident 64-thread workgroups (wavefronts) of a kernel on the whole GPU.
 int resident_waves(Ctx *c, const void *fn) {
   auto it = c->occ.find(fn);
   if (it != c->occ.end()) return it->second;
   int nb = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, 64, 0) != hipSuccess || nb < 1) {
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, 64 * kWaves, 0) != hipSuccess || nb < 1) {
     (void)hipGetLastError();
     nb = 4;
   }
-  const int q = nb * std::max(1, c->num_cu);
+  const int q = nb * kWaves * std::max(1, c->num_cu);
   c->occ[fn] = q;
   return q;
 }
@@ -367,7 +370,7 @@ void launch_down_ns(Ctx *c, Lvl<T> &f, Lvl<T> &g, bool zi, bool store) {
   else if (store) k = f.eq ? (K)k_sdown<T, NS, true, true> : (K)k_sdown<T, NS, true, false>;
   else k = f.eq ? (K)k_sdown<T, NS, true, true, false> : (K)k_sdown<T, NS, true, false, false>;
   const LevelView<T> a = sview(c, (const void *)k, f, f.u, f.u2, H, P);
-  hipLaunchKernelGGL(k, dim3(8 * a.chunk), dim3(64), 0, c->st, a, g.rhs, g.nx, g.ny, g.pitch);
+  hipLaunchKernelGGL(k, dim3(8 * a.chunk), dim3(64 * kWaves), 0, c->st, a, g.rhs, g.nx, g.ny, g.pitch);
 }
 
 template <typename T> void down(Ctx *c, Lvl<T> &f, Lvl<T> &g, int lvl, bool zi, bool store) {
@@ -388,7 +391,7 @@ int launch_sup(Ctx *c, Lvl<T> &f, const Lvl<T> &g, const MixView &mx) {
   using K = void (*)(LevelView<T>, const T *, int, int, int, double *, MixView);
   const K k = f.eq ? (K)k_sup<T, NS, EPI, true, N1> : (K)k_sup<T, NS, EPI, false, N1>;
   const LevelView<T> a = sview(c, (const void *)k, f, f.u2, f.u, H, P);
-  hipLaunchKernelGGL(k, dim3(8 * a.chunk), dim3(64), 0, c->st, a, g.u, g.nx, g.ny, g.pitch, c->part, mx);
+  hipLaunchKernelGGL(k, dim3(8 * a.chunk), dim3(64 * kWaves), 0, c->st, a, g.u, g.nx, g.ny, g.pitch, c->part, mx);
   return a.tiles_i * a.tiles_j;
 }
 
@@ -448,10 +451,15 @@ template <typename T> void coarse(Ctx *c, Hier<T> &H, bool zero_first, double *s
   a.snap = snap;
   // every level in LDS when it fits in one workgroup's share, else in global
   // memory (the working set then sits in L2)
+  // (the LDS kernel maps work items with shifts: rows of 2^k + 1 nodes)
   int nx[kMaxLevels], ny[kMaxLevels];
-  for (int l = H.lc; l < L; ++l) nx[l] = H.lv[l].nx, ny[l] = H.lv[l].ny;
+  bool pow2_rows = true;
+  for (int l = H.lc; l < L; ++l) {
+    nx[l] = H.lv[l].nx, ny[l] = H.lv[l].ny;
+    pow2_rows = pow2_rows && ny[l] >= 3 && is_pow2(ny[l] - 1);
+  }
   const size_t lds = coarse_lds_bytes(nx, ny, H.lc, L, sizeof(T));
-  if (!(c->o.flags & VLP2D_FLAG_COARSE_L2) && lds <= (size_t)c->max_lds) {
+  if (!(c->o.flags & VLP2D_FLAG_COARSE_L2) && pow2_rows && lds <= (size_t)c->max_lds) {
     if (lds > 65536) (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_coarse_lds<T>),
                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     launch(c, KC_COARSE, H.lc, bytes,
@@ -481,10 +489,14 @@ template <typename T> int vcycle_fused(Ctx *c, Hier<T> &H, bool zero0, int epi, 
 
 // residual norm^2 of level 0 (FP64) -> dscal[slot]; optional FP32 copy
 void resnorm(Ctx *c, Lvl<double> &v, const double *u, int slot, float *r32) {
-  dim3 g = grid_red(v.ny, v.nx);
-  int np = (int)(g.x * g.y);
+  // column-streaming tiles of the refinement kernel without the update
+  const int rows = refine_row_blocks(v.nx);
+  const int np = refine_strips(v.ny) * rows;
+  const int chunk = (np + 7) / 8;
   launch(c, KC_RESNORM, 0, (16.0 + (r32 ? 4.0 : 0.0)) * nodes(v), [&] {
-    hipLaunchKernelGGL(k_resnorm64, g, kBlock, 0, c->st, u, v.rhs, v.nx, v.ny, v.pitch, (double)v.hx2, (double)v.hy2, c->part, r32);
+    hipLaunchKernelGGL(k_refine_stream<false>, dim3(8 * chunk), dim3(256), 0, c->st, u, nullptr, nullptr, v.rhs, r32,
+                       v.nx, v.ny, v.pitch, (double)v.hx2, (double)v.hy2, (double)v.ihx2, (double)v.ihy2,
+                       v.pw2 ? 1 : 0, rows, np, chunk, c->part, nullptr);
   });
   launch(c, KC_REDUCE, 0, 8.0 * np, [&] { hipLaunchKernelGGL(k_reduce<false>, dim3(1), dim3(RED_THREADS), 0, c->st, c->part, np, c->dscal + slot); });
 }
@@ -570,7 +582,7 @@ void mixed_step(Ctx *c, Lvl<double> &P) {
   const int np = refine_strips(P.ny) * rows;
   const int chunk = (np + 7) / 8;
   launch(c, KC_UPDATE, 0, 32.0 * nodes(P), [&] {
-    hipLaunchKernelGGL(k_refine_stream, dim3(8 * chunk), dim3(256), 0, c->st, P.u, P.u2, z.u, P.rhs, z.rhs2, P.nx,
+    hipLaunchKernelGGL(k_refine_stream<true>, dim3(8 * chunk), dim3(256), 0, c->st, P.u, P.u2, z.u, P.rhs, z.rhs2, P.nx,
                        P.ny, P.pitch, (double)P.hx2, (double)P.hy2, (double)P.ihx2, (double)P.ihy2, P.pw2 ? 1 : 0, rows,
                        np, chunk, c->part, c->part2);
   });
@@ -742,8 +754,6 @@ void invalidate_graphs(Ctx *c) {
   }
   c->graphs.clear();
 }
-
-bool is_pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
 
 int check_finite(const double *a, size_t n) {
   for (size_t k = 0; k < n; ++k)

@@ -52,8 +52,10 @@ inline __host__ __device__ int refine_strips(int ny) { return (ny - 2) / RS_COLS
 inline __host__ __device__ int refine_row_blocks(int nx) { return (nx - 2 + RS_ROWS - 1) / RS_ROWS; }
 
 // Updated iterate at (i, j0) and (i, j0 + 1): x = uin + (double) e on
-// interior nodes, uin on boundary nodes.  j0 is even and j0 <= ny - 1, so
-// both columns lie inside the row's pitch (pitch >= ny rounded up to 32).
+// interior nodes, uin on boundary nodes (UPDATE = false: x = uin).  j0 is
+// even and j0 <= ny - 1, so both columns lie inside the row's pitch (pitch
+// >= ny rounded up to 32).
+template <bool UPDATE>
 __device__ inline void refined_pair(const double *__restrict__ uin, const float *__restrict__ e, int g, int i, int j0,
                                     int nx, int ny, double &x0, double &x1, float &e0, float &e1) {
   const dvec2 u = ld_stream(reinterpret_cast<const dvec2 *>(uin + g));
@@ -61,7 +63,7 @@ __device__ inline void refined_pair(const double *__restrict__ uin, const float 
   x1 = u.y;
   e0 = 0.0f;
   e1 = 0.0f;
-  if (i >= 1 && i <= nx - 2) {
+  if (UPDATE && i >= 1 && i <= nx - 2) {
     const fvec2 ee = ld_stream(reinterpret_cast<const fvec2 *>(e + g));
     if (j0 >= 1 && j0 <= ny - 2) {
       e0 = ee.x;
@@ -74,16 +76,20 @@ __device__ inline void refined_pair(const double *__restrict__ uin, const float 
   }
 }
 
+template <bool UPDATE>
 __device__ inline double refined_one(const double *__restrict__ uin, const float *__restrict__ e, int i, int j, int nx,
                                      int ny, int pitch) {
   const int g = i * pitch + j;
   double x = uin[g];
-  if (i >= 1 && i <= nx - 2 && j >= 1 && j <= ny - 2) x = x + (double)e[g];
+  if (UPDATE && i >= 1 && i <= nx - 2 && j >= 1 && j <= ny - 2) x = x + (double)e[g];
   return x;
 }
 
 // grid: 8 * chunk workgroups of 256 threads (see tile_of_block); tiles =
-// strips * row_blocks; partials are written per tile.
+// strips * row_blocks; partials are written per tile.  UPDATE = false is the
+// residual of uin alone (e, uout and part_max unused; r32out may be null):
+// the initial residual of a solve, the same per-node formula and order.
+template <bool UPDATE>
 __global__ void __launch_bounds__(256) k_refine_stream(const double *__restrict__ uin, double *__restrict__ uout,
                                                        const float *__restrict__ e, const double *__restrict__ fn,
                                                        float *__restrict__ r32out, int nx, int ny, int pitch,
@@ -103,8 +109,8 @@ __global__ void __launch_bounds__(256) k_refine_stream(const double *__restrict_
   double a0 = 0.0, a1 = 0.0, b0 = 0.0, b1 = 0.0;
   float eb0 = 0.0f, eb1 = 0.0f, ed0, ed1;
   if (lane_in) {
-    refined_pair(uin, e, (i0 - 1) * pitch + j0, i0 - 1, j0, nx, ny, a0, a1, ed0, ed1);
-    refined_pair(uin, e, i0 * pitch + j0, i0, j0, nx, ny, b0, b1, eb0, eb1);
+    refined_pair<UPDATE>(uin, e, (i0 - 1) * pitch + j0, i0 - 1, j0, nx, ny, a0, a1, ed0, ed1);
+    refined_pair<UPDATE>(uin, e, i0 * pitch + j0, i0, j0, nx, ny, b0, b1, eb0, eb1);
   }
   // No early exit from the row loop: rows past the grid are predicated off
   // (wavefront-uniform), so the loads of later rows can be issued ahead.
@@ -117,26 +123,26 @@ __global__ void __launch_bounds__(256) k_refine_stream(const double *__restrict_
     float ec0 = 0.0f, ec1 = 0.0f;
     dvec2 f = {0.0, 0.0};
     if (lane_in && row_in) {
-      refined_pair(uin, e, g + pitch, i + 1, j0, nx, ny, c0, c1, ec0, ec1);
+      refined_pair<UPDATE>(uin, e, g + pitch, i + 1, j0, nx, ny, c0, c1, ec0, ec1);
       f = ld_stream(reinterpret_cast<const dvec2 *>(fn + g));
     }
     double l = __shfl(b1, (m + 63) & 63, 64);
     double rt = __shfl(b0, (m + 1) & 63, 64);
-    if (m == 0 && row_in && j0 >= 1) l = refined_one(uin, e, i, j0 - 1, nx, ny, pitch);
-    if (m == 63 && row_in && j0 + 2 <= ny - 1) rt = refined_one(uin, e, i, j0 + 2, nx, ny, pitch);
+    if (m == 0 && row_in && j0 >= 1) l = refined_one<UPDATE>(uin, e, i, j0 - 1, nx, ny, pitch);
+    if (m == 63 && row_in && j0 + 2 <= ny - 1) rt = refined_one<UPDATE>(uin, e, i, j0 + 2, nx, ny, pitch);
     if (row_in) {
       const double u20 = b0 * 2.0, u21 = b1 * 2.0;
       const double res0 = f.x - (divh(a0 + c0 - u20, hx2, ihx2, pw2) + divh(l + b1 - u20, hy2, ihy2, pw2));
       const double res1 = f.y - (divh(a1 + c1 - u21, hx2, ihx2, pw2) + divh(b0 + rt - u21, hy2, ihy2, pw2));
-      if (out0 && out1) {
-        st_stream(reinterpret_cast<dvec2 *>(uout + g), dvec2{b0, b1});
-        st_stream(reinterpret_cast<fvec2 *>(r32out + g), fvec2{(float)res0, (float)res1});
-      } else if (out0) {
-        st_stream(uout + g, b0);
-        st_stream(r32out + g, (float)res0);
-      } else if (out1) {
-        st_stream(uout + g + 1, b1);
-        st_stream(r32out + g + 1, (float)res1);
+      if (UPDATE) {
+        if (out0 && out1) st_stream(reinterpret_cast<dvec2 *>(uout + g), dvec2{b0, b1});
+        else if (out0) st_stream(uout + g, b0);
+        else if (out1) st_stream(uout + g + 1, b1);
+      }
+      if (UPDATE || r32out) {
+        if (out0 && out1) st_stream(reinterpret_cast<fvec2 *>(r32out + g), fvec2{(float)res0, (float)res1});
+        else if (out0) st_stream(r32out + g, (float)res0);
+        else if (out1) st_stream(r32out + g + 1, (float)res1);
       }
       if (out0) {
         ss += res0 * res0;
@@ -156,8 +162,10 @@ __global__ void __launch_bounds__(256) k_refine_stream(const double *__restrict_
   }
   ss = block_reduce<false>(ss, red, tid, 256);
   if (tid == 0) part_sum[t] = ss;
-  em = block_reduce<true>(em, red, tid, 256);
-  if (tid == 0) part_max[t] = em;
+  if (UPDATE) {
+    em = block_reduce<true>(em, red, tid, 256);
+    if (tid == 0) part_max[t] = em;
+  }
 }
 
 }  // namespace vlp2d

@@ -1,8 +1,8 @@
 // Register-streaming fused multigrid kernels for gfx950.
 //
-// One wavefront (one 64-thread workgroup) owns a tile of R output rows by
-// TJ = 128 - 2H output columns of one level (R is chosen per launch so that
-// the tiles fill the GPU's resident wavefront slots about once).  It stages a strip of 128
+// One wavefront owns a tile of R output rows by TJ = 128 - 2H output
+// columns of one level (R is chosen per launch so that the tiles fill the
+// GPU's resident wavefront slots about once).  It stages a strip of 128
 // columns (lane m holds the column pair cb + 2m, cb + 2m + 1) and walks it
 // top to bottom, one staged row per step, LR = R + 2H rows in all.  Every
 // row is loaded from HBM once per tile; all red-black Gauss-Seidel
@@ -11,7 +11,8 @@
 // half-sweep s to row q - s for s = 1..S, and runs the epilogue on the rows
 // the last half-sweep has finished.  Vertical neighbours are registers of the
 // same lane; horizontal neighbours of the other lane come through DPP
-// wavefront shifts (wave_shr:1 / wave_shl:1).  No LDS, no barriers.
+// wavefront shifts (wave_shr:1 / wave_shl:1).  No LDS; the only barrier
+// keeps the kWaves wavefronts of a workgroup (adjacent strips) in step.
 //
 // Red-black bookkeeping.  The tile origin (gi0, cb) has both coordinates
 // even, so node (gi0 + p, cb + 2m + e) has colour (p + e) & 1 (0 = red,
@@ -63,6 +64,19 @@ namespace vlp2d {
 
 constexpr int SCOLS = 128;  // staged columns per strip (64 lanes x 2)
 
+// Wavefronts per workgroup: kWaves tiles that are neighbours in tile order
+// (adjacent strips of one row band) run in one workgroup and step through
+// their rows together (one s_barrier per step, no memory fence), so the
+// cache lines their halos share are requested together (measured on MI355X
+// at 8193^2: 4 waves fetch 7% less than 1 on the level-0 ascent).
+#ifndef VLP2D_STREAM_WAVES
+#define VLP2D_STREAM_WAVES 4
+#endif
+constexpr int kWaves = VLP2D_STREAM_WAVES;
+__device__ __forceinline__ void group_step() {
+  if constexpr (kWaves > 1) __builtin_amdgcn_s_barrier();
+}
+
 // Steps per unrolled loop body: a multiple of the colour period (2), the
 // level rings (6) and the right-hand-side ring (S + 4 rows live).
 __host__ __device__ constexpr int s_period(int ns) { return 2 * ns + 4 <= 6 ? 6 : 12; }
@@ -86,7 +100,7 @@ inline STileGrid s_grid(int nx, int ny, int R, int H) {
   g.tiles_i = (nx - 2) / R + 1;  // output rows [ti R, ti R + R) cover 1..nx-2
   g.tiles_j = (ny - 2) / TJ + 1;
   g.tiles = g.tiles_i * g.tiles_j;
-  g.chunk = (g.tiles + 7) / 8;
+  g.chunk = ((g.tiles + kWaves - 1) / kWaves + 7) / 8;  // workgroups per XCD
   return g;
 }
 
@@ -306,7 +320,12 @@ template <int H> __device__ __forceinline__ bool s_tile(int R, int tiles_i, int 
                                                         STile &tl) {
   constexpr int TJ = SCOLS - 2 * H;
   const int LR = R + 2 * H;
-  tl.t = tile_of_block(tiles_i * tiles_j, chunk);
+  {
+    // workgroup -> XCD as tile_of_block, kWaves consecutive tiles per workgroup
+    const int b = blockIdx.x;
+    const int t = ((b & 7) * chunk + (b >> 3)) * kWaves + (int)(threadIdx.x >> 6);
+    tl.t = t < tiles_i * tiles_j ? t : -1;
+  }
   if (tl.t < 0) return false;
   int ti, tj;
   tile_coords(tl.t, tiles_i, tiles_j, ti, tj);
@@ -314,6 +333,12 @@ template <int H> __device__ __forceinline__ bool s_tile(int R, int tiles_i, int 
   tl.cb = tj * TJ - H;
   tl.interior = tl.gi0 >= 1 && tl.gi0 + LR - 1 <= nx - 2 && tl.cb >= 1 && tl.cb + SCOLS - 1 <= ny - 2;
   return true;
+}
+
+// A wavefront without a tile still takes part in its workgroup's LR steps.
+__device__ __forceinline__ void s_idle(int LR) {
+  if constexpr (kWaves > 1)
+    for (int q = 0; q < LR; ++q) group_step();
 }
 
 // ---------------------------------------------------------------------------
@@ -344,6 +369,7 @@ __device__ __forceinline__ void sdown_body(const LevelView<T> &a, const STile &t
     static_for<0, P>([&](auto jc) {
       constexpr int J = decltype(jc)::value;
       const int q = q0 + J;
+      group_step();
       s_prefetch<T, NS, ZI, EDGE, J>(st, a, L, gi0, q, LR);
       // level 0 of row q: red element (J & 1), black element
       constexpr int er = J & 1;
@@ -391,11 +417,14 @@ __device__ __forceinline__ void sdown_body(const LevelView<T> &a, const STile &t
 // STORE = false: the smoothed iterate is not written (zero-initialised
 // levels whose ascent recomputes it, k_sup with N1 = NS).
 template <typename T, int NS, bool ZI, bool EQ, bool STORE = true>
-__global__ void __launch_bounds__(64) k_sdown(LevelView<T> a, T *__restrict__ rhsc, int ncx, int ncy, int pc) {
+__global__ void __launch_bounds__(64 * kWaves) k_sdown(LevelView<T> a, T *__restrict__ rhsc, int ncx, int ncy, int pc) {
   static_assert(STORE || ZI, "only a zero-initialised descent can be recomputed");
   constexpr int H = s_down_halo(NS);
   STile tl;
-  if (!s_tile<H>(a.rows, a.tiles_i, a.tiles_j, a.chunk, a.nx, a.ny, tl)) return;
+  if (!s_tile<H>(a.rows, a.tiles_i, a.tiles_j, a.chunk, a.nx, a.ny, tl)) {
+    s_idle(a.rows + 2 * H);
+    return;
+  }
   if (a.pw2) {
     if (tl.interior) sdown_body<T, NS, ZI, STORE, EQ, true, false>(a, tl, rhsc, ncx, ncy, pc);
     else sdown_body<T, NS, ZI, STORE, EQ, true, true>(a, tl, rhsc, ncx, ncy, pc);
@@ -471,6 +500,7 @@ __device__ __forceinline__ void sup_body(const LevelView<T> &a, const STile &tl,
     static_for<0, P>([&](auto jc) {
       constexpr int J = decltype(jc)::value;
       const int q = q0 + J;
+      group_step();
       s_prefetch<T, N1 + NS, (N1 > 0), EDGE, J>(st, a, L, gi0, q, LR);
       // staged row q (level -1: red element J & 1, level 0: black element)
       constexpr int er = J & 1;
@@ -659,11 +689,14 @@ __device__ __forceinline__ void sup_body(const LevelView<T> &a, const STile &tl,
 // FP32 steps of a are (the host checks).
 // N1 > 0 pairs with k_sdown<T, N1, true, EQ, false> on the same level.
 template <typename T, int NS, int EPI, bool EQ, int N1 = 0>
-__global__ void __launch_bounds__(64) k_sup(LevelView<T> a, const T *__restrict__ uc, int ncx, int ncy, int pc,
+__global__ void __launch_bounds__(64 * kWaves) k_sup(LevelView<T> a, const T *__restrict__ uc, int ncx, int ncy, int pc,
                                             double *__restrict__ part, MixView mx) {
   constexpr int H = s_up_halo(N1 + NS, EPI);
   STile tl;
-  if (!s_tile<H>(a.rows, a.tiles_i, a.tiles_j, a.chunk, a.nx, a.ny, tl)) return;
+  if (!s_tile<H>(a.rows, a.tiles_i, a.tiles_j, a.chunk, a.nx, a.ny, tl)) {
+    s_idle(a.rows + 2 * H);
+    return;
+  }
   if (a.pw2) {
     if (tl.interior) sup_body<T, NS, N1, EPI, EQ, true, false>(a, tl, uc, ncx, ncy, pc, part, mx);
     else sup_body<T, NS, N1, EPI, EQ, true, true>(a, tl, uc, ncx, ncy, pc, part, mx);
