@@ -29,7 +29,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
 PMC_SUMMARY = os.path.join(ROOT, "profiles", "poisson8193_level0_pmc.md")
 # launch classes that run on the FP32 hierarchy in a mixed-precision solve
-FP32_CLASSES_IN_MIXED = {"down_fused", "up_fused", "coarse_vcycle", "rb_half", "residual", "restrict",
+FP32_CLASSES_IN_MIXED = {"down_fused", "up_fused", "up_lds", "coarse_vcycle", "rb_half", "residual", "restrict",
                          "prolong", "fill"}
 
 

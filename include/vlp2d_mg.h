@@ -55,6 +55,9 @@ extern "C" {
                                      smoothing sweep: the descent stores the
                                      iterate and the ascent reads it back
                                      (default: the ascent recomputes it)    */
+#define VLP2D_FLAG_STREAM_SMALL 64u /* levels below level 0 with <= 1025^2
+                                     nodes ascend with the register-streaming
+                                     kernel (default: the LDS-tiled one)    */
 
 typedef struct vlp2d_mg_ctx *vlp2d_mg_handle;
 

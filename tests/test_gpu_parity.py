@@ -73,6 +73,7 @@ KERNEL_PATHS = {
     "small_levels_in_l2": {"flags": _lib.FLAG_COARSE_L2},  # k_coarse instead of k_coarse_lds
     "small_levels_lds_65": {"coarse_nodes": 65 * 65},   # k_coarse_lds over > 64 KB of LDS in FP64
     "store_presmooth": {"flags": _lib.FLAG_STORE_PRESMOOTH},  # ascent reloads the pre-smoothed level
+    "stream_small": {"flags": _lib.FLAG_STREAM_SMALL},  # streaming ascent on levels <= 1025^2 (not k_lt_up)
     "one_op_per_launch": {"flags": _lib.FLAG_SIMPLE},
     "no_graph": {"flags": _lib.FLAG_NO_GRAPH},
     "separate_refine": {"flags": _lib.FLAG_SEPARATE_REFINE},
@@ -127,9 +128,10 @@ def test_kernel_paths_bitwise(path, seg_x, seg_y, hx, hy, laplace):
                                                        (512, 64, 1 / 512, 1 / 512, False)])
 def test_recomputed_presmoothing_bitwise(seg_x, seg_y, hx, hy, laplace, mixed, nu2):
     """V(1,nu2): the descent of a zero-initialised level does not store its
-    pre-smoothed iterate and the ascent recomputes it.  Same cycles and the
-    same iterate, bit for bit, as the oracle, and as the path that stores
-    and reloads the level.  V(1,1) is left out on the 128 x 256 grid with
+    pre-smoothed iterate and the ascent recomputes it (streaming levels), or
+    stores it for the LDS-tiled ascent (levels <= 1025^2).  Same cycles and
+    the same iterate, bit for bit, as the oracle, as the path that stores
+    and reloads every level, and as the streaming ascent on every level.  V(1,1) is left out on the 128 x 256 grid with
     steps e/128, pi/256 ((hx/hy)^2 = 3): point red-black smoothing diverges
     there (the oracle's residual grows by 1.76 per cycle)."""
     if nu2 == 1 and hx != hy:
@@ -139,7 +141,7 @@ def test_recomputed_presmoothing_bitwise(seg_x, seg_y, hx, hy, laplace, mixed, n
     z = np.zeros_like(g) if laplace else f
     solve = mixed_solve if mixed else plain_solve
     ur, kr, _ = solve(seg_x, seg_y, hx, hy, z, g, 1, nu2, 3, 1e-10, 60)
-    for flags in (0, _lib.FLAG_STORE_PRESMOOTH):
+    for flags in (0, _lib.FLAG_STORE_PRESMOOTH, _lib.FLAG_STREAM_SMALL):
         s = MultiGridSchemeHIP(seg_x, seg_y, hx, hy, z,
                                options=Options.performance(mixed, nu1=1, nu2=nu2, flags=flags))
         s.un0[:] = g

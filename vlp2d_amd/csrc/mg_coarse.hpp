@@ -10,7 +10,10 @@ namespace vlp2d {
 // workgroup barrier between dependent steps.  Level l0 is zero-initialised
 // unless it is level 0 of the problem (zero_first = 0).
 // ---------------------------------------------------------------------------
-constexpr int CNT = 1024;
+#ifndef VLP2D_COARSE_THREADS
+#define VLP2D_COARSE_THREADS 1024
+#endif
+constexpr int CNT = VLP2D_COARSE_THREADS;  // threads of the one workgroup
 constexpr int kMaxLevels = 16;
 
 template <typename T> struct CoarseLevel {

@@ -33,6 +33,7 @@
 #include "mg_coarse.hpp"
 #include "mg_refine.hpp"
 #include "mg_stream.hpp"
+#include "mg_ltile.hpp"
 #include "mg_setup.hpp"
 
 
@@ -58,11 +59,13 @@ enum KClass {
   KC_UP,
   KC_COARSE,
   KC_SETUP,
+  KC_UP_LDS,
   KC_COUNT
 };
 const char *kKClassName[KC_COUNT] = {"rb_half",   "residual",     "restrict", "fill",    "prolong",
                                      "copy_interior", "absdiff_max", "resnorm64", "reduce", "update_mixed",
-                                     "adjoint",   "down_fused",   "up_fused", "coarse_vcycle", "setup"};
+                                     "adjoint",   "down_fused",   "up_fused", "coarse_vcycle", "setup",
+                                     "up_lds"};
 
 // Epilogues of the level-0 ascent.
 enum Epi { EPI_NONE = 0, EPI_RESNORM = 1, EPI_METRIC = 2, EPI_MIXED = 3 };
@@ -361,6 +364,20 @@ bool recompute_presmooth(const Ctx *c, bool zi, int epi) {
   return zi && c->o.nu1 == 1 && epi != EPI_METRIC && !(c->o.flags & VLP2D_FLAG_STORE_PRESMOOTH);
 }
 
+// Levels below level 0 with at most kLtNodes nodes (1025^2 and smaller) run
+// the LDS-tiled ascent k_lt_up: there the streaming ascent is bound by the
+// latency of its row pipeline.  Their descent stores the pre-smoothed
+// iterate, which k_lt_up stages.
+#ifndef VLP2D_LT_NODES
+#define VLP2D_LT_NODES 1100000
+#endif
+constexpr long long kLtNodes = VLP2D_LT_NODES;
+template <typename T> bool lds_ascent(const Ctx *c, const Hier<T> &H, int l) {
+  const Lvl<T> &v = H.lv[l];
+  return l >= 1 && l < H.lc && (long long)v.nx * v.ny <= kLtNodes && c->o.nu2 >= 1 && c->o.nu2 <= 3 &&
+         !(c->o.flags & VLP2D_FLAG_STREAM_SMALL);
+}
+
 template <typename T, int NS>
 void launch_down_ns(Ctx *c, Lvl<T> &f, Lvl<T> &g, bool zi, bool store) {
   constexpr int H = s_down_halo(NS), P = s_period(NS);
@@ -431,6 +448,27 @@ template <typename T> int up(Ctx *c, Lvl<T> &f, Lvl<T> &g, int lvl, int epi, boo
   return np;
 }
 
+template <typename T, int NS> void launch_lt_up_ns(Ctx *c, Lvl<T> &f, const Lvl<T> &g) {
+  constexpr int H = lt_halo(NS);
+  const LtGrid lg = lt_grid(f.nx, f.ny, H);
+  const LevelView<T> a = view(f, f.u2, f.u, H);
+  using K = void (*)(LevelView<T>, const T *, int, int, int, int);
+  const K k = f.eq ? (K)k_lt_up<T, NS, true> : (K)k_lt_up<T, NS, false>;
+  hipLaunchKernelGGL(k, dim3(lg.tiles), dim3(64 * kLtWaves), 0, c->st, a, g.u, g.nx, g.ny, g.pitch, lg.tiles_j);
+}
+
+// LDS-tiled ascent (prolongation + nu2 sweeps) of a zero-initialised level
+// whose descent stored the pre-smoothed iterate in u2: u2 -> u.
+template <typename T> void up_lds(Ctx *c, Lvl<T> &f, Lvl<T> &g, int lvl) {
+  launch(c, KC_UP_LDS, lvl, sizeof(T) * (3.0 * nodes(f) + nodes(g)), [&] {
+    switch (c->o.nu2) {
+      case 1: launch_lt_up_ns<T, 1>(c, f, g); break;
+      case 2: launch_lt_up_ns<T, 2>(c, f, g); break;
+      default: launch_lt_up_ns<T, 3>(c, f, g); break;
+    }
+  });
+}
+
 template <typename T> void coarse(Ctx *c, Hier<T> &H, bool zero_first, double *snap) {
   CoarseArgs<T> a;
   std::memset(&a, 0, sizeof(a));
@@ -477,11 +515,16 @@ template <typename T> int vcycle_fused(Ctx *c, Hier<T> &H, bool zero0, int epi, 
   const int lc = H.lc;
   auto zi = [&](int l) { return l > 0 || zero0; };
   auto ep = [&](int l) { return l == 0 ? epi : (int)EPI_NONE; };
-  for (int l = 0; l < lc; ++l) down(c, H.lv[l], H.lv[l + 1], l, zi(l), !recompute_presmooth(c, zi(l), ep(l)));
+  auto rz = [&](int l) { return recompute_presmooth(c, zi(l), ep(l)) && !lds_ascent(c, H, l); };
+  for (int l = 0; l < lc; ++l) down(c, H.lv[l], H.lv[l + 1], l, zi(l), !rz(l));
   coarse(c, H, lc > 0 || zero0, lc == 0 ? snap : nullptr);
   int np = 0;
   for (int l = lc - 1; l >= 0; --l) {
-    const int n = up(c, H.lv[l], H.lv[l + 1], l, ep(l), recompute_presmooth(c, zi(l), ep(l)), mx);
+    if (lds_ascent(c, H, l)) {
+      up_lds(c, H.lv[l], H.lv[l + 1], l);
+      continue;
+    }
+    const int n = up(c, H.lv[l], H.lv[l + 1], l, ep(l), rz(l), mx);
     if (l == 0) np = n;
   }
   return np;

@@ -29,6 +29,7 @@ import sys
 KERNEL_CLASS = {
     "k_sdown": "down_fused",
     "k_sup": "up_fused",
+    "k_lt_up": "up_lds",
     "k_coarse": "coarse_vcycle",
     "k_refine_stream": "update_mixed",
     "k_refine": "update_mixed",
