@@ -7,7 +7,8 @@ boundary.  One step = one complete solve from the zero interior guess until
 iterate is restored on the device between steps, outside the timed region).
 
 Reported: the mixed-precision solver (FP32 V(1,3) cycles inside FP64
-iterative refinement; --cycle picks another V(nu1,nu2)) as ``value``; the
+iterative refinement, at most two FP32 cycles per refinement step; --cycle
+picks another V(nu1,nu2), --inner the cap) as ``value``; the
 plain FP64 multigrid with the same cycle on the same
 problem beside it (``fp64``); the dominant kernel's roofline from hipEvent
 timings of a profiled solve; and the oracle's CPU multigrid on a bounded
@@ -29,7 +30,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
 PMC_SUMMARY = os.path.join(ROOT, "profiles", "poisson8193_level0_pmc.md")
 # launch classes that run on the FP32 hierarchy in a mixed-precision solve
-FP32_CLASSES_IN_MIXED = {"down_fused", "up_fused", "up_lds", "coarse_vcycle", "rb_half", "residual", "restrict",
+FP32_CLASSES_IN_MIXED = {"down_fused", "up_fused", "up_fused_refine", "up_lds", "coarse_vcycle", "rb_half", "residual", "restrict",
                          "prolong", "fill"}
 
 
@@ -41,7 +42,7 @@ def solve_times(scheme, steps, warmup, rtol):
     for _ in range(warmup):
         scheme.restore_iterate()
         scheme.solve(rtol, 100)
-    wall, dev, cycles, rel = [], [], [], []
+    wall, dev, cycles, rel, vcyc = [], [], [], [], []
     for _ in range(steps):
         scheme.restore_iterate()  # device copy, before the clock starts
         t0 = time.perf_counter()
@@ -50,16 +51,19 @@ def solve_times(scheme, steps, warmup, rtol):
         dev.append(st["seconds"])
         cycles.append(st["cycles"])
         rel.append(st["rel_residual"])
+        vcyc.append(st["vcycles"])
+    solve_times.vcycles = vcyc
     return wall, dev, cycles, rel
 
 
-def profile_solve(p, mixed, rtol, nu):
+def profile_solve(p, mixed, rtol, nu, inner=1):
     """One solve with every launch bracketed by hipEvents; returns per
     (kernel, level) launches / ms / algorithmic bytes."""
     from vlp2d_amd import MultiGridSchemeHIP, Options
     from vlp2d_amd._lib import FLAG_PROFILE
     s = MultiGridSchemeHIP(p.seg_x, p.seg_y, p.hx, p.hy, p.f,
-                           options=Options.performance(mixed, nu1=nu[0], nu2=nu[1], flags=FLAG_PROFILE))
+                           options=Options.performance(mixed, nu1=nu[0], nu2=nu[1], flags=FLAG_PROFILE,
+                                                       max_inner=inner if mixed else 1))
     s.un0[:] = p.initial("zero")
     s.init_after_boundaries_and_initial_iteration_inited()
     s.save_iterate()
@@ -130,6 +134,8 @@ def main():
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--cycle", default="1,3",
                     help="nu1,nu2 of the V-cycle (V(1,3): fewest cycles to 1e-10 on this workload, measured)")
+    ap.add_argument("--inner", type=int, default=2,
+                    help="mixed: at most this many FP32 V-cycles per FP64 refinement step")
     args = ap.parse_args()
     nu = tuple(int(x) for x in args.cycle.split(","))
     if args.gpus != 1:
@@ -146,15 +152,16 @@ def main():
     res = {}
     for mixed in (True, False):
         s = MultiGridSchemeHIP(p.seg_x, p.seg_y, p.hx, p.hy, p.f,
-                               options=Options.performance(mixed, nu1=nu[0], nu2=nu[1]))
+                               options=Options.performance(mixed, nu1=nu[0], nu2=nu[1],
+                                                           max_inner=args.inner if mixed else 1))
         s.un0[:] = p.initial("zero")
         s.init_after_boundaries_and_initial_iteration_inited()
         s.save_iterate()
         wall, dev, cyc, rel = solve_times(s, args.steps, args.warmup, args.rtol)
         res[mixed] = {"wall": wall, "dev": dev, "cycles": cyc, "rel": rel, "info": s.info(),
-                      "u": s.get_array().copy()}
-        log(f"{'mixed' if mixed else 'fp64 '}: cycles {cyc}, wall ms {[round(w * 1e3, 3) for w in wall]}, "
-            f"rel {max(rel):.2e}")
+                      "vcycles": solve_times.vcycles[0], "u": s.get_array().copy()}
+        log(f"{'mixed' if mixed else 'fp64 '}: steps {cyc}, V-cycles {solve_times.vcycles[0]}, "
+            f"wall ms {[round(w * 1e3, 3) for w in wall]}, rel {max(rel):.2e}")
         s.cleanup()
 
     mixed_ms = 1e3 * float(np.mean(res[True]["wall"]))
@@ -164,7 +171,7 @@ def main():
     roofline = None
     kernels = None
     if not args.no_profile:
-        kt = profile_solve(p, True, args.rtol, nu)
+        kt = profile_solve(p, True, args.rtol, nu, args.inner)
         tot = sum(e[3] for e in kt)
         dom = max(kt, key=lambda e: e[3])
         name, lvl, launches, ms, nbytes = dom
@@ -179,7 +186,8 @@ def main():
                     "alg_bytes_per_launch": per_launch_bytes, "share_of_solve": round(ms / tot, 3)}
         kernels = sorted(([e[0], e[1], e[2], round(e[3], 4), e[4]] for e in kt), key=lambda e: -e[3])[:12]
     # whole-solve accounting with the per-operator byte model (unfused operators)
-    solve_bytes = {m: alg_bytes_per_operator(p, m, res[m]["cycles"][0], nu[0], nu[1]) for m in (True, False)}
+    solve_bytes = {m: alg_bytes_per_operator(p, m, res[m]["cycles"][0], res[m]["vcycles"], nu[0], nu[1])
+                   for m in (True, False)}
     solve_roofline = {
         "model": "per-operator algorithmic bytes (each operator one streaming pass; V(nu1,nu2): "
                  "(3(nu1+nu2)+6+3/4) s B/node/cycle, refinement 40 B/node/step, residual check 16 B/node)",
@@ -220,6 +228,8 @@ def main():
         "fp64": {"value": round(p.unknowns / (fp64_ms * 1e-3) / 1e6, 2), "ms_per_step": round(fp64_ms, 4),
                  "cycles": res[False]["cycles"][0]},
         "mixed_cycles": res[True]["cycles"][0],
+        "mixed_vcycles": res[True]["vcycles"],
+        "mixed_max_inner": args.inner,
         "speedup_mixed_vs_fp64": round(fp64_ms / mixed_ms, 3),
         "roofline": roofline,
         "solve_roofline": solve_roofline,
@@ -230,12 +240,13 @@ def main():
     print(json.dumps(line), flush=True)
 
 
-def alg_bytes_per_operator(p, mixed, cycles, nu1=2, nu2=2):
+def alg_bytes_per_operator(p, mixed, cycles, vcycles, nu1=2, nu2=2):
     """Bytes a solve moves if every operator is one streaming pass: per
-    non-coarsest level (3(nu1+nu2) + 6) s N_l + 3 s N_{l+1}, coarsest 3 s N_c;
-    mixed adds 40 B/node per refinement step (FP64 residual + cast, FP64
-    update) and 20 B/node for the first residual; FP64 adds a 16 B/node
-    residual check per cycle and one initial check."""
+    V-cycle and non-coarsest level (3(nu1+nu2) + 6) s N_l + 3 s N_{l+1},
+    coarsest 3 s N_c; mixed adds 40 B/node per refinement step (`cycles`
+    steps: FP64 residual + cast, FP64 update) and 20 B/node for the first
+    residual; FP64 adds a 16 B/node residual check per cycle and one
+    initial check."""
     s = 4 if mixed else 8
     nx, ny = p.seg_x, p.seg_y
     levels = []
@@ -251,8 +262,8 @@ def alg_bytes_per_operator(p, mixed, cycles, nu1=2, nu2=2):
     per_cycle += 3 * s * levels[-1]
     n0 = levels[0]
     if mixed:
-        return float(cycles * (per_cycle + 40 * n0) + 20 * n0)
-    return float(cycles * (per_cycle + 16 * n0) + 16 * n0)
+        return float(vcycles * per_cycle + cycles * 40 * n0 + 20 * n0)
+    return float(vcycles * per_cycle + cycles * 16 * n0 + 16 * n0)
 
 
 def measured_traffic(kernel_key):

@@ -69,6 +69,11 @@ typedef struct {
   unsigned flags;  /* VLP2D_FLAG_*                                          */
   int coarse_nodes;/* levels with nx*ny <= this run inside one workgroup;
                       0 = library default, -1 = never                       */
+  int max_inner;   /* mixed solves: at most this many FP32 V-cycles per
+                      FP64 refinement step, in [1,8] (default 1).  Step k
+                      runs the fewest cycles that the contraction per cycle
+                      measured on step k-1 predicts reach rtol, capped at
+                      max_inner (step 0 runs max_inner).                    */
 } vlp2d_mg_opts;
 
 typedef struct {
@@ -78,6 +83,7 @@ typedef struct {
   double rel_residual;    /* final ||r||_2 / r0                               */
   double seconds;         /* device time of the solve, hipEvents              */
   double munknowns_per_s; /* interior unknowns / seconds / 1e6                */
+  int vcycles;            /* V-cycles run (mixed: over all refinement steps) */
 } vlp2d_mg_stats;
 
 typedef struct {

@@ -564,12 +564,16 @@ int orc_plain_solve(void *p, int nu1, int nu2, int nuc, double rtol, int max_cyc
 
 // FP32 V-cycles inside FP64 iterative refinement.
 //   r = fn - Lap(u) (FP64, reference association);  rhs32 = (float) r
-//   e = one FP32 V(nu1,nu2) cycle on Lap(e) = rhs32 from e = 0 (zero boundary)
+//   e = m FP32 V(nu1,nu2) cycles on Lap(e) = rhs32 from e = 0 (zero boundary)
 //   u += (double) e on interior nodes
+// m: step 0 runs `inner` cycles; step k > 0 the fewest m <= inner with
+// rel_k rho^m <= rtol, rho = (rel_k / rel_{k-1})^(1/m_{k-1}) the contraction
+// per cycle of the previous step (inner when rho >= 1); inner = 1 is one
+// cycle per step.
 // Stops when ||r||_2 / ||r_0||_2 <= rtol. u (all nodes, FP64) is updated in
 // place. hist[k] = relative residual after k outer steps. Returns steps run.
 int orc_mixed_solve(int segX, int segY, double hx, double hy, const double *f, double *u, int nu1, int nu2,
-                    int nuc, double rtol, int max_outer, double *hist, int nthreads) {
+                    int nuc, double rtol, int max_outer, double *hist, int nthreads, int inner) {
   set_threads(nthreads);
   void *p64 = orc_create(0, segX, segY, hx, hy, f, nthreads);
   MG<double> &d = *(MG<double> *)((OrcHandle *)p64)->mg;
@@ -589,16 +593,31 @@ int orc_mixed_solve(int segX, int segY, double hx, double hy, const double *f, d
   };
   double r0 = resid();
   if (hist) hist[0] = 1.0;
-  int k = 0;
+  int k = 0, m_prev = 1;
+  double rel = 1.0, rel_prev = 1.0;
   if (r0 > 0.0) {
     while (k < max_outer) {
       Level<float> &z = s.lv[0];
       for (size_t q = 0; q < n; ++q) { z.rhs[q] = (float)r[q]; z.u[q] = 0.0f; }
-      s.vcycle(nu1, nu2, nuc);
+      // FP32 cycles of this step: step 0 runs max_inner; later steps the
+      // fewest that the last step's contraction per cycle predicts reach
+      // rtol, at most max_inner
+      int m = std::max(inner, 1);
+      if (k > 0 && m > 1) {
+        const double rho = std::pow(rel / rel_prev, 1.0 / m_prev);
+        if (rho < 1.0) {
+          int mm = 1;
+          for (double pr = rel * rho; mm < m && pr > rtol; pr *= rho) ++mm;
+          m = mm;
+        }
+      }
+      for (int c = 0; c < m; ++c) s.vcycle(nu1, nu2, nuc);
+      m_prev = m;
       for (int i = 1; i < nx - 1; ++i)
         for (int j = 1; j < ny - 1; ++j) d.lv[0].u[(size_t)i * ny + j] += (double)z.u[(size_t)i * ny + j];
       ++k;
-      double rel = resid() / r0;
+      rel_prev = rel;
+      rel = resid() / r0;
       if (hist) hist[k] = rel;
       if (rel <= rtol) break;
     }

@@ -48,7 +48,7 @@ def load():
     lib.orc_mixed_solve.restype = ctypes.c_int
     lib.orc_mixed_solve.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp, _dp,
                                     ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, _dp,
-                                    ctypes.c_int]
+                                    ctypes.c_int, ctypes.c_int]
     lib.orc_forward_cycles.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp, _dp,
                                        ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
     lib.orc_adjoint_cycles.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
@@ -127,13 +127,13 @@ def plain_solve(seg_x, seg_y, hx, hy, f, u, nu1=2, nu2=2, nuc=3, rtol=1e-10, max
     return out, k, hist
 
 
-def mixed_solve(seg_x, seg_y, hx, hy, f, u, nu1=2, nu2=2, nuc=3, rtol=1e-10, max_outer=100, nthreads=0):
+def mixed_solve(seg_x, seg_y, hx, hy, f, u, nu1=2, nu2=2, nuc=3, rtol=1e-10, max_outer=100, nthreads=0, inner=1):
     lib = load()
     u = np.array(u, dtype=np.float64, order="C").reshape(seg_x + 1, seg_y + 1)
     fa = None if f is None else np.ascontiguousarray(f, dtype=np.float64)
     hist = np.zeros(max_outer + 1)
     k = lib.orc_mixed_solve(seg_x, seg_y, hx, hy, _ptr(fa), _ptr(u), nu1, nu2, nuc, rtol, max_outer, _ptr(hist),
-                            nthreads)
+                            nthreads, inner)
     return u, k, hist[: k + 1]
 
 

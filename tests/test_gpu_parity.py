@@ -88,7 +88,8 @@ def test_kernel_paths_bitwise(path, seg_x, seg_y, hx, hy, laplace):
     """Every kernel path (streaming tiles, one-workgroup small levels in LDS
     or in L2, one operator per launch, graph or direct launches) gives the
     oracle's iterate bit for bit, in parity mode (V(3,3) + update norm) and
-    in the mixed V(2,2) solve."""
+    in the mixed V(2,2) solve with one or (at most) two FP32 cycles per
+    refinement step."""
     _say(f"paths {path} {seg_x}x{seg_y}")
     f, g = _random(seg_x, seg_y, 17 + seg_x, laplace)
     u0 = _mean_init(g)
@@ -104,21 +105,23 @@ def test_kernel_paths_bitwise(path, seg_x, seg_y, hx, hy, laplace):
         assert s.last_metric == m_ref
     s.cleanup()
     z = np.zeros_like(g) if laplace else f
-    ur, kr, _ = mixed_solve(seg_x, seg_y, hx, hy, z, g, 2, 2, 3, 1e-10, 40)
-    s = MultiGridSchemeHIP(seg_x, seg_y, hx, hy, z, options=Options.performance(True, **KERNEL_PATHS[path]))
-    s.un0[:] = g
-    s.init_after_boundaries_and_initial_iteration_inited()
-    st = s.solve(1e-10, 40)
-    assert st["cycles"] == kr
-    assert np.array_equal(s.get_array(), ur), np.abs(s.get_array() - ur).max()
-    # a second solve from the restored state replays the captured graphs
-    s.save_iterate()
-    s.restore_iterate()
-    s.un0[:] = g
-    s.init_after_boundaries_and_initial_iteration_inited()
-    st = s.solve(1e-10, 40)
-    assert st["cycles"] == kr and np.array_equal(s.get_array(), ur)
-    s.cleanup()
+    for inner in (1, 2):
+        ur, kr, _ = mixed_solve(seg_x, seg_y, hx, hy, z, g, 2, 2, 3, 1e-10, 40, inner=inner)
+        s = MultiGridSchemeHIP(seg_x, seg_y, hx, hy, z,
+                               options=Options.performance(True, max_inner=inner, **KERNEL_PATHS[path]))
+        s.un0[:] = g
+        s.init_after_boundaries_and_initial_iteration_inited()
+        st = s.solve(1e-10, 40)
+        assert st["cycles"] == kr, (inner, st["cycles"], kr)
+        assert np.array_equal(s.get_array(), ur), (inner, np.abs(s.get_array() - ur).max())
+        # a second solve from the restored state replays the captured graphs
+        s.save_iterate()
+        s.restore_iterate()
+        s.un0[:] = g
+        s.init_after_boundaries_and_initial_iteration_inited()
+        st = s.solve(1e-10, 40)
+        assert st["cycles"] == kr and np.array_equal(s.get_array(), ur)
+        s.cleanup()
 
 
 @pytest.mark.parametrize("nu2", [1, 2, 3])
@@ -175,22 +178,23 @@ def test_config1_reference_stop_rule_cycle_count():
 
 
 @pytest.mark.parametrize("nu", [(2, 2), (1, 3)])
-@pytest.mark.parametrize("mixed", [False, True])
+@pytest.mark.parametrize("mixed,inner", [(False, 1), (True, 1), (True, 2), (True, 3)])
 @pytest.mark.parametrize("cfg,seg", [(2, 512), (3, 256), (1, 128), (3, 1024)])
-def test_performance_mode_matches_oracle(cfg, seg, mixed, nu):
+def test_performance_mode_matches_oracle(cfg, seg, mixed, inner, nu):
     """V(nu1,nu2) to relative residual 1e-10 from a zero interior (V(2,2), and
-    the benchmark's V(1,3)), FP64 or FP32 cycles in FP64 refinement: same
-    number of cycles as the oracle and the same iterate (bitwise), which is
-    the DST-I solution to solver accuracy."""
+    the benchmark's V(1,3)), FP64 or FP32 cycles in FP64 refinement (at most
+    `inner` FP32 cycles per refinement step, chosen by the predicted
+    contraction): same number of steps as the oracle and the same iterate
+    (bitwise), which is the DST-I solution to solver accuracy."""
     p = problems.config(cfg, seg)
-    _say(f"perf cfg{cfg} seg{seg} mixed={mixed} nu={nu}")
+    _say(f"perf cfg{cfg} seg{seg} mixed={mixed} inner={inner} nu={nu}")
     u0 = p.initial("zero")
     if mixed:
-        ur, kr, hr = mixed_solve(p.seg_x, p.seg_y, p.hx, p.hy, p.f, u0, nu[0], nu[1], 3, 1e-10, 60)
+        ur, kr, hr = mixed_solve(p.seg_x, p.seg_y, p.hx, p.hy, p.f, u0, nu[0], nu[1], 3, 1e-10, 60, inner=inner)
     else:
         ur, kr, hr = plain_solve(p.seg_x, p.seg_y, p.hx, p.hy, p.f, u0, nu[0], nu[1], 3, 1e-10, 60)
     s = MultiGridSchemeHIP(p.seg_x, p.seg_y, p.hx, p.hy, p.f,
-                           options=Options.performance(mixed, nu1=nu[0], nu2=nu[1]))
+                           options=Options.performance(mixed, nu1=nu[0], nu2=nu[1], max_inner=inner))
     s.un0[:] = u0
     s.init_after_boundaries_and_initial_iteration_inited()
     st = s.solve(1e-10, 60)

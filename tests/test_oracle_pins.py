@@ -212,6 +212,36 @@ def test_mixed_precision_reaches_fp64_accuracy():
     assert np.abs(u - ex).max() <= 1e-8 * np.abs(ex).max()
 
 
+@pytest.mark.parametrize("nu,steps,schedule", [((1, 3), 3, [2, 2, 2]), ((2, 2), 4, [2, 2, 2, 1])])
+def test_mixed_inner_cycles_schedule(nu, steps, schedule):
+    """At most two FP32 cycles per refinement step.  Independent check of the
+    schedule: the per-step cycle counts are recomputed here from the
+    residual history (step 0 two cycles, then the fewest whose predicted
+    rel * rho^m reaches rtol), and after step k the residual equals that of
+    the one-cycle-per-step solve after the same number of cycles (an FP32
+    cycle on the correction contracts like one FP64 refinement step while
+    the correction's rounding is far below the residual); the fixed
+    two-cycle count, which runs 8 cycles for V(2,2), would miss the 7-cycle
+    residual."""
+    p = problems.config(3, 256)
+    _, k1, h1 = mixed_solve(p.seg_x, p.seg_y, p.hx, p.hy, p.f, p.initial("zero"), nu[0], nu[1], 3, 1e-10, 60,
+                            inner=1)
+    u2, k2, h2 = mixed_solve(p.seg_x, p.seg_y, p.hx, p.hy, p.f, p.initial("zero"), nu[0], nu[1], 3, 1e-10, 60,
+                             inner=2)
+    assert k2 == steps and h2[-1] <= 1e-10
+    ms = [2]
+    for k in range(1, k2):
+        rho = (h2[k] / h2[k - 1]) ** (1.0 / ms[-1])
+        ms.append(1 if h2[k] * rho <= 1e-10 else 2)
+    assert ms == schedule
+    cum = np.cumsum([0] + ms)
+    assert cum[-1] == k1
+    for k in range(1, k2 + 1):
+        assert abs(h2[k] / h1[cum[k]] - 1) < 0.01, (k, h2[k], h1[cum[k]])
+    ex = dst_solve(p.f, p.g, p.hx, p.hy)
+    assert np.abs(u2 - ex).max() <= 1e-8 * np.abs(ex).max()
+
+
 def test_float_and_dd_oracles_converge():
     p = problems.laplace1(64)
     for prec, tol in (("float", 2e-4), ("dd", 1e-25)):

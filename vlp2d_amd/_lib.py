@@ -41,13 +41,14 @@ class VLP2DError(RuntimeError):
 
 class Opts(ctypes.Structure):
     _fields_ = [("nu1", ctypes.c_int), ("nu2", ctypes.c_int), ("nu_coarse", ctypes.c_int),
-                ("precision", ctypes.c_int), ("flags", ctypes.c_uint), ("coarse_nodes", ctypes.c_int)]
+                ("precision", ctypes.c_int), ("flags", ctypes.c_uint), ("coarse_nodes", ctypes.c_int),
+                ("max_inner", ctypes.c_int)]
 
 
 class Stats(ctypes.Structure):
     _fields_ = [("cycles", ctypes.c_int), ("levels", ctypes.c_int), ("r0", ctypes.c_double),
                 ("rel_residual", ctypes.c_double), ("seconds", ctypes.c_double),
-                ("munknowns_per_s", ctypes.c_double)]
+                ("munknowns_per_s", ctypes.c_double), ("vcycles", ctypes.c_int)]
 
 
 class Info(ctypes.Structure):

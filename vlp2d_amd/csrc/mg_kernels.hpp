@@ -64,6 +64,25 @@ __global__ void __launch_bounds__(RED_THREADS) k_reduce(const double *__restrict
   if (threadIdx.x == 0) out[0] = v;
 }
 
+// The two result scalars of a cycle, published to mapped, coherent pinned
+// host memory: values first, then (after a system-scope fence) the sequence
+// number the host polls for.  The host thus learns a cycle's result without
+// a device-to-host copy and a stream synchronisation.
+struct HostPub {
+  double v[2];
+  unsigned long long seq;
+};
+__global__ void k_publish(const double *__restrict__ dscal, unsigned long long *__restrict__ dseq, HostPub *pub) {
+  if (threadIdx.x == 0) {
+    const unsigned long long s = dseq[0] + 1;
+    dseq[0] = s;
+    pub->v[0] = dscal[0];
+    pub->v[1] = dscal[1];
+    __threadfence_system();
+    __hip_atomic_store(&pub->seq, s, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Red-black Gauss-Seidel half-sweep (one colour).  Red = (i+j) even, updated
 // first (VisualLaplacePoisson2D/Common/GridIterator.cs:184-188).  Thread

@@ -60,12 +60,13 @@ enum KClass {
   KC_COARSE,
   KC_SETUP,
   KC_UP_LDS,
+  KC_UP_REFINE,
   KC_COUNT
 };
 const char *kKClassName[KC_COUNT] = {"rb_half",   "residual",     "restrict", "fill",    "prolong",
                                      "copy_interior", "absdiff_max", "resnorm64", "reduce", "update_mixed",
                                      "adjoint",   "down_fused",   "up_fused", "coarse_vcycle", "setup",
-                                     "up_lds"};
+                                     "up_lds",    "up_fused_refine"};
 
 // Epilogues of the level-0 ascent.
 enum Epi { EPI_NONE = 0, EPI_RESNORM = 1, EPI_METRIC = 2, EPI_MIXED = 3 };
@@ -117,6 +118,10 @@ struct vlp2d_mg_ctx {
   int part_cap = 0;
   double *dscal = nullptr;  // device scalars
   double *hscal = nullptr;  // pinned host scalars
+  HostPub *hpub = nullptr;  // mapped pinned results the solve loop polls
+  HostPub *hpub_dev = nullptr;
+  unsigned long long *dseq = nullptr;  // device count of published results
+  unsigned long long pub_seq = 0;      // results published so far
   bool r32_valid = false;   // h32 level-0 rhs holds (float)(fn - Lap u64)
   // transposed-cycle buffers
   std::vector<double *> aub, arb, aresb;
@@ -361,7 +366,10 @@ LevelView<T> sview(Ctx *c, const void *fn, const Lvl<T> &v, const T *uin, T *uou
 // descent; the ascent recomputes it from the right-hand side (k_sup N1 = 1).
 // Not with the update metric, which needs the iterate before post-smoothing.
 bool recompute_presmooth(const Ctx *c, bool zi, int epi) {
-  return zi && c->o.nu1 == 1 && epi != EPI_METRIC && !(c->o.flags & VLP2D_FLAG_STORE_PRESMOOTH);
+  // a non-zero start is recomputed only by the refinement ascent, which
+  // writes no iterate and so may stage the descent's input in place
+  return (zi || epi == EPI_MIXED) && c->o.nu1 == 1 && epi != EPI_METRIC &&
+         !(c->o.flags & VLP2D_FLAG_STORE_PRESMOOTH);
 }
 
 // Levels below level 0 with at most kLtNodes nodes (1025^2 and smaller) run
@@ -383,7 +391,8 @@ void launch_down_ns(Ctx *c, Lvl<T> &f, Lvl<T> &g, bool zi, bool store) {
   constexpr int H = s_down_halo(NS), P = s_period(NS);
   using K = void (*)(LevelView<T>, T *, int, int, int);
   K k;
-  if (!zi) k = f.eq ? (K)k_sdown<T, NS, false, true> : (K)k_sdown<T, NS, false, false>;
+  if (!zi && store) k = f.eq ? (K)k_sdown<T, NS, false, true> : (K)k_sdown<T, NS, false, false>;
+  else if (!zi) k = f.eq ? (K)k_sdown<T, NS, false, true, false> : (K)k_sdown<T, NS, false, false, false>;
   else if (store) k = f.eq ? (K)k_sdown<T, NS, true, true> : (K)k_sdown<T, NS, true, false>;
   else k = f.eq ? (K)k_sdown<T, NS, true, true, false> : (K)k_sdown<T, NS, true, false, false>;
   const LevelView<T> a = sview(c, (const void *)k, f, f.u, f.u2, H, P);
@@ -402,12 +411,13 @@ template <typename T> void down(Ctx *c, Lvl<T> &f, Lvl<T> &g, int lvl, bool zi, 
   });
 }
 
-template <typename T, int NS, int EPI, int N1>
+template <typename T, int NS, int EPI, int N1, bool ZS = true>
 int launch_sup(Ctx *c, Lvl<T> &f, const Lvl<T> &g, const MixView &mx) {
   constexpr int H = s_up_halo(N1 + NS, EPI), P = s_period(N1 + NS);
   using K = void (*)(LevelView<T>, const T *, int, int, int, double *, MixView);
-  const K k = f.eq ? (K)k_sup<T, NS, EPI, true, N1> : (K)k_sup<T, NS, EPI, false, N1>;
-  const LevelView<T> a = sview(c, (const void *)k, f, f.u2, f.u, H, P);
+  const K k = f.eq ? (K)k_sup<T, NS, EPI, true, N1, ZS> : (K)k_sup<T, NS, EPI, false, N1, ZS>;
+  // a recomputed non-zero start is staged from the descent's input f.u
+  const LevelView<T> a = sview(c, (const void *)k, f, ZS ? f.u2 : f.u, f.u, H, P);
   hipLaunchKernelGGL(k, dim3(8 * a.chunk), dim3(64 * kWaves), 0, c->st, a, g.u, g.nx, g.ny, g.pitch, c->part, mx);
   return a.tiles_i * a.tiles_j;
 }
@@ -416,7 +426,7 @@ int launch_sup(Ctx *c, Lvl<T> &f, const Lvl<T> &g, const MixView &mx) {
 // (the correction cycle of a mixed step) none / fused refinement.  rz: the
 // level's pre-smoothed iterate is recomputed (one pre-smoothing sweep).
 template <typename T, int NS>
-int launch_up_ns(Ctx *c, Lvl<T> &f, const Lvl<T> &g, int epi, bool rz, const MixView &mx) {
+int launch_up_ns(Ctx *c, Lvl<T> &f, const Lvl<T> &g, int epi, bool rz, bool zi, const MixView &mx) {
   if constexpr (sizeof(T) == 8) {
     switch (epi) {
       case EPI_RESNORM:
@@ -425,24 +435,27 @@ int launch_up_ns(Ctx *c, Lvl<T> &f, const Lvl<T> &g, int epi, bool rz, const Mix
       default: return rz ? launch_sup<T, NS, EPI_NONE, 1>(c, f, g, mx) : launch_sup<T, NS, EPI_NONE, 0>(c, f, g, mx);
     }
   } else {
-    if (epi == EPI_MIXED)
+    if (epi == EPI_MIXED) {
+      if (rz && !zi) return launch_sup<T, NS, EPI_MIXED, 1, false>(c, f, g, mx);
       return rz ? launch_sup<T, NS, EPI_MIXED, 1>(c, f, g, mx) : launch_sup<T, NS, EPI_MIXED, 0>(c, f, g, mx);
+    }
     return rz ? launch_sup<T, NS, EPI_NONE, 1>(c, f, g, mx) : launch_sup<T, NS, EPI_NONE, 0>(c, f, g, mx);
   }
 }
 
 // returns the number of partials the epilogue writes
-template <typename T> int up(Ctx *c, Lvl<T> &f, Lvl<T> &g, int lvl, int epi, bool rz, const MixView &mx) {
+template <typename T>
+int up(Ctx *c, Lvl<T> &f, Lvl<T> &g, int lvl, int epi, bool rz, bool zi, const MixView &mx) {
   const double s = sizeof(T);
-  const double rin = rz ? 0.0 : 1.0;  // the staged iterate, unless recomputed
+  const double rin = rz && zi ? 0.0 : 1.0;  // the staged iterate, unless recomputed from zero
   double bytes = s * ((rin + 2.0) * nodes(f) + nodes(g));
   if (epi == EPI_MIXED) bytes = s * ((rin + 1.0) * nodes(f) + nodes(g)) + 28.0 * nodes(f);
   int np = 0;
-  launch(c, KC_UP, lvl, bytes, [&] {
+  launch(c, epi == EPI_MIXED ? KC_UP_REFINE : KC_UP, lvl, bytes, [&] {
     switch (c->o.nu2) {
-      case 1: np = launch_up_ns<T, 1>(c, f, g, epi, rz, mx); break;
-      case 2: np = launch_up_ns<T, 2>(c, f, g, epi, rz, mx); break;
-      default: np = launch_up_ns<T, 3>(c, f, g, epi, rz, mx); break;
+      case 1: np = launch_up_ns<T, 1>(c, f, g, epi, rz, zi, mx); break;
+      case 2: np = launch_up_ns<T, 2>(c, f, g, epi, rz, zi, mx); break;
+      default: np = launch_up_ns<T, 3>(c, f, g, epi, rz, zi, mx); break;
     }
   });
   return np;
@@ -524,7 +537,7 @@ template <typename T> int vcycle_fused(Ctx *c, Hier<T> &H, bool zero0, int epi, 
       up_lds(c, H.lv[l], H.lv[l + 1], l);
       continue;
     }
-    const int n = up(c, H.lv[l], H.lv[l + 1], l, ep(l), rz(l), mx);
+    const int n = up(c, H.lv[l], H.lv[l + 1], l, ep(l), rz(l), zi(l), mx);
     if (l == 0) np = n;
   }
   return np;
@@ -584,13 +597,21 @@ void cycle64(Ctx *c, Hier<double> &H, int epi) {
 // e = V-cycle(rhs32) from e = 0; u_next = u + e (max|e| -> dscal[0]);
 // r32_next = fn - Lap u_next, ||r||^2 -> dscal[1].  u/u2 of P and rhs/rhs2 of
 // h32 level 0 are swapped by the caller afterwards.
-void mixed_step(Ctx *c, Lvl<double> &P) {
+// m >= 2: the correction equation gets m FP32 V-cycles, the first from
+// e = 0, the refinement update follows the last.
+void mixed_step(Ctx *c, Lvl<double> &P, int m = 1) {
   Hier<float> &Z = c->h32;
   Lvl<float> &z = Z.lv[0];
+  MixView none{};
+  // all cycles but the last (correction e -> z.u)
+  for (int k = 0; k + 1 < m; ++k) {
+    if (c->simple) vcycle_simple(c, Z, k == 0, nullptr);
+    else vcycle_fused(c, Z, k == 0, EPI_NONE, none, nullptr);
+  }
+  const bool zero_last = m == 1;
   if (c->simple || Z.lc == 0) {
-    MixView none{};
-    if (c->simple) vcycle_simple(c, Z, true, nullptr);
-    else vcycle_fused(c, Z, true, EPI_NONE, none, nullptr);
+    if (c->simple) vcycle_simple(c, Z, zero_last, nullptr);
+    else vcycle_fused(c, Z, zero_last, EPI_NONE, none, nullptr);
     dim3 g = grid_red(P.ny, P.nx);
     int np = (int)(g.x * g.y);
     launch(c, KC_UPDATE, 0, 20.0 * nodes(P), [&] {
@@ -615,12 +636,12 @@ void mixed_step(Ctx *c, Lvl<double> &P) {
     // refinement update inside the level-0 ascent of the FP32 cycle (the
     // kernel evaluates both precisions' divisions the same way, as
     // multiplications when the steps are powers of two)
-    const int np = vcycle_fused(c, Z, true, EPI_MIXED, mx, nullptr);
+    const int np = vcycle_fused(c, Z, zero_last, EPI_MIXED, mx, nullptr);
     reduce_part(c, c->part2, np, 0, true);
     reduce_part(c, c->part, np, 1, false);
     return;
   }
-  vcycle_fused(c, Z, true, EPI_NONE, mx, nullptr);  // correction e -> z.u
+  vcycle_fused(c, Z, zero_last, EPI_NONE, mx, nullptr);  // correction e -> z.u
   const int rows = refine_row_blocks(P.nx);
   const int np = refine_strips(P.ny) * rows;
   const int chunk = (np + 7) / 8;
@@ -666,12 +687,38 @@ template <typename F> int run_program(Ctx *c, int key, F &&body) {
 
 // graph keys: program id * 4 + swap parities of the buffers it captured
 enum Prog { PG_CYCLE_METRIC = 0, PG_CYCLE_NORM = 1, PG_MIXED = 2, PG_ADJ_CYCLE_NORM = 3, PG_ADJ_MIXED = 4 };
-int gkey(int prog, int par, int rpar) { return prog * 4 + par * 2 + rpar; }
+int gkey(int prog, int par, int rpar, int m = 1) { return ((prog * 16 + m) * 2 + par) * 2 + rpar; }
 
 int sync_read(Ctx *c, int nslots) {
   HIPCHK(c, hipMemcpyAsync(c->hscal, c->dscal, sizeof(double) * nslots, hipMemcpyDeviceToHost, c->st));
   HIPCHK(c, hipStreamSynchronize(c->st));
   if (c->profile) collect_profile(c);
+  return VLP2D_OK;
+}
+
+// dscal[0..1] -> hscal[0..1] through k_publish: the host spins on the
+// sequence number instead of synchronising the stream (used between the
+// cycles of a solve; profiling runs keep sync_read).
+int publish_wait(Ctx *c) {
+  if (c->profile) return sync_read(c, 2);
+  hipLaunchKernelGGL(k_publish, dim3(1), dim3(64), 0, c->st, c->dscal, c->dseq, c->hpub_dev);
+  HIPCHK(c, hipGetLastError());
+  const unsigned long long want = ++c->pub_seq;
+  for (unsigned long long spin = 1;; ++spin) {
+    if (__atomic_load_n(&c->hpub->seq, __ATOMIC_ACQUIRE) >= want) break;
+    if ((spin & 4095) == 0) {
+      // the stream drained without the result: report, do not spin forever
+      const hipError_t q = hipStreamQuery(c->st);
+      if (q == hipSuccess) {
+        if (__atomic_load_n(&c->hpub->seq, __ATOMIC_ACQUIRE) >= want) break;
+        return fail(c, VLP2D_E_DEVICE, "published cycle result did not arrive");
+      }
+      if (q != hipErrorNotReady) HIPCHK(c, q);
+    }
+    __builtin_ia32_pause();
+  }
+  c->hscal[0] = c->hpub->v[0];
+  c->hscal[1] = c->hpub->v[1];
   return VLP2D_OK;
 }
 
@@ -824,6 +871,18 @@ int mirror_level0(Ctx *c, Lvl<double> &P) {
   return VLP2D_OK;
 }
 
+// FP32 V-cycles of the next refinement step: the fewest that the contraction
+// per cycle of the last step, rho = (rel / rel_prev)^(1/m_prev), predicts
+// bring rel to rtol, at most max_inner (all max_inner when rho >= 1).
+int inner_cycles(double rel, double rel_prev, int m_prev, int max_inner, double rtol) {
+  if (max_inner <= 1) return 1;
+  const double rho = std::pow(rel / rel_prev, 1.0 / m_prev);
+  if (!(rho < 1.0)) return max_inner;
+  int m = 1;
+  for (double pr = rel * rho; m < max_inner && pr > rtol; pr *= rho) ++m;
+  return m;
+}
+
 // Solve from the current level-0 iterate of H until ||r|| <= rtol ||r0||.
 int solve_impl(Ctx *c, Hier<double> &H, int prog_fp64, int prog_mixed, double rtol, int max_cycles,
                vlp2d_mg_stats *st) {
@@ -837,12 +896,12 @@ int solve_impl(Ctx *c, Hier<double> &H, int prog_fp64, int prog_mixed, double rt
   HIPCHK(c, hipEventRecord(e0, c->st));
   resnorm(c, P, P.u, 1, mixed ? c->h32.lv[0].rhs : nullptr);
   HIPCHK(c, hipGetLastError());
-  RCHK(sync_read(c, 2));
+  RCHK(publish_wait(c));
   const double r0 = std::sqrt(c->hscal[1]);
   c->r32_valid = mixed && (&H == &c->h64);
   if (!std::isfinite(r0)) return fail(c, VLP2D_E_NONFINITE, "initial residual is not finite");
-  double rel = r0 > 0 ? 1.0 : 0.0;
-  int k = 0;
+  double rel = r0 > 0 ? 1.0 : 0.0, rel_prev = 1.0;
+  int k = 0, vcycles = 0, m_prev = 1;
   int rc = VLP2D_OK;
   while (r0 > 0 && rel > rtol) {
     if (k >= max_cycles) {
@@ -854,13 +913,18 @@ int solve_impl(Ctx *c, Hier<double> &H, int prog_fp64, int prog_mixed, double rt
       break;
     }
     if (mixed) {
-      RCHK(run_program(c, gkey(prog_mixed, P.par, c->h32.lv[0].rpar), [&] { mixed_step(c, P); }));
+      const int m = k == 0 ? c->o.max_inner : inner_cycles(rel, rel_prev, m_prev, c->o.max_inner, rtol);
+      RCHK(run_program(c, gkey(prog_mixed, P.par, c->h32.lv[0].rpar, m), [&] { mixed_step(c, P, m); }));
       swap_mixed(c, P);
+      vcycles += m;
+      m_prev = m;
     } else {
       RCHK(run_program(c, gkey(prog_fp64, P.par, 0), [&] { cycle64(c, H, EPI_RESNORM); }));
+      ++vcycles;
     }
-    RCHK(sync_read(c, 2));
+    RCHK(publish_wait(c));
     ++k;
+    rel_prev = rel;
     rel = std::sqrt(c->hscal[1]) / r0;
     if (!std::isfinite(rel)) {
       rc = VLP2D_E_NONFINITE;
@@ -876,6 +940,7 @@ int solve_impl(Ctx *c, Hier<double> &H, int prog_fp64, int prog_mixed, double rt
   (void)hipEventDestroy(e1);
   if (st) {
     st->cycles = k;
+    st->vcycles = vcycles;
     st->levels = c->L;
     st->r0 = r0;
     st->rel_residual = rel;
@@ -1029,9 +1094,10 @@ void vlp2d_mg_default_opts(vlp2d_mg_opts *o) {
   o->precision = VLP2D_PREC_FP64;
   o->flags = 0;
   o->coarse_nodes = 0;
+  o->max_inner = 1;
 }
 
-int vlp2d_mg_version(void) { return (1 << 16) | 1; }
+int vlp2d_mg_version(void) { return (1 << 16) | 2; }
 
 static int create_impl(int segX, int segY, double hx, double hy, const double *f, const Expr *fe, double eps,
                        int device, const vlp2d_mg_opts *opts, vlp2d_mg_handle *out) {
@@ -1052,8 +1118,10 @@ static int create_impl(int segX, int segY, double hx, double hy, const double *f
   vlp2d_mg_default_opts(&o);
   if (opts) o = *opts;
   if (o.nu1 < 0 || o.nu2 < 0 || o.nu_coarse < 1 || o.nu1 > 32 || o.nu2 > 32 || o.nu_coarse > 1024 ||
-      (o.precision != VLP2D_PREC_FP64 && o.precision != VLP2D_PREC_MIXED) || o.coarse_nodes < -1) {
-    g_create_error = "invalid options (nu1, nu2 in [0,32], nu_coarse in [1,1024], precision 0 or 1, coarse_nodes >= -1)";
+      (o.precision != VLP2D_PREC_FP64 && o.precision != VLP2D_PREC_MIXED) || o.coarse_nodes < -1 ||
+      o.max_inner < 1 || o.max_inner > 8) {
+    g_create_error = "invalid options (nu1, nu2 in [0,32], nu_coarse in [1,1024], precision 0 or 1, coarse_nodes >= -1, "
+                     "max_inner in [1,8])";
     return VLP2D_E_ARG;
   }
   const int nx = segX + 1, ny = segY + 1;
@@ -1132,6 +1200,17 @@ static int create_impl(int segX, int segY, double hx, double hy, const double *f
   if (hipHostMalloc((void **)&c->hscal, 8 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
     c->err = "hipHostMalloc failed";
     return bail(VLP2D_E_OOM);
+  }
+  if (hipHostMalloc((void **)&c->hpub, sizeof(HostPub), hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess ||
+      hipHostGetDevicePointer((void **)&c->hpub_dev, c->hpub, 0) != hipSuccess) {
+    c->err = "hipHostMalloc (mapped result buffer) failed";
+    return bail(VLP2D_E_OOM);
+  }
+  std::memset(c->hpub, 0, sizeof(HostPub));
+  if ((rc = dalloc(c, &c->dseq, 1)) != VLP2D_OK) return bail(rc);
+  if (hipMemsetAsync(c->dseq, 0, sizeof(unsigned long long), c->st) != hipSuccess) {
+    c->err = "hipMemsetAsync failed";
+    return bail(VLP2D_E_DEVICE);
   }
   if (f) {
     // fn = -f on interior nodes, 0 on the boundary (MultiGridSchemeCU.cs:41-45)
@@ -1418,6 +1497,8 @@ void vlp2d_mg_destroy(vlp2d_mg_handle c) {
   dfree(c, c->dscal, 8);
   dfree(c, c->dprog, (size_t)kProgSlots * kExprMaxOps);
   if (c->hscal) (void)hipHostFree(c->hscal);
+  if (c->hpub) (void)hipHostFree(c->hpub);
+  dfree(c, c->dseq, 1);
   if (c->st) (void)hipStreamDestroy(c->st);
   delete c;
 }

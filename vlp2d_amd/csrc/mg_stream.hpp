@@ -414,11 +414,10 @@ __device__ __forceinline__ void sdown_body(const LevelView<T> &a, const STile &t
   }
 }
 
-// STORE = false: the smoothed iterate is not written (zero-initialised
-// levels whose ascent recomputes it, k_sup with N1 = NS).
+// STORE = false: the smoothed iterate is not written (the ascent recomputes
+// it, k_sup with N1 = NS).
 template <typename T, int NS, bool ZI, bool EQ, bool STORE = true>
 __global__ void __launch_bounds__(64 * kWaves) k_sdown(LevelView<T> a, T *__restrict__ rhsc, int ncx, int ncy, int pc) {
-  static_assert(STORE || ZI, "only a zero-initialised descent can be recomputed");
   constexpr int H = s_down_halo(NS);
   STile tl;
   if (!s_tile<H>(a.rows, a.tiles_i, a.tiles_j, a.chunk, a.nx, a.ny, tl)) {
@@ -436,14 +435,14 @@ __global__ void __launch_bounds__(64 * kWaves) k_sdown(LevelView<T> a, T *__rest
 // ---------------------------------------------------------------------------
 // Ascent
 // ---------------------------------------------------------------------------
-template <typename T, int NS, int N1, int EPI, bool EQ, bool PW2, bool EDGE>
+template <typename T, int NS, int N1, int EPI, bool EQ, bool PW2, bool EDGE, bool ZS>
 __device__ __forceinline__ void sup_body(const LevelView<T> &a, const STile &tl, const T *__restrict__ uc, int ncx,
                                          int ncy, int pc, double *__restrict__ part, const MixView &mx) {
-  // N1 > 0: the level was zero-initialised and pre-smoothed with N1 sweeps
-  // by the descent, which did not store it; the pipeline recomputes it from
-  // the right-hand side (half-sweeps 1..S1 on staged zeros, the same
-  // operations in the same order, hence the same bits) and adds the
-  // prolongation to level S1 of each row.
+  // N1 > 0: the level was pre-smoothed with N1 sweeps by the descent, which
+  // did not store it; the pipeline recomputes it (half-sweeps 1..S1 on the
+  // staged iterate: zeros when ZS, else a.uin, the iterate the descent
+  // started from; the same operations in the same order, hence the same
+  // bits) and adds the prolongation to level S1 of each row.
   constexpr int S1 = 2 * N1, ST = S1 + 2 * NS;
   constexpr int H = s_up_halo(N1 + NS, EPI), P = s_period(N1 + NS), TJ = SCOLS - 2 * H;
   static_assert(N1 == 0 || EPI != 2, "the update metric needs the staged iterate");
@@ -493,7 +492,7 @@ __device__ __forceinline__ void sup_body(const LevelView<T> &a, const STile &tl,
   const unsigned boff8 = L.ld ? (unsigned)L.gj * 8u : 0u;
   static_for<-2, 0>([&](auto jc) {
     constexpr int J = decltype(jc)::value;
-    s_prefetch<T, N1 + NS, (N1 > 0), EDGE, J>(st, a, L, gi0, J, LR);
+    s_prefetch<T, N1 + NS, ZS, EDGE, J>(st, a, L, gi0, J, LR);
   });
 #pragma unroll 1
   for (int q0 = 0; q0 < LR; q0 += P) {
@@ -501,7 +500,7 @@ __device__ __forceinline__ void sup_body(const LevelView<T> &a, const STile &tl,
       constexpr int J = decltype(jc)::value;
       const int q = q0 + J;
       group_step();
-      s_prefetch<T, N1 + NS, (N1 > 0), EDGE, J>(st, a, L, gi0, q, LR);
+      s_prefetch<T, N1 + NS, ZS, EDGE, J>(st, a, L, gi0, q, LR);
       // staged row q (level -1: red element J & 1, level 0: black element)
       constexpr int er = J & 1;
       if constexpr (N1 > 0) {
@@ -687,21 +686,25 @@ __device__ __forceinline__ void sup_body(const LevelView<T> &a, const STile &tl,
 
 // EPI 3 requires the FP64 steps of mx to be powers of two exactly when the
 // FP32 steps of a are (the host checks).
-// N1 > 0 pairs with k_sdown<T, N1, true, EQ, false> on the same level.
-template <typename T, int NS, int EPI, bool EQ, int N1 = 0>
+// N1 > 0 pairs with k_sdown<T, N1, ZS, EQ, false> on the same level: ZS, the
+// descent started from zero; otherwise from a.uin, which the ascent then
+// stages (it may not write a.uin: only EPI 3, which writes no iterate).
+template <typename T, int NS, int EPI, bool EQ, int N1 = 0, bool ZS = true>
 __global__ void __launch_bounds__(64 * kWaves) k_sup(LevelView<T> a, const T *__restrict__ uc, int ncx, int ncy, int pc,
                                             double *__restrict__ part, MixView mx) {
   constexpr int H = s_up_halo(N1 + NS, EPI);
+  constexpr bool Z = ZS && N1 > 0;
+  static_assert(ZS || (N1 > 0 && EPI == 3), "a recomputed non-zero start is read back in place: EPI 3 only");
   STile tl;
   if (!s_tile<H>(a.rows, a.tiles_i, a.tiles_j, a.chunk, a.nx, a.ny, tl)) {
     s_idle(a.rows + 2 * H);
     return;
   }
   if (a.pw2) {
-    if (tl.interior) sup_body<T, NS, N1, EPI, EQ, true, false>(a, tl, uc, ncx, ncy, pc, part, mx);
-    else sup_body<T, NS, N1, EPI, EQ, true, true>(a, tl, uc, ncx, ncy, pc, part, mx);
+    if (tl.interior) sup_body<T, NS, N1, EPI, EQ, true, false, Z>(a, tl, uc, ncx, ncy, pc, part, mx);
+    else sup_body<T, NS, N1, EPI, EQ, true, true, Z>(a, tl, uc, ncx, ncy, pc, part, mx);
   } else {
-    sup_body<T, NS, N1, EPI, EQ, false, true>(a, tl, uc, ncx, ncy, pc, part, mx);
+    sup_body<T, NS, N1, EPI, EQ, false, true, Z>(a, tl, uc, ncx, ncy, pc, part, mx);
   }
 }
 

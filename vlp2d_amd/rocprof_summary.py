@@ -47,7 +47,7 @@ KERNEL_CLASS = {
 }
 
 # classes whose largest grid is the level-0 (finest grid) launch
-LEVEL0_CLASSES = {"down_fused", "up_fused", "update_mixed", "resnorm64", "rb_half", "residual", "restrict",
+LEVEL0_CLASSES = {"down_fused", "up_fused", "up_fused_refine", "update_mixed", "resnorm64", "rb_half", "residual", "restrict",
                   "prolong", "copy_interior", "absdiff_max"}
 
 COPY_PATTERN = re.compile(r"copyBuffer", re.I)
@@ -67,6 +67,17 @@ def base_and_precision(short):
     args = short[len(base):]
     prec = "f32" if "float" in args.split(",", 1)[0] else "f64"
     return base, prec
+
+
+def kernel_class(short):
+    """Launch class of a kernel: the ascent with the mixed-precision
+    refinement epilogue (k_sup<T, NS, 3, ...>) is its own class."""
+    base, _ = base_and_precision(short)
+    if base == "k_sup":
+        targs = [a.strip() for a in short[len(base) + 1:].rstrip(">").split(",")]
+        if len(targs) >= 3 and targs[2] == "3":
+            return "up_fused_refine"
+    return KERNEL_CLASS.get(base)
 
 
 def grid_of(row):
@@ -127,8 +138,8 @@ def pmc_report(fetch_csv, write_csv, copy_bytes):
             continue
         b = fetch[k] * 1024.0 * cf + write[k] * 1024.0 * cw
         lines.append(f"| `{k[0]}` | {k[1]} | {nf[k]} | {fetch[k]:.0f} | {write[k]:.0f} | {b:.4g} |")
-        base, prec = base_and_precision(k[0])
-        cls = KERNEL_CLASS.get(base)
+        _, prec = base_and_precision(k[0])
+        cls = kernel_class(k[0])
         if cls not in LEVEL0_CLASSES:
             continue
         tag = f"{cls}@level0/{prec}"

@@ -30,6 +30,7 @@ class Options:
     precision: int = _lib.PREC_FP64
     flags: int = 0
     coarse_nodes: int = 0
+    max_inner: int = 1  # mixed: FP32 V-cycles per FP64 refinement step, at most
 
     @staticmethod
     def performance(mixed=False, **kw):
@@ -40,7 +41,8 @@ class Options:
         return Options(**base)
 
     def to_c(self):
-        return _lib.Opts(self.nu1, self.nu2, self.nu_coarse, self.precision, self.flags, self.coarse_nodes)
+        return _lib.Opts(self.nu1, self.nu2, self.nu_coarse, self.precision, self.flags, self.coarse_nodes,
+                         self.max_inner)
 
 
 def _sample_rhs(f_ksi, nx, ny, hx, hy):
@@ -282,7 +284,7 @@ class MultiGridSchemeHIP:
         if rc != _lib.OK and (raise_on_notconv or rc != _lib.E_NOTCONV):
             check(self.lib, self.h, rc)
         return {"cycles": st.cycles, "levels": st.levels, "r0": st.r0, "rel_residual": st.rel_residual,
-                "seconds": st.seconds, "munknowns_per_s": st.munknowns_per_s, "rc": rc}
+                "seconds": st.seconds, "munknowns_per_s": st.munknowns_per_s, "vcycles": st.vcycles, "rc": rc}
 
     def adjoint_exact(self, dL_du, rtol=1e-12, max_cycles=100):
         shape = (self.dim_x, self.dim_y)
