@@ -74,6 +74,7 @@ KERNEL_PATHS = {
     "small_levels_lds_65": {"coarse_nodes": 65 * 65},   # k_coarse_lds over > 64 KB of LDS in FP64
     "store_presmooth": {"flags": _lib.FLAG_STORE_PRESMOOTH},  # ascent reloads the pre-smoothed level
     "stream_small": {"flags": _lib.FLAG_STREAM_SMALL},  # streaming ascent on levels <= 1025^2 (not k_lt_up)
+    "no_cycle_fusion": {"flags": _lib.FLAG_NO_CYCLE_FUSION},  # two-cycle steps: level 0 of each cycle apart
     "one_op_per_launch": {"flags": _lib.FLAG_SIMPLE},
     "no_graph": {"flags": _lib.FLAG_NO_GRAPH},
     "separate_refine": {"flags": _lib.FLAG_SEPARATE_REFINE},
@@ -132,27 +133,33 @@ def test_kernel_paths_bitwise(path, seg_x, seg_y, hx, hy, laplace):
 def test_recomputed_presmoothing_bitwise(seg_x, seg_y, hx, hy, laplace, mixed, nu2):
     """V(1,nu2): the descent of a zero-initialised level does not store its
     pre-smoothed iterate and the ascent recomputes it (streaming levels), or
-    stores it for the LDS-tiled ascent (levels <= 1025^2).  Same cycles and
-    the same iterate, bit for bit, as the oracle, as the path that stores
-    and reloads every level, and as the streaming ascent on every level.  V(1,1) is left out on the 128 x 256 grid with
+    stores it for the LDS-tiled ascent (levels <= 1025^2); mixed solves also
+    with two FP32 cycles per step, whose level-0 kernels between the cycles
+    run as one launch (ascent continued into the next descent).  Same
+    cycles and the same iterate, bit for bit, as the oracle, as the path
+    that stores and reloads every level, as the streaming ascent on every
+    level and as separate level-0 launches.  V(1,1) is left out on the 128 x 256 grid with
     steps e/128, pi/256 ((hx/hy)^2 = 3): point red-black smoothing diverges
     there (the oracle's residual grows by 1.76 per cycle)."""
     if nu2 == 1 and hx != hy:
         pytest.skip("V(1,1) diverges on the anisotropic grid (oracle included)")
-    _say(f"presmooth {seg_x}x{seg_y} mixed={mixed} nu=(1,{nu2})")
     f, g = _random(seg_x, seg_y, 5 + seg_y, laplace)
     z = np.zeros_like(g) if laplace else f
-    solve = mixed_solve if mixed else plain_solve
-    ur, kr, _ = solve(seg_x, seg_y, hx, hy, z, g, 1, nu2, 3, 1e-10, 60)
-    for flags in (0, _lib.FLAG_STORE_PRESMOOTH, _lib.FLAG_STREAM_SMALL):
-        s = MultiGridSchemeHIP(seg_x, seg_y, hx, hy, z,
-                               options=Options.performance(mixed, nu1=1, nu2=nu2, flags=flags))
-        s.un0[:] = g
-        s.init_after_boundaries_and_initial_iteration_inited()
-        st = s.solve(1e-10, 60)
-        assert st["cycles"] == kr
-        assert np.array_equal(s.get_array(), ur), (flags, np.abs(s.get_array() - ur).max())
-        s.cleanup()
+    for inner in ((1, 2) if mixed else (1,)):
+        _say(f"presmooth {seg_x}x{seg_y} mixed={mixed} nu=(1,{nu2}) inner={inner}")
+        if mixed:
+            ur, kr, _ = mixed_solve(seg_x, seg_y, hx, hy, z, g, 1, nu2, 3, 1e-10, 60, inner=inner)
+        else:
+            ur, kr, _ = plain_solve(seg_x, seg_y, hx, hy, z, g, 1, nu2, 3, 1e-10, 60)
+        for flags in (0, _lib.FLAG_STORE_PRESMOOTH, _lib.FLAG_STREAM_SMALL, _lib.FLAG_NO_CYCLE_FUSION):
+            s = MultiGridSchemeHIP(seg_x, seg_y, hx, hy, z,
+                                   options=Options.performance(mixed, nu1=1, nu2=nu2, flags=flags, max_inner=inner))
+            s.un0[:] = g
+            s.init_after_boundaries_and_initial_iteration_inited()
+            st = s.solve(1e-10, 60)
+            assert st["cycles"] == kr
+            assert np.array_equal(s.get_array(), ur), (flags, inner, np.abs(s.get_array() - ur).max())
+            s.cleanup()
 
 
 def test_config1_reference_stop_rule_cycle_count():

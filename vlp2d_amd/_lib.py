@@ -20,6 +20,7 @@ PREC_FP64, PREC_MIXED = 0, 1
 FLAG_NO_GRAPH, FLAG_PROFILE, FLAG_SIMPLE, FLAG_SEPARATE_REFINE, FLAG_COARSE_L2 = 1, 2, 4, 8, 16
 FLAG_STORE_PRESMOOTH = 32
 FLAG_STREAM_SMALL = 64
+FLAG_NO_CYCLE_FUSION = 128
 
 # Every entry point declared in include/vlp2d_mg.h.
 EXPORTS = [

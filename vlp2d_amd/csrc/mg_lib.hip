@@ -61,12 +61,13 @@ enum KClass {
   KC_SETUP,
   KC_UP_LDS,
   KC_UP_REFINE,
+  KC_UP_DOWN,
   KC_COUNT
 };
 const char *kKClassName[KC_COUNT] = {"rb_half",   "residual",     "restrict", "fill",    "prolong",
                                      "copy_interior", "absdiff_max", "resnorm64", "reduce", "update_mixed",
                                      "adjoint",   "down_fused",   "up_fused", "coarse_vcycle", "setup",
-                                     "up_lds",    "up_fused_refine"};
+                                     "up_lds",    "up_fused_refine", "up_down_fused"};
 
 // Epilogues of the level-0 ascent.
 enum Epi { EPI_NONE = 0, EPI_RESNORM = 1, EPI_METRIC = 2, EPI_MIXED = 3 };
@@ -324,6 +325,7 @@ template <typename T> LevelView<T> view(const Lvl<T> &v, const T *uin, T *uout, 
   a.ihx2 = v.ihx2;
   a.ihy2 = v.ihy2;
   a.pw2 = v.pw2 ? 1 : 0;
+  a.rc = nullptr;
   a.tiles_i = a.tiles_j = a.chunk = a.rows = 0;
   return a;
 }
@@ -523,16 +525,19 @@ template <typename T> void coarse(Ctx *c, Hier<T> &H, bool zero_first, double *s
 // V-cycle with fused kernels.  Levels [0, lc) run k_down / k_up (out of place:
 // u -> u2 on the way down, u2 -> u on the way up); [lc, L) run in k_coarse on
 // u.  The level-0 ascent performs epilogue `epi` (partials into c->part);
-// returns the number of partials it writes (0 when lc == 0).
-template <typename T> int vcycle_fused(Ctx *c, Hier<T> &H, bool zero0, int epi, const MixView &mx, double *snap) {
+// returns the number of partials it writes (0 when lc == 0).  from / to:
+// first descent / last ascent level (1: level 0's kernel is fused with the
+// neighbouring cycle's, up_down0).
+template <typename T>
+int vcycle_fused(Ctx *c, Hier<T> &H, bool zero0, int epi, const MixView &mx, double *snap, int from = 0, int to = 0) {
   const int lc = H.lc;
   auto zi = [&](int l) { return l > 0 || zero0; };
   auto ep = [&](int l) { return l == 0 ? epi : (int)EPI_NONE; };
   auto rz = [&](int l) { return recompute_presmooth(c, zi(l), ep(l)) && !lds_ascent(c, H, l); };
-  for (int l = 0; l < lc; ++l) down(c, H.lv[l], H.lv[l + 1], l, zi(l), !rz(l));
+  for (int l = from; l < lc; ++l) down(c, H.lv[l], H.lv[l + 1], l, zi(l), !rz(l));
   coarse(c, H, lc > 0 || zero0, lc == 0 ? snap : nullptr);
   int np = 0;
-  for (int l = lc - 1; l >= 0; --l) {
+  for (int l = lc - 1; l >= to; --l) {
     if (lds_ascent(c, H, l)) {
       up_lds(c, H.lv[l], H.lv[l + 1], l);
       continue;
@@ -593,6 +598,28 @@ void cycle64(Ctx *c, Hier<double> &H, int epi) {
   if (epi == EPI_METRIC) reduce_part(c, c->part, np, 0, true);
 }
 
+// Level 0 between the two FP32 cycles of a mixed step, one launch: the first
+// cycle's ascent (pre-smoothed iterate recomputed from zero, prolongation,
+// nu2 sweeps, e -> z.u) continues into the second cycle's descent (one sweep
+// on e, residual, restriction into level 1's right-hand side).
+template <int NS> void launch_up_down0_ns(Ctx *c, Lvl<float> &f, Lvl<float> &g) {
+  constexpr int NB = 1, H = s_up_halo(1 + NS + NB, 4), P = s_period(1 + NS + NB);
+  using K = void (*)(LevelView<float>, const float *, int, int, int, double *, MixView);
+  const K k = f.eq ? (K)k_sup<float, NS, 4, true, 1, true, NB> : (K)k_sup<float, NS, 4, false, 1, true, NB>;
+  LevelView<float> a = sview(c, (const void *)k, f, f.u2, f.u, H, P);
+  a.rc = g.rhs;
+  hipLaunchKernelGGL(k, dim3(8 * a.chunk), dim3(64 * kWaves), 0, c->st, a, g.u, g.nx, g.ny, g.pitch, c->part, MixView{});
+}
+void up_down0(Ctx *c, Lvl<float> &f, Lvl<float> &g) {
+  launch(c, KC_UP_DOWN, 0, 4.0 * (2.0 * nodes(f) + 2.0 * nodes(g)), [&] {
+    switch (c->o.nu2) {
+      case 1: launch_up_down0_ns<1>(c, f, g); break;
+      case 2: launch_up_down0_ns<2>(c, f, g); break;
+      default: launch_up_down0_ns<3>(c, f, g); break;
+    }
+  });
+}
+
 // One mixed-precision refinement step on the FP64 problem level P, using h32:
 // e = V-cycle(rhs32) from e = 0; u_next = u + e (max|e| -> dscal[0]);
 // r32_next = fn - Lap u_next, ||r||^2 -> dscal[1].  u/u2 of P and rhs/rhs2 of
@@ -603,6 +630,30 @@ void mixed_step(Ctx *c, Lvl<double> &P, int m = 1) {
   Hier<float> &Z = c->h32;
   Lvl<float> &z = Z.lv[0];
   MixView none{};
+  MixView mx;
+  mx.u64in = P.u;
+  mx.u64out = P.u2;
+  mx.fn64 = P.rhs;
+  mx.r32out = z.rhs2;
+  mx.hx2 = P.hx2;
+  mx.hy2 = P.hy2;
+  mx.ihx2 = P.ihx2;
+  mx.ihy2 = P.ihy2;
+  mx.pw2 = P.pw2 ? 1 : 0;
+  mx.part_max = c->part2;
+  const bool fused_refine = !c->simple && Z.lc > 0 && !(c->o.flags & VLP2D_FLAG_SEPARATE_REFINE) &&
+                            z.pw2 == P.pw2 && z.pitch == P.pitch;
+  if (m == 2 && fused_refine && c->o.nu1 == 1 && c->o.nu2 <= 3 &&
+      !(c->o.flags & (VLP2D_FLAG_STORE_PRESMOOTH | VLP2D_FLAG_NO_CYCLE_FUSION))) {
+    // first cycle down to level 1, level 0 of both cycles in between in one
+    // launch, second cycle from level 1 with the refinement ascent
+    vcycle_fused(c, Z, true, EPI_NONE, none, nullptr, 0, 1);
+    up_down0(c, z, Z.lv[1]);
+    const int np = vcycle_fused(c, Z, false, EPI_MIXED, mx, nullptr, 1, 0);
+    reduce_part(c, c->part2, np, 0, true);
+    reduce_part(c, c->part, np, 1, false);
+    return;
+  }
   // all cycles but the last (correction e -> z.u)
   for (int k = 0; k + 1 < m; ++k) {
     if (c->simple) vcycle_simple(c, Z, k == 0, nullptr);
@@ -621,18 +672,7 @@ void mixed_step(Ctx *c, Lvl<double> &P, int m = 1) {
     resnorm(c, P, P.u2, 1, z.rhs2);
     return;
   }
-  MixView mx;
-  mx.u64in = P.u;
-  mx.u64out = P.u2;
-  mx.fn64 = P.rhs;
-  mx.r32out = z.rhs2;
-  mx.hx2 = P.hx2;
-  mx.hy2 = P.hy2;
-  mx.ihx2 = P.ihx2;
-  mx.ihy2 = P.ihy2;
-  mx.pw2 = P.pw2 ? 1 : 0;
-  mx.part_max = c->part2;
-  if (!(c->o.flags & VLP2D_FLAG_SEPARATE_REFINE) && z.pw2 == P.pw2 && z.pitch == P.pitch) {
+  if (fused_refine) {
     // refinement update inside the level-0 ascent of the FP32 cycle (the
     // kernel evaluates both precisions' divisions the same way, as
     // multiplications when the steps are powers of two)

@@ -7,7 +7,7 @@
 // top to bottom, one staged row per step, LR = R + 2H rows in all.  Every
 // row is loaded from HBM once per tile; all red-black Gauss-Seidel
 // half-sweeps of the launch are pipelined over the rows (temporal blocking in
-// registers): at step q the wavefront loads row q + 2 (prefetch), applies
+// registers): at step q the wavefront loads row q + kPf (prefetch), applies
 // half-sweep s to row q - s for s = 1..S, and runs the epilogue on the rows
 // the last half-sweep has finished.  Vertical neighbours are registers of the
 // same lane; horizontal neighbours of the other lane come through DPP
@@ -46,7 +46,11 @@
 //             (FP32 correction e on level 0 of a mixed-precision step: the
 //             FP64 refinement u64 += e, r32 = (float)(fn - Lap u64), partial
 //             sums of r^2 and partial max |e|; e itself is not written;
-//             H = S + 2).
+//             H = S + 2) or 4 (the level-0 correction e of the first FP32
+//             cycle of a two-cycle mixed step: write e, then continue into
+//             the second cycle's descent on the same rows, NB pre-sweeps on
+//             e, residual and half-weighting restriction into the coarse
+//             right-hand side a.rc; H = S + 2NB + 2).
 // On a zero-initialised level the pre-smoothed iterate is a local function
 // of the right-hand side (radius 2 NS1 for NS1 pre-sweeps).  There the
 // descent may skip writing it (k_sdown STORE = false) and the ascent
@@ -79,10 +83,19 @@ __device__ __forceinline__ void group_step() {
 
 // Steps per unrolled loop body: a multiple of the colour period (2), the
 // level rings (6) and the right-hand-side ring (S + 4 rows live).
-__host__ __device__ constexpr int s_period(int ns) { return 2 * ns + 4 <= 6 ? 6 : 12; }
+// Prefetch distance: staged row q + kPf is loaded at step q.
+#ifndef VLP2D_PF
+#define VLP2D_PF 2
+#endif
+constexpr int kPf = VLP2D_PF;
+constexpr int kNru = kPf <= 2 ? 3 : 6;  // prefetch ring slots (a divisor of every period)
+static_assert(kPf >= 1 && kPf <= 5, "prefetch distance");
+__host__ __device__ constexpr int s_period(int ns) {
+  return 2 * ns + kPf + 2 <= 6 ? 6 : 2 * ns + kPf + 2 <= 12 ? 12 : 18;
+}
 __host__ __device__ constexpr int s_down_halo(int ns) { return 2 * ns + 2; }
 __host__ __device__ constexpr int s_up_halo(int ns, int epi) {
-  return (epi == 1 || epi == 3) ? 2 * ns + 2 : (2 * ns < 2 ? 2 : 2 * ns);
+  return (epi == 1 || epi == 3 || epi == 4) ? 2 * ns + 2 : (2 * ns < 2 ? 2 : 2 * ns);
 }
 // Output rows per tile R (a launch parameter): the staged rows R + 2H must
 // be a whole number of loop bodies; the smallest such R >= want.
@@ -231,10 +244,10 @@ __device__ __forceinline__ void s_store(T *uout, int pitch, int gi, const Lane &
 template <typename T, int NS> struct SweepState {
   static constexpr int S = 2 * NS;
   static constexpr int P = s_period(NS);
-  static_assert(S + 4 <= P, "right-hand-side ring too short for the pipeline");
+  static_assert(S + kPf + 2 <= P, "right-hand-side ring too short for the pipeline");
   T U[S + 2][6];  // U[s + 1][p % 6]: level s in -1..S
   T F[P][2];      // right-hand side of row p (0 on Laplace levels and outside the domain)
-  T RU[3][2];     // prefetched staged rows (raw)
+  T RU[kNru][2];  // prefetched staged rows (raw)
   __device__ void zero() {
 #pragma unroll
     for (int s = 0; s < S + 2; ++s)
@@ -243,7 +256,7 @@ template <typename T, int NS> struct SweepState {
 #pragma unroll
     for (int k = 0; k < P; ++k) F[k][0] = F[k][1] = T(0);
 #pragma unroll
-    for (int k = 0; k < 3; ++k) RU[k][0] = RU[k][1] = T(0);
+    for (int k = 0; k < kNru; ++k) RU[k][0] = RU[k][1] = T(0);
   }
   // full value of element e of row p = q - d (J the step phase) after
   // half-sweep s
@@ -289,13 +302,13 @@ __device__ __forceinline__ void s_sweeps(SweepState<T, NS> &st, const LevelView<
   });
 }
 
-// Load the raw pair of staged row q + 2 (u unless zero-initialised) and its
-// right-hand side.
+// Load the raw pair of staged row q + kPf (u unless zero-initialised) and
+// its right-hand side.
 template <typename T, int NS, bool ZI, bool EDGE, int J>
 __device__ __forceinline__ void s_prefetch(SweepState<T, NS> &st, const LevelView<T> &a, const Lane &L, int gi0, int q,
                                            int LR) {
   constexpr int P = SweepState<T, NS>::P;
-  const int qq = q + 2, gi = gi0 + qq;
+  const int qq = q + kPf, gi = gi0 + qq;
   Pair<T> u{T(0), T(0)}, f{T(0), T(0)};
   bool go = qq < LR;
   if constexpr (EDGE) go = go && gi >= 0 && gi <= a.nx - 1 && L.ld;
@@ -303,10 +316,10 @@ __device__ __forceinline__ void s_prefetch(SweepState<T, NS> &st, const LevelVie
     if constexpr (!ZI) u = ld_pair(row_of(a.uin, gi, a.pitch), L.boff);
     if (!a.lap) f = ld_pair(row_of(a.f, gi, a.pitch), L.boff);
   }
-  st.RU[md<3>(J + 2)][0] = u.x0;
-  st.RU[md<3>(J + 2)][1] = u.x1;
-  st.F[md<P>(J + 2)][0] = f.x0;
-  st.F[md<P>(J + 2)][1] = f.x1;
+  st.RU[md<kNru>(J + kPf)][0] = u.x0;
+  st.RU[md<kNru>(J + kPf)][1] = u.x1;
+  st.F[md<P>(J + kPf)][0] = f.x0;
+  st.F[md<P>(J + kPf)][1] = f.x1;
 }
 
 // Tile of this workgroup: output rows [ti R, ti R + R), columns
@@ -356,8 +369,8 @@ __device__ __forceinline__ void sdown_body(const LevelView<T> &a, const STile &t
   T RR[3][2];  // residual of rows (slot p % 3); element 1 only on even rows
 #pragma unroll
   for (int k = 0; k < 3; ++k) RR[k][0] = RR[k][1] = T(0);
-  // rows 0 and 1 (the loop prefetches q + 2)
-  static_for<-2, 0>([&](auto jc) {
+  // rows 0 .. kPf - 1 (the loop prefetches q + kPf)
+  static_for<-kPf, 0>([&](auto jc) {
     constexpr int J = decltype(jc)::value;
     s_prefetch<T, NS, ZI, EDGE, J>(st, a, L, gi0, J, LR);
   });
@@ -373,8 +386,8 @@ __device__ __forceinline__ void sdown_body(const LevelView<T> &a, const STile &t
       s_prefetch<T, NS, ZI, EDGE, J>(st, a, L, gi0, q, LR);
       // level 0 of row q: red element (J & 1), black element
       constexpr int er = J & 1;
-      st.U[0][md<6>(J)] = st.RU[md<3>(J)][er];
-      st.U[1][md<6>(J)] = st.RU[md<3>(J)][1 - er];
+      st.U[0][md<6>(J)] = st.RU[md<kNru>(J)][er];
+      st.U[1][md<6>(J)] = st.RU[md<kNru>(J)][1 - er];
       s_sweeps<T, NS, EQ, PW2, EDGE, J>(st, a, L, gi0, q);
       // store row q - S (unless the ascent recomputes it)
       if constexpr (STORE) {
@@ -435,7 +448,7 @@ __global__ void __launch_bounds__(64 * kWaves) k_sdown(LevelView<T> a, T *__rest
 // ---------------------------------------------------------------------------
 // Ascent
 // ---------------------------------------------------------------------------
-template <typename T, int NS, int N1, int EPI, bool EQ, bool PW2, bool EDGE, bool ZS>
+template <typename T, int NS, int N1, int EPI, bool EQ, bool PW2, bool EDGE, bool ZS, int NB>
 __device__ __forceinline__ void sup_body(const LevelView<T> &a, const STile &tl, const T *__restrict__ uc, int ncx,
                                          int ncy, int pc, double *__restrict__ part, const MixView &mx) {
   // N1 > 0: the level was pre-smoothed with N1 sweeps by the descent, which
@@ -443,13 +456,16 @@ __device__ __forceinline__ void sup_body(const LevelView<T> &a, const STile &tl,
   // staged iterate: zeros when ZS, else a.uin, the iterate the descent
   // started from; the same operations in the same order, hence the same
   // bits) and adds the prolongation to level S1 of each row.
-  constexpr int S1 = 2 * N1, ST = S1 + 2 * NS;
-  constexpr int H = s_up_halo(N1 + NS, EPI), P = s_period(N1 + NS), TJ = SCOLS - 2 * H;
+  // half-sweeps: 1..S1 recompute, S1+1..ST post-smoothing, ST+1..SE the
+  // next cycle's pre-smoothing (EPI 4)
+  constexpr int NT = N1 + NS + NB, S1 = 2 * N1, ST = S1 + 2 * NS, SE = ST + 2 * NB;
+  constexpr int H = s_up_halo(NT, EPI), P = s_period(NT), TJ = SCOLS - 2 * H;
   static_assert(N1 == 0 || EPI != 2, "the update metric needs the staged iterate");
+  static_assert((NB > 0) == (EPI == 4), "pre-sweeps of the next cycle belong to epilogue 4");
   const int R = a.rows, LR = R + 2 * H;  // LR is a multiple of P
   const int gi0 = tl.gi0, cb = tl.cb;
   const Lane L = make_lane<T>(cb, H, TJ, a.ny, a.pitch);
-  SweepState<T, N1 + NS> st;
+  SweepState<T, NT> st;
   st.zero();
   // coarse rows kk = 0, 1, ... of the strip: global row gi0/2 + kk, column
   // Jc = cb/2 + m (fine column gj = 2 Jc); ring slot kk % 3.  The right
@@ -490,9 +506,16 @@ __device__ __forceinline__ void sup_body(const LevelView<T> &a, const STile &tl,
 #pragma unroll
   for (int k = 0; k < N3; ++k) X64[k][0] = X64[k][1] = XR[k][0] = XR[k][1] = FN[k][0] = FN[k][1] = 0.0;
   const unsigned boff8 = L.ld ? (unsigned)L.gj * 8u : 0u;
-  static_for<-2, 0>([&](auto jc) {
+  // EPI 4: residual of rows (slot p % 3; element 1 only on even rows) and
+  // the coarse output column of element 0 (fine column 2 Jc)
+  constexpr int N4 = EPI == 4 ? 3 : 1;
+  T RR[N4][2];
+#pragma unroll
+  for (int k = 0; k < N4; ++k) RR[k][0] = RR[k][1] = T(0);
+  const bool oc_c = EDGE ? L.oc && Jc >= 1 && Jc <= ncy - 2 : L.oc;
+  static_for<-kPf, 0>([&](auto jc) {
     constexpr int J = decltype(jc)::value;
-    s_prefetch<T, N1 + NS, ZS, EDGE, J>(st, a, L, gi0, J, LR);
+    s_prefetch<T, NT, ZS, EDGE, J>(st, a, L, gi0, J, LR);
   });
 #pragma unroll 1
   for (int q0 = 0; q0 < LR; q0 += P) {
@@ -500,13 +523,13 @@ __device__ __forceinline__ void sup_body(const LevelView<T> &a, const STile &tl,
       constexpr int J = decltype(jc)::value;
       const int q = q0 + J;
       group_step();
-      s_prefetch<T, N1 + NS, ZS, EDGE, J>(st, a, L, gi0, q, LR);
+      s_prefetch<T, NT, ZS, EDGE, J>(st, a, L, gi0, q, LR);
       // staged row q (level -1: red element J & 1, level 0: black element)
       constexpr int er = J & 1;
       if constexpr (N1 > 0) {
-        st.U[0][md<6>(J)] = st.RU[md<3>(J)][er];
-        st.U[1][md<6>(J)] = st.RU[md<3>(J)][1 - er];
-        s_sweeps<T, N1 + NS, EQ, PW2, EDGE, J, 1, S1>(st, a, L, gi0, q);
+        st.U[0][md<6>(J)] = st.RU[md<kNru>(J)][er];
+        st.U[1][md<6>(J)] = st.RU[md<kNru>(J)][1 - er];
+        s_sweeps<T, NT, EQ, PW2, EDGE, J, 1, S1>(st, a, L, gi0, q);
       }
       // prolongation of row pr = q - S1, whose level S1 is now complete
       {
@@ -535,7 +558,7 @@ __device__ __forceinline__ void sup_body(const LevelView<T> &a, const STile &tl,
           a1 = rin && L.in1;
         }
         if constexpr (N1 == 0) {
-          T v0 = st.RU[md<3>(J)][0], v1 = st.RU[md<3>(J)][1];
+          T v0 = st.RU[md<kNru>(J)][0], v1 = st.RU[md<kNru>(J)][1];
           if (a0) v0 = v0 + p0;
           if (a1) v1 = v1 + p1;
           st.U[0][md<6>(J)] = er == 0 ? v0 : v1;
@@ -573,7 +596,7 @@ __device__ __forceinline__ void sup_body(const LevelView<T> &a, const STile &tl,
         FN[md<3>(J - ST + 1)][0] = f.x0;
         FN[md<3>(J - ST + 1)][1] = f.x1;
       }
-      s_sweeps<T, N1 + NS, EQ, PW2, EDGE, J, S1 + 1, ST>(st, a, L, gi0, q);
+      s_sweeps<T, NT, EQ, PW2, EDGE, J, S1 + 1, ST>(st, a, L, gi0, q);
       // store row q - ST (+ update metric)
       {
         const int p = q - ST, gi = gi0 + p;
@@ -602,6 +625,37 @@ __device__ __forceinline__ void sup_body(const LevelView<T> &a, const STile &tl,
           if (orow && L.oc) {
             if (!EDGE || L.in0) acc = nanmax(acc, fabs((double)(x0 - V0[pv][0])));
             if (!EDGE || L.in1) acc = nanmax(acc, fabs((double)(x1 - V0[pv][1])));
+          }
+        }
+      }
+      if constexpr (EPI == 4) {
+        // the next cycle's pre-smoothing of the rows just stored, then the
+        // residual of row q - SE - 1 and the restriction at even row
+        // q - SE - 2 (as k_sdown, MultiGridScheme.cs:134-169)
+        s_sweeps<T, NT, EQ, PW2, EDGE, J, ST + 1, SE>(st, a, L, gi0, q);
+        {
+          constexpr int d = SE + 1;
+          constexpr int pr = J - d;
+          const T f0 = st.F[md<P>(pr)][0], f1 = st.F[md<P>(pr)][1];
+          const T x0 = st.template full<SE, J, d, 0>(), x1 = st.template full<SE, J, d, 1>();
+          const T a0 = st.template full<SE, J, d + 1, 0>(), b0 = st.template full<SE, J, d - 1, 0>();
+          RR[md<3>(pr)][0] = s_res<T, PW2>(x0, a0, b0, from_left(x1), x1, f0, a);
+          if constexpr (md<2>(pr) == 0) {
+            const T a1 = st.template full<SE, J, d + 1, 1>(), b1 = st.template full<SE, J, d - 1, 1>();
+            RR[md<3>(pr)][1] = s_res<T, PW2>(x1, a1, b1, x0, from_right(x0), f1, a);
+          }
+        }
+        {
+          constexpr int px = J - SE - 2;
+          if constexpr (md<2>(px) == 0) {
+            const int p = q - SE - 2;
+            const T r0 = RR[md<3>(px)][0], rjp = RR[md<3>(px)][1];
+            const T rjm = from_left(rjp);
+            const T rim = RR[md<3>(px - 1)][0], rip = RR[md<3>(px + 1)][0];
+            const int I = (gi0 + p) / 2;
+            bool go = p >= H && p < H + R;
+            if constexpr (EDGE) go = go && I >= 1 && I <= ncx - 2;
+            if (go && oc_c) a.rc[I * pc + Jc] = T(0.125) * (r0 * T(4) + (rip + rim + rjp + rjm));
           }
         }
       }
@@ -689,10 +743,10 @@ __device__ __forceinline__ void sup_body(const LevelView<T> &a, const STile &tl,
 // N1 > 0 pairs with k_sdown<T, N1, ZS, EQ, false> on the same level: ZS, the
 // descent started from zero; otherwise from a.uin, which the ascent then
 // stages (it may not write a.uin: only EPI 3, which writes no iterate).
-template <typename T, int NS, int EPI, bool EQ, int N1 = 0, bool ZS = true>
+template <typename T, int NS, int EPI, bool EQ, int N1 = 0, bool ZS = true, int NB = 0>
 __global__ void __launch_bounds__(64 * kWaves) k_sup(LevelView<T> a, const T *__restrict__ uc, int ncx, int ncy, int pc,
                                             double *__restrict__ part, MixView mx) {
-  constexpr int H = s_up_halo(N1 + NS, EPI);
+  constexpr int H = s_up_halo(N1 + NS + NB, EPI);
   constexpr bool Z = ZS && N1 > 0;
   static_assert(ZS || (N1 > 0 && EPI == 3), "a recomputed non-zero start is read back in place: EPI 3 only");
   STile tl;
@@ -701,10 +755,10 @@ __global__ void __launch_bounds__(64 * kWaves) k_sup(LevelView<T> a, const T *__
     return;
   }
   if (a.pw2) {
-    if (tl.interior) sup_body<T, NS, N1, EPI, EQ, true, false, Z>(a, tl, uc, ncx, ncy, pc, part, mx);
-    else sup_body<T, NS, N1, EPI, EQ, true, true, Z>(a, tl, uc, ncx, ncy, pc, part, mx);
+    if (tl.interior) sup_body<T, NS, N1, EPI, EQ, true, false, Z, NB>(a, tl, uc, ncx, ncy, pc, part, mx);
+    else sup_body<T, NS, N1, EPI, EQ, true, true, Z, NB>(a, tl, uc, ncx, ncy, pc, part, mx);
   } else {
-    sup_body<T, NS, N1, EPI, EQ, false, true, Z>(a, tl, uc, ncx, ncy, pc, part, mx);
+    sup_body<T, NS, N1, EPI, EQ, false, true, Z, NB>(a, tl, uc, ncx, ncy, pc, part, mx);
   }
 }
 

@@ -9,6 +9,7 @@ template <typename T> struct LevelView {
   const T *uin;  // staged iterate (ignored when zero-initialised)
   T *uout;       // written iterate (tile interior nodes of the domain)
   const T *f;    // right-hand side (fn on level 0, restricted residual below)
+  T *rc;         // coarse right-hand side an ascent continuing into the next descent restricts into
   int nx, ny, pitch;
   T hx2, hy2, coef;
   T ihx2, ihy2;  // 1/hx2, 1/hy2: exact when pw2 (both squares are powers of two)
