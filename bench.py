@@ -99,9 +99,10 @@ def cpu_baseline(seg, rtol, nu):
     return rec, u
 
 
-def dd128_check(nu, seg=512, rtol=1e-10):
-    """Error of the GPU mixed-precision solution of the 513^2 replica against
-    the oracle's double-double multigrid converged to its floor."""
+def dd128_check(nu, seg=512, rtol=1e-10, inner=1):
+    """Errors of the GPU mixed-precision and FP64 solutions (to rtol) of the
+    513^2 replica against the oracle's double-double multigrid converged to
+    its floor: (mixed, fp64)."""
     import numpy as np
     from oracle import Oracle
     from vlp2d_amd import MultiGridSchemeHIP, Options, problems
@@ -113,13 +114,18 @@ def dd128_check(nu, seg=512, rtol=1e-10):
             break
     hi, lo = o.get_u(with_lo=True)
     o.close()
-    s = MultiGridSchemeHIP(p.seg_x, p.seg_y, p.hx, p.hy, p.f, options=Options.performance(True, nu1=nu[0], nu2=nu[1]))
-    s.un0[:] = p.initial("zero")
-    s.init_after_boundaries_and_initial_iteration_inited()
-    s.solve(rtol, 100)
-    u = s.get_array()
-    s.cleanup()
-    return float(np.abs((u - hi) - lo).max() / np.abs(hi).max())
+    err = []
+    for mixed in (True, False):
+        s = MultiGridSchemeHIP(p.seg_x, p.seg_y, p.hx, p.hy, p.f,
+                               options=Options.performance(mixed, nu1=nu[0], nu2=nu[1],
+                                                           max_inner=inner if mixed else 1))
+        s.un0[:] = p.initial("zero")
+        s.init_after_boundaries_and_initial_iteration_inited()
+        s.solve(rtol, 100)
+        u = s.get_array()
+        s.cleanup()
+        err.append(float(np.abs((u - hi) - lo).max() / np.abs(hi).max()))
+    return tuple(err)
 
 
 def main():
@@ -206,7 +212,9 @@ def main():
             scale = np.abs(u_oracle).max()
             accuracy["mixed_vs_fp64_oracle_rel"] = float(np.abs(res[True]["u"] - u_oracle).max() / scale)
             accuracy["fp64_gpu_vs_fp64_oracle_rel"] = float(np.abs(res[False]["u"] - u_oracle).max() / scale)
-        accuracy["mixed_513_vs_dd128_oracle_rel"] = dd128_check(nu)
+        em, e64 = dd128_check(nu, inner=args.inner)
+        accuracy["mixed_513_vs_dd128_oracle_rel"] = em
+        accuracy["fp64_513_vs_dd128_oracle_rel"] = e64
 
     line = {
         "metric": "Munknowns/s to rel. residual 1e-10: FP32+FP64-refine vs FP64 MG; % HBM roofline",

@@ -47,7 +47,7 @@ KERNEL_CLASS = {
 }
 
 # classes whose largest grid is the level-0 (finest grid) launch
-LEVEL0_CLASSES = {"down_fused", "up_fused", "up_fused_refine", "update_mixed", "resnorm64", "rb_half", "residual", "restrict",
+LEVEL0_CLASSES = {"down_fused", "up_fused", "up_fused_refine", "up_down_fused", "update_mixed", "resnorm64", "rb_half", "residual", "restrict",
                   "prolong", "copy_interior", "absdiff_max"}
 
 COPY_PATTERN = re.compile(r"copyBuffer", re.I)
@@ -71,12 +71,18 @@ def base_and_precision(short):
 
 def kernel_class(short):
     """Launch class of a kernel: the ascent with the mixed-precision
-    refinement epilogue (k_sup<T, NS, 3, ...>) is its own class."""
+    refinement epilogue (k_sup<T, NS, 3, ...>) and the ascent continued into
+    the next cycle's descent (k_sup<T, NS, 4, ...>) are classes of their own;
+    k_refine_stream<false> is the residual norm, <true> the update."""
     base, _ = base_and_precision(short)
-    if base == "k_sup":
-        targs = [a.strip() for a in short[len(base) + 1:].rstrip(">").split(",")]
-        if len(targs) >= 3 and targs[2] == "3":
+    targs = [a.strip() for a in short[len(base) + 1:].rstrip(">").split(",")]
+    if base == "k_sup" and len(targs) >= 3:
+        if targs[2] == "3":
             return "up_fused_refine"
+        if targs[2] == "4":
+            return "up_down_fused"
+    if base == "k_refine_stream" and targs and targs[0] == "false":
+        return "resnorm64"
     return KERNEL_CLASS.get(base)
 
 
