@@ -65,15 +65,35 @@ __global__ void __launch_bounds__(RED_THREADS) k_reduce(const double *__restrict
 }
 
 // The two result scalars of a cycle, published to mapped, coherent pinned
-// host memory: values first, then (after a system-scope fence) the sequence
-// number the host polls for.  The host thus learns a cycle's result without
-// a device-to-host copy and a stream synchronisation.
+// host memory (k_finish): values first, then, after a system-scope fence, the
+// sequence number the host polls for.  The host thus learns a cycle's result
+// without a device-to-host copy and a stream synchronisation.
 struct HostPub {
   double v[2];
   unsigned long long seq;
 };
-__global__ void k_publish(const double *__restrict__ dscal, unsigned long long *__restrict__ dseq, HostPub *pub) {
-  if (threadIdx.x == 0) {
+
+// The reductions that end a cycle or a refinement step, in one launch:
+// dscal[1] = sum of part_sum[0..nsum); with MAX also dscal[0] = max of
+// part_max[0..nmax) (NaN-propagating); then, with pub, the two scalars are
+// published to the HostPub.  Same summation order as k_reduce.
+template <bool MAX>
+__global__ void __launch_bounds__(RED_THREADS) k_finish(const double *__restrict__ part_max, int nmax,
+                                                        const double *__restrict__ part_sum, int nsum,
+                                                        double *__restrict__ dscal, unsigned long long *dseq,
+                                                        HostPub *pub) {
+  __shared__ double sm[RED_THREADS / 64];
+  double v = 0.0;
+  for (int k = threadIdx.x; k < nsum; k += RED_THREADS) v = v + part_sum[k];
+  v = block_reduce<false>(v, sm, threadIdx.x, RED_THREADS);
+  if (threadIdx.x == 0) dscal[1] = v;
+  if constexpr (MAX) {
+    double w = 0.0;
+    for (int k = threadIdx.x; k < nmax; k += RED_THREADS) w = nanmax(w, part_max[k]);
+    w = block_reduce<true>(w, sm, threadIdx.x, RED_THREADS);
+    if (threadIdx.x == 0) dscal[0] = w;
+  }
+  if (pub && threadIdx.x == 0) {
     const unsigned long long s = dseq[0] + 1;
     dseq[0] = s;
     pub->v[0] = dscal[0];

@@ -548,8 +548,23 @@ int vcycle_fused(Ctx *c, Hier<T> &H, bool zero0, int epi, const MixView &mx, dou
   return np;
 }
 
-// residual norm^2 of level 0 (FP64) -> dscal[slot]; optional FP32 copy
-void resnorm(Ctx *c, Lvl<double> &v, const double *u, int slot, float *r32) {
+// dscal[1] = sum of psum[0..nsum); with pmax also dscal[0] = max of
+// pmax[0..nmax); pub: then publish both for the solve loop (publish_wait).
+void finish(Ctx *c, const double *pmax, int nmax, const double *psum, int nsum, bool pub) {
+  HostPub *hp = pub && !c->profile ? c->hpub_dev : nullptr;
+  launch(c, KC_REDUCE, 0, 8.0 * (nmax + nsum), [&] {
+    if (pmax)
+      hipLaunchKernelGGL(k_finish<true>, dim3(1), dim3(RED_THREADS), 0, c->st, pmax, nmax, psum, nsum, c->dscal,
+                         c->dseq, hp);
+    else
+      hipLaunchKernelGGL(k_finish<false>, dim3(1), dim3(RED_THREADS), 0, c->st, pmax, nmax, psum, nsum, c->dscal,
+                         c->dseq, hp);
+  });
+}
+
+// residual norm^2 of level 0 (FP64) -> dscal[1]; optional FP32 copy; pub:
+// published for the solve loop
+void resnorm(Ctx *c, Lvl<double> &v, const double *u, float *r32, bool pub = false) {
   // column-streaming tiles of the refinement kernel without the update
   const int rows = refine_row_blocks(v.nx);
   const int np = refine_strips(v.ny) * rows;
@@ -559,7 +574,7 @@ void resnorm(Ctx *c, Lvl<double> &v, const double *u, int slot, float *r32) {
                        v.nx, v.ny, v.pitch, (double)v.hx2, (double)v.hy2, (double)v.ihx2, (double)v.ihy2,
                        v.pw2 ? 1 : 0, rows, np, chunk, c->part, nullptr);
   });
-  launch(c, KC_REDUCE, 0, 8.0 * np, [&] { hipLaunchKernelGGL(k_reduce<false>, dim3(1), dim3(RED_THREADS), 0, c->st, c->part, np, c->dscal + slot); });
+  finish(c, nullptr, 0, c->part, np, pub);
 }
 
 void reduce_part(Ctx *c, const double *part, int np, int slot, bool mx) {
@@ -582,19 +597,19 @@ void maxdiff(Ctx *c, Lvl<double> &v, const double *snap, int slot) {
 
 // FP64 cycle on H; epi: EPI_NONE, EPI_RESNORM (||r||^2 -> dscal[1]) or
 // EPI_METRIC (update norm -> dscal[0]).
-void cycle64(Ctx *c, Hier<double> &H, int epi) {
+void cycle64(Ctx *c, Hier<double> &H, int epi, bool pub = false) {
   Lvl<double> &P = H.lv[0];
   MixView mx{};
   if (c->simple || H.lc == 0) {
     double *snap = epi == EPI_METRIC ? c->snap : nullptr;
     if (c->simple) vcycle_simple(c, H, false, snap);
     else vcycle_fused(c, H, false, EPI_NONE, mx, snap);
-    if (epi == EPI_RESNORM) resnorm(c, P, P.u, 1, nullptr);
+    if (epi == EPI_RESNORM) resnorm(c, P, P.u, nullptr, pub);
     if (epi == EPI_METRIC) maxdiff(c, P, c->snap, 0);
     return;
   }
   const int np = vcycle_fused(c, H, false, epi, mx, nullptr);
-  if (epi == EPI_RESNORM) reduce_part(c, c->part, np, 1, false);
+  if (epi == EPI_RESNORM) finish(c, nullptr, 0, c->part, np, pub);
   if (epi == EPI_METRIC) reduce_part(c, c->part, np, 0, true);
 }
 
@@ -626,7 +641,7 @@ void up_down0(Ctx *c, Lvl<float> &f, Lvl<float> &g) {
 // h32 level 0 are swapped by the caller afterwards.
 // m >= 2: the correction equation gets m FP32 V-cycles, the first from
 // e = 0, the refinement update follows the last.
-void mixed_step(Ctx *c, Lvl<double> &P, int m = 1) {
+void mixed_step(Ctx *c, Lvl<double> &P, int m = 1, bool pub = false) {
   Hier<float> &Z = c->h32;
   Lvl<float> &z = Z.lv[0];
   MixView none{};
@@ -650,8 +665,7 @@ void mixed_step(Ctx *c, Lvl<double> &P, int m = 1) {
     vcycle_fused(c, Z, true, EPI_NONE, none, nullptr, 0, 1);
     up_down0(c, z, Z.lv[1]);
     const int np = vcycle_fused(c, Z, false, EPI_MIXED, mx, nullptr, 1, 0);
-    reduce_part(c, c->part2, np, 0, true);
-    reduce_part(c, c->part, np, 1, false);
+    finish(c, c->part2, np, c->part, np, pub);
     return;
   }
   // all cycles but the last (correction e -> z.u)
@@ -669,7 +683,7 @@ void mixed_step(Ctx *c, Lvl<double> &P, int m = 1) {
       hipLaunchKernelGGL(k_update_oop, g, kBlock, 0, c->st, P.u, P.u2, z.u, P.nx, P.ny, P.pitch, c->part);
     });
     reduce_part(c, c->part, np, 0, true);
-    resnorm(c, P, P.u2, 1, z.rhs2);
+    resnorm(c, P, P.u2, z.rhs2, pub);
     return;
   }
   if (fused_refine) {
@@ -677,8 +691,7 @@ void mixed_step(Ctx *c, Lvl<double> &P, int m = 1) {
     // kernel evaluates both precisions' divisions the same way, as
     // multiplications when the steps are powers of two)
     const int np = vcycle_fused(c, Z, zero_last, EPI_MIXED, mx, nullptr);
-    reduce_part(c, c->part2, np, 0, true);
-    reduce_part(c, c->part, np, 1, false);
+    finish(c, c->part2, np, c->part, np, pub);
     return;
   }
   vcycle_fused(c, Z, zero_last, EPI_NONE, mx, nullptr);  // correction e -> z.u
@@ -690,8 +703,7 @@ void mixed_step(Ctx *c, Lvl<double> &P, int m = 1) {
                        P.ny, P.pitch, (double)P.hx2, (double)P.hy2, (double)P.ihx2, (double)P.ihy2, P.pw2 ? 1 : 0, rows,
                        np, chunk, c->part, c->part2);
   });
-  reduce_part(c, c->part2, np, 0, true);
-  reduce_part(c, c->part, np, 1, false);
+  finish(c, c->part2, np, c->part, np, pub);
 }
 
 void swap_mixed(Ctx *c, Lvl<double> &P) {
@@ -728,6 +740,7 @@ template <typename F> int run_program(Ctx *c, int key, F &&body) {
 // graph keys: program id * 4 + swap parities of the buffers it captured
 enum Prog { PG_CYCLE_METRIC = 0, PG_CYCLE_NORM = 1, PG_MIXED = 2, PG_ADJ_CYCLE_NORM = 3, PG_ADJ_MIXED = 4 };
 int gkey(int prog, int par, int rpar, int m = 1) { return ((prog * 16 + m) * 2 + par) * 2 + rpar; }
+constexpr int kPubKey = 1 << 20;  // the solve loop's programs end by publishing their result
 
 int sync_read(Ctx *c, int nslots) {
   HIPCHK(c, hipMemcpyAsync(c->hscal, c->dscal, sizeof(double) * nslots, hipMemcpyDeviceToHost, c->st));
@@ -736,13 +749,12 @@ int sync_read(Ctx *c, int nslots) {
   return VLP2D_OK;
 }
 
-// dscal[0..1] -> hscal[0..1] through k_publish: the host spins on the
-// sequence number instead of synchronising the stream (used between the
-// cycles of a solve; profiling runs keep sync_read).
+// dscal[0..1] -> hscal[0..1] as the last kernel of a solve-loop program
+// (k_finish with pub) published them: the host spins on the sequence number
+// instead of synchronising the stream (profiling runs, which publish
+// nothing, keep sync_read).
 int publish_wait(Ctx *c) {
   if (c->profile) return sync_read(c, 2);
-  hipLaunchKernelGGL(k_publish, dim3(1), dim3(64), 0, c->st, c->dscal, c->dseq, c->hpub_dev);
-  HIPCHK(c, hipGetLastError());
   const unsigned long long want = ++c->pub_seq;
   for (unsigned long long spin = 1;; ++spin) {
     if (__atomic_load_n(&c->hpub->seq, __ATOMIC_ACQUIRE) >= want) break;
@@ -934,7 +946,7 @@ int solve_impl(Ctx *c, Hier<double> &H, int prog_fp64, int prog_mixed, double rt
   HIPCHK(c, hipEventCreate(&e0));
   HIPCHK(c, hipEventCreate(&e1));
   HIPCHK(c, hipEventRecord(e0, c->st));
-  resnorm(c, P, P.u, 1, mixed ? c->h32.lv[0].rhs : nullptr);
+  resnorm(c, P, P.u, mixed ? c->h32.lv[0].rhs : nullptr, true);
   HIPCHK(c, hipGetLastError());
   RCHK(publish_wait(c));
   const double r0 = std::sqrt(c->hscal[1]);
@@ -954,12 +966,12 @@ int solve_impl(Ctx *c, Hier<double> &H, int prog_fp64, int prog_mixed, double rt
     }
     if (mixed) {
       const int m = k == 0 ? c->o.max_inner : inner_cycles(rel, rel_prev, m_prev, c->o.max_inner, rtol);
-      RCHK(run_program(c, gkey(prog_mixed, P.par, c->h32.lv[0].rpar, m), [&] { mixed_step(c, P, m); }));
+      RCHK(run_program(c, gkey(prog_mixed, P.par, c->h32.lv[0].rpar, m) + kPubKey, [&] { mixed_step(c, P, m, true); }));
       swap_mixed(c, P);
       vcycles += m;
       m_prev = m;
     } else {
-      RCHK(run_program(c, gkey(prog_fp64, P.par, 0), [&] { cycle64(c, H, EPI_RESNORM); }));
+      RCHK(run_program(c, gkey(prog_fp64, P.par, 0) + kPubKey, [&] { cycle64(c, H, EPI_RESNORM, true); }));
       ++vcycles;
     }
     RCHK(publish_wait(c));
@@ -1297,7 +1309,7 @@ int vlp2d_mg_cycle(vlp2d_mg_handle c, double *stop_metric) {
   if (c->o.precision == VLP2D_PREC_MIXED) {
     RCHK(ensure_h32(c));
     if (!c->r32_valid) {
-      resnorm(c, P, P.u, 1, c->h32.lv[0].rhs);
+      resnorm(c, P, P.u, c->h32.lv[0].rhs);
       HIPCHK(c, hipGetLastError());
       c->r32_valid = true;
     }
@@ -1318,7 +1330,7 @@ int vlp2d_mg_residual_norm(vlp2d_mg_handle c, double *norm) {
   if (!c || !norm) return VLP2D_E_ARG;
   HIPCHK(c, hipSetDevice(c->dev));
   Lvl<double> &P = c->h64.lv[0];
-  resnorm(c, P, P.u, 1, nullptr);
+  resnorm(c, P, P.u, nullptr);
   HIPCHK(c, hipGetLastError());
   RCHK(sync_read(c, 2));
   *norm = std::sqrt(c->hscal[1]);

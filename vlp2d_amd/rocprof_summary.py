@@ -37,6 +37,7 @@ KERNEL_CLASS = {
     "k_update_oop": "update_mixed",
     "k_resnorm64": "resnorm64",
     "k_reduce": "reduce",
+    "k_finish": "reduce",
     "k_rb_half": "rb_half",
     "k_residual": "residual",
     "k_restrict": "restrict",
