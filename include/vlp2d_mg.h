@@ -58,6 +58,9 @@ extern "C" {
 #define VLP2D_FLAG_NO_CYCLE_FUSION 128u /* mixed two-cycle steps: level 0 of
                                      the two cycles in separate launches
                                      (default: one launch between them)     */
+#define VLP2D_FLAG_GRAPH_SOLVE 256u /* vlp2d_mg_solve replays hipGraphs of its
+                                     steps (default: direct launches, one
+                                     step launched ahead)                   */
 #define VLP2D_FLAG_STREAM_SMALL 64u /* levels below level 0 with <= 1025^2
                                      nodes ascend with the register-streaming
                                      kernel (default: the LDS-tiled one)    */

@@ -77,6 +77,7 @@ KERNEL_PATHS = {
     "no_cycle_fusion": {"flags": _lib.FLAG_NO_CYCLE_FUSION},  # two-cycle steps: level 0 of each cycle apart
     "one_op_per_launch": {"flags": _lib.FLAG_SIMPLE},
     "no_graph": {"flags": _lib.FLAG_NO_GRAPH},
+    "graph_solve": {"flags": _lib.FLAG_GRAPH_SOLVE},  # the solve loop replays captured graphs
     "separate_refine": {"flags": _lib.FLAG_SEPARATE_REFINE},
 }
 
@@ -115,7 +116,7 @@ def test_kernel_paths_bitwise(path, seg_x, seg_y, hx, hy, laplace):
         st = s.solve(1e-10, 40)
         assert st["cycles"] == kr, (inner, st["cycles"], kr)
         assert np.array_equal(s.get_array(), ur), (inner, np.abs(s.get_array() - ur).max())
-        # a second solve from the restored state replays the captured graphs
+        # a second solve from the restored state (replaying the captured graphs where the path has them)
         s.save_iterate()
         s.restore_iterate()
         s.un0[:] = g
@@ -160,6 +161,43 @@ def test_recomputed_presmoothing_bitwise(seg_x, seg_y, hx, hy, laplace, mixed, n
             assert st["cycles"] == kr
             assert np.array_equal(s.get_array(), ur), (flags, inner, np.abs(s.get_array() - ur).max())
             s.cleanup()
+
+
+@pytest.mark.parametrize("rtol", [1e-5, 1e-7, 1e-10])
+def test_speculative_steps_match_sequential_schedule(rtol):
+    """Mixed solves launch the next refinement step before the current one's
+    result is known.  With rtol 1e-5 the step launched ahead (two FP32
+    cycles) turns out to need one, and is dropped and relaunched; the steps
+    accepted and the iterate equal the oracle's sequential schedule bit for
+    bit at every tolerance, and equal the profiled (sequential) GPU loop and
+    the loop that replays captured graphs."""
+    p = problems.config(3, 256)
+    _say(f"speculation rtol={rtol}")
+    u0 = p.initial("zero")
+    ur, kr, hr = mixed_solve(p.seg_x, p.seg_y, p.hx, p.hy, p.f, u0, 1, 3, 3, rtol, 60, inner=2)
+    for flags in (0, _lib.FLAG_PROFILE, _lib.FLAG_GRAPH_SOLVE):
+        s = MultiGridSchemeHIP(p.seg_x, p.seg_y, p.hx, p.hy, p.f,
+                               options=Options.performance(True, nu1=1, nu2=3, max_inner=2, flags=flags))
+        s.un0[:] = u0
+        s.init_after_boundaries_and_initial_iteration_inited()
+        st = s.solve(rtol, 60)
+        assert st["cycles"] == kr and st["rel_residual"] <= rtol, (flags, st, kr)
+        assert np.array_equal(s.get_array(), ur), (flags, np.abs(s.get_array() - ur).max())
+        s.cleanup()
+
+
+def test_speculative_step_dropped_when_already_solved():
+    """f = 0 and g = 0 from a zero guess: the initial residual is 0, the
+    step launched ahead is dropped, no step is counted and the iterate stays
+    exactly zero."""
+    z = np.zeros((65, 65))
+    s = MultiGridSchemeHIP(64, 64, 1 / 64, 1 / 64, z, options=Options.performance(True, nu1=1, nu2=3, max_inner=2))
+    s.un0[:] = z
+    s.init_after_boundaries_and_initial_iteration_inited()
+    st = s.solve(1e-10, 20)
+    assert st["cycles"] == 0 and st["rel_residual"] == 0.0
+    assert np.array_equal(s.get_array(), z) and not np.signbit(s.get_array()).any()
+    s.cleanup()
 
 
 def test_config1_reference_stop_rule_cycle_count():

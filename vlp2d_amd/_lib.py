@@ -21,6 +21,7 @@ FLAG_NO_GRAPH, FLAG_PROFILE, FLAG_SIMPLE, FLAG_SEPARATE_REFINE, FLAG_COARSE_L2 =
 FLAG_STORE_PRESMOOTH = 32
 FLAG_STREAM_SMALL = 64
 FLAG_NO_CYCLE_FUSION = 128
+FLAG_GRAPH_SOLVE = 256
 
 # Every entry point declared in include/vlp2d_mg.h.
 EXPORTS = [

@@ -69,24 +69,26 @@ __global__ void __launch_bounds__(RED_THREADS) k_reduce(const double *__restrict
 // sequence number the host polls for.  The host thus learns a cycle's result
 // without a device-to-host copy and a stream synchronisation.
 struct HostPub {
-  double v[2];
+  double v[3];  // dscal[0..2]
   unsigned long long seq;
 };
+constexpr int kPubRing = 4;  // result s goes to slot s % kPubRing
 
 // The reductions that end a cycle or a refinement step, in one launch:
-// dscal[1] = sum of part_sum[0..nsum); with MAX also dscal[0] = max of
-// part_max[0..nmax) (NaN-propagating); then, with pub, the two scalars are
-// published to the HostPub.  Same summation order as k_reduce.
+// dscal[sum_slot] = sum of part_sum[0..nsum); with MAX also dscal[0] = max of
+// part_max[0..nmax) (NaN-propagating); then, with pub, dscal[0..2] are
+// published as result s = dseq + 1 into ring slot s % kPubRing.  Same
+// summation order as k_reduce.
 template <bool MAX>
 __global__ void __launch_bounds__(RED_THREADS) k_finish(const double *__restrict__ part_max, int nmax,
                                                         const double *__restrict__ part_sum, int nsum,
-                                                        double *__restrict__ dscal, unsigned long long *dseq,
-                                                        HostPub *pub) {
+                                                        double *__restrict__ dscal, int sum_slot,
+                                                        unsigned long long *dseq, HostPub *pub) {
   __shared__ double sm[RED_THREADS / 64];
   double v = 0.0;
   for (int k = threadIdx.x; k < nsum; k += RED_THREADS) v = v + part_sum[k];
   v = block_reduce<false>(v, sm, threadIdx.x, RED_THREADS);
-  if (threadIdx.x == 0) dscal[1] = v;
+  if (threadIdx.x == 0) dscal[sum_slot] = v;
   if constexpr (MAX) {
     double w = 0.0;
     for (int k = threadIdx.x; k < nmax; k += RED_THREADS) w = nanmax(w, part_max[k]);
@@ -96,10 +98,12 @@ __global__ void __launch_bounds__(RED_THREADS) k_finish(const double *__restrict
   if (pub && threadIdx.x == 0) {
     const unsigned long long s = dseq[0] + 1;
     dseq[0] = s;
-    pub->v[0] = dscal[0];
-    pub->v[1] = dscal[1];
+    HostPub *slot = pub + (s % kPubRing);
+    slot->v[0] = dscal[0];
+    slot->v[1] = dscal[1];
+    slot->v[2] = dscal[2];
     __threadfence_system();
-    __hip_atomic_store(&pub->seq, s, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    __hip_atomic_store(&slot->seq, s, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
   }
 }
 

@@ -119,7 +119,7 @@ struct vlp2d_mg_ctx {
   int part_cap = 0;
   double *dscal = nullptr;  // device scalars
   double *hscal = nullptr;  // pinned host scalars
-  HostPub *hpub = nullptr;  // mapped pinned results the solve loop polls
+  HostPub *hpub = nullptr;  // mapped pinned ring of kPubRing results the solve loop polls
   HostPub *hpub_dev = nullptr;
   unsigned long long *dseq = nullptr;  // device count of published results
   unsigned long long pub_seq = 0;      // results published so far
@@ -550,21 +550,21 @@ int vcycle_fused(Ctx *c, Hier<T> &H, bool zero0, int epi, const MixView &mx, dou
 
 // dscal[1] = sum of psum[0..nsum); with pmax also dscal[0] = max of
 // pmax[0..nmax); pub: then publish both for the solve loop (publish_wait).
-void finish(Ctx *c, const double *pmax, int nmax, const double *psum, int nsum, bool pub) {
+void finish(Ctx *c, const double *pmax, int nmax, const double *psum, int nsum, bool pub, int sum_slot = 1) {
   HostPub *hp = pub && !c->profile ? c->hpub_dev : nullptr;
   launch(c, KC_REDUCE, 0, 8.0 * (nmax + nsum), [&] {
     if (pmax)
       hipLaunchKernelGGL(k_finish<true>, dim3(1), dim3(RED_THREADS), 0, c->st, pmax, nmax, psum, nsum, c->dscal,
-                         c->dseq, hp);
+                         sum_slot, c->dseq, hp);
     else
       hipLaunchKernelGGL(k_finish<false>, dim3(1), dim3(RED_THREADS), 0, c->st, pmax, nmax, psum, nsum, c->dscal,
-                         c->dseq, hp);
+                         sum_slot, c->dseq, hp);
   });
 }
 
-// residual norm^2 of level 0 (FP64) -> dscal[1]; optional FP32 copy; pub:
-// published for the solve loop
-void resnorm(Ctx *c, Lvl<double> &v, const double *u, float *r32, bool pub = false) {
+// residual norm^2 of level 0 (FP64) -> dscal[slot] (1 unless given);
+// optional FP32 copy; pub: published for the solve loop
+void resnorm(Ctx *c, Lvl<double> &v, const double *u, float *r32, bool pub = false, int slot = 1) {
   // column-streaming tiles of the refinement kernel without the update
   const int rows = refine_row_blocks(v.nx);
   const int np = refine_strips(v.ny) * rows;
@@ -574,7 +574,7 @@ void resnorm(Ctx *c, Lvl<double> &v, const double *u, float *r32, bool pub = fal
                        v.nx, v.ny, v.pitch, (double)v.hx2, (double)v.hy2, (double)v.ihx2, (double)v.ihy2,
                        v.pw2 ? 1 : 0, rows, np, chunk, c->part, nullptr);
   });
-  finish(c, nullptr, 0, c->part, np, pub);
+  finish(c, nullptr, 0, c->part, np, pub, slot);
 }
 
 void reduce_part(Ctx *c, const double *part, int np, int slot, bool mx) {
@@ -715,8 +715,14 @@ void swap_mixed(Ctx *c, Lvl<double> &P) {
 }
 
 // ---- graph cache ---------------------------------------------------------
+constexpr int kPubKey = 1 << 20;  // keys of the solve loop's programs, which end by publishing their result
+// Programs of the solve loop (key >= kPubKey) launch their kernels directly
+// unless VLP2D_FLAG_GRAPH_SOLVE: the loop keeps one step launched ahead, and
+// graph replay measured slower there (a graph's launch stalls the queue in
+// the middle of a step); single cycles replay their captured graph.
 template <typename F> int run_program(Ctx *c, int key, F &&body) {
   bool use_graph = !(c->o.flags & VLP2D_FLAG_NO_GRAPH) && !c->profile;
+  if (key >= kPubKey && !(c->o.flags & VLP2D_FLAG_GRAPH_SOLVE)) use_graph = false;
   if (!use_graph) {
     body();
     HIPCHK(c, hipGetLastError());
@@ -740,7 +746,6 @@ template <typename F> int run_program(Ctx *c, int key, F &&body) {
 // graph keys: program id * 4 + swap parities of the buffers it captured
 enum Prog { PG_CYCLE_METRIC = 0, PG_CYCLE_NORM = 1, PG_MIXED = 2, PG_ADJ_CYCLE_NORM = 3, PG_ADJ_MIXED = 4 };
 int gkey(int prog, int par, int rpar, int m = 1) { return ((prog * 16 + m) * 2 + par) * 2 + rpar; }
-constexpr int kPubKey = 1 << 20;  // the solve loop's programs end by publishing their result
 
 int sync_read(Ctx *c, int nslots) {
   HIPCHK(c, hipMemcpyAsync(c->hscal, c->dscal, sizeof(double) * nslots, hipMemcpyDeviceToHost, c->st));
@@ -753,25 +758,33 @@ int sync_read(Ctx *c, int nslots) {
 // (k_finish with pub) published them: the host spins on the sequence number
 // instead of synchronising the stream (profiling runs, which publish
 // nothing, keep sync_read).
-int publish_wait(Ctx *c) {
-  if (c->profile) return sync_read(c, 2);
-  const unsigned long long want = ++c->pub_seq;
+// Wait for published result `want` and copy its dscal[0..2] into hscal.  At
+// most two results are in flight, so ring slot want % kPubRing holds result
+// want or an older one.
+int wait_result(Ctx *c, unsigned long long want) {
+  HostPub *slot = c->hpub + (want % kPubRing);
   for (unsigned long long spin = 1;; ++spin) {
-    if (__atomic_load_n(&c->hpub->seq, __ATOMIC_ACQUIRE) >= want) break;
+    if (__atomic_load_n(&slot->seq, __ATOMIC_ACQUIRE) >= want) break;
     if ((spin & 4095) == 0) {
       // the stream drained without the result: report, do not spin forever
       const hipError_t q = hipStreamQuery(c->st);
       if (q == hipSuccess) {
-        if (__atomic_load_n(&c->hpub->seq, __ATOMIC_ACQUIRE) >= want) break;
+        if (__atomic_load_n(&slot->seq, __ATOMIC_ACQUIRE) >= want) break;
         return fail(c, VLP2D_E_DEVICE, "published cycle result did not arrive");
       }
       if (q != hipErrorNotReady) HIPCHK(c, q);
     }
     __builtin_ia32_pause();
   }
-  c->hscal[0] = c->hpub->v[0];
-  c->hscal[1] = c->hpub->v[1];
+  for (int k = 0; k < 3; ++k) c->hscal[k] = slot->v[k];
   return VLP2D_OK;
+}
+
+// The result of the program just launched (profiling runs, which publish
+// nothing, synchronise instead).
+int publish_wait(Ctx *c) {
+  if (c->profile) return sync_read(c, 3);
+  return wait_result(c, ++c->pub_seq);
 }
 
 // ---- hierarchy construction ------------------------------------------------
@@ -935,6 +948,100 @@ int inner_cycles(double rel, double rel_prev, int m_prev, int max_inner, double 
   return m;
 }
 
+// Whether the refinement step after the one in flight is certain to be
+// needed and to run max_inner cycles, so that it may be launched before the
+// result of the one in flight is known: the predicted residual after the one
+// in flight, times rho^(max_inner - 1), stays 100 x above rtol.  rho is the
+// contraction per cycle of the last accepted step (0.2 before any).
+bool speculate_next(double rel, double rel_prev, int m_prev, bool have_rate, int m_flight, int max_inner,
+                    double rtol) {
+  const double rho = have_rate ? std::pow(rel / rel_prev, 1.0 / m_prev) : 0.2;
+  if (!(rho < 1.0)) return true;
+  return rel * std::pow(rho, m_flight + max_inner - 1) > 100.0 * rtol;
+}
+
+// Mixed solve: refinement steps are launched one ahead of the host while the
+// next one is certain to be needed (speculate_next), so the host's wait for a
+// step's result overlaps the next step.  A step writes the iterate and the
+// FP32 residual into the other buffer of each pair, so a launched step that
+// turns out not to be needed (rtol reached) or to run the wrong number of
+// cycles (the schedule of inner_cycles decides from the actual residual) is
+// dropped by restoring the buffer parity; the accepted steps are exactly the
+// ones the sequential loop runs.
+int solve_mixed(Ctx *c, Lvl<double> &P, int prog_mixed, double rtol, int max_cycles, int &k, int &vcycles,
+                double &r0, double &rel) {
+  const int M = c->o.max_inner;
+  resnorm(c, P, P.u, c->h32.lv[0].rhs, false, 2);  // ||r0||^2 -> dscal[2], published with step 0
+  HIPCHK(c, hipGetLastError());
+  struct Step {
+    int m;
+    unsigned long long seq;
+  };
+  Step fl[2] = {};
+  int nfl = 0;
+  auto launch_step = [&](int m) -> int {
+    RCHK(run_program(c, gkey(prog_mixed, P.par, c->h32.lv[0].rpar, m) + kPubKey, [&] { mixed_step(c, P, m, true); }));
+    swap_mixed(c, P);
+    fl[nfl++] = Step{m, ++c->pub_seq};
+    return VLP2D_OK;
+  };
+  auto drop = [&](int n) {
+    for (int q = 0; q < n; ++q) swap_mixed(c, P);
+  };
+  k = 0;
+  vcycles = 0;
+  rel = 1.0;
+  r0 = NAN;
+  double rel_prev = 1.0;
+  int m_prev = 1;
+  for (;;) {
+    if (c->cancel.load()) {
+      drop(nfl);
+      return fail(c, VLP2D_E_CANCELLED, "cancelled");
+    }
+    if (nfl == 0) {
+      if (k >= max_cycles) return fail(c, VLP2D_E_NOTCONV, "max_cycles reached before rtol");
+      RCHK(launch_step(k == 0 ? M : inner_cycles(rel, rel_prev, m_prev, M, rtol)));
+    }
+    if (nfl == 1 && k + 1 < max_cycles && speculate_next(rel, rel_prev, m_prev, k > 0, fl[0].m, M, rtol))
+      RCHK(launch_step(M));
+    RCHK(wait_result(c, fl[0].seq));
+    const Step s0 = fl[0];
+    fl[0] = fl[1];
+    --nfl;
+    if (k == 0) {
+      r0 = std::sqrt(c->hscal[2]);
+      if (!std::isfinite(r0)) {
+        drop(1 + nfl);
+        return fail(c, VLP2D_E_NONFINITE, "initial residual is not finite");
+      }
+      if (!(r0 > 0)) {  // already the discrete solution: no step
+        drop(1 + nfl);
+        rel = 0.0;
+        return VLP2D_OK;
+      }
+    }
+    if (s0.m != (k == 0 ? M : inner_cycles(rel, rel_prev, m_prev, M, rtol))) {
+      drop(1 + nfl);  // launched with the wrong cycle count: relaunch
+      nfl = 0;
+      continue;
+    }
+    ++k;
+    vcycles += s0.m;
+    m_prev = s0.m;
+    rel_prev = rel;
+    rel = std::sqrt(c->hscal[1]) / r0;
+    if (!std::isfinite(rel)) {
+      drop(nfl);
+      return fail(c, VLP2D_E_NONFINITE, "residual became non-finite");
+    }
+    if (rel <= rtol) {
+      drop(nfl);
+      return VLP2D_OK;
+    }
+  }
+}
+
 // Solve from the current level-0 iterate of H until ||r|| <= rtol ||r0||.
 int solve_impl(Ctx *c, Hier<double> &H, int prog_fp64, int prog_mixed, double rtol, int max_cycles,
                vlp2d_mg_stats *st) {
@@ -946,44 +1053,50 @@ int solve_impl(Ctx *c, Hier<double> &H, int prog_fp64, int prog_mixed, double rt
   HIPCHK(c, hipEventCreate(&e0));
   HIPCHK(c, hipEventCreate(&e1));
   HIPCHK(c, hipEventRecord(e0, c->st));
-  resnorm(c, P, P.u, mixed ? c->h32.lv[0].rhs : nullptr, true);
-  HIPCHK(c, hipGetLastError());
-  RCHK(publish_wait(c));
-  const double r0 = std::sqrt(c->hscal[1]);
-  c->r32_valid = mixed && (&H == &c->h64);
-  if (!std::isfinite(r0)) return fail(c, VLP2D_E_NONFINITE, "initial residual is not finite");
-  double rel = r0 > 0 ? 1.0 : 0.0, rel_prev = 1.0;
-  int k = 0, vcycles = 0, m_prev = 1;
+  double r0 = 0.0, rel = 0.0;
+  int k = 0, vcycles = 0;
   int rc = VLP2D_OK;
-  while (r0 > 0 && rel > rtol) {
-    if (k >= max_cycles) {
-      rc = VLP2D_E_NOTCONV;
-      break;
-    }
-    if (c->cancel.load()) {
-      rc = VLP2D_E_CANCELLED;
-      break;
-    }
-    if (mixed) {
-      const int m = k == 0 ? c->o.max_inner : inner_cycles(rel, rel_prev, m_prev, c->o.max_inner, rtol);
-      RCHK(run_program(c, gkey(prog_mixed, P.par, c->h32.lv[0].rpar, m) + kPubKey, [&] { mixed_step(c, P, m, true); }));
-      swap_mixed(c, P);
-      vcycles += m;
-      m_prev = m;
-    } else {
-      RCHK(run_program(c, gkey(prog_fp64, P.par, 0) + kPubKey, [&] { cycle64(c, H, EPI_RESNORM, true); }));
-      ++vcycles;
-    }
+  if (mixed && !c->profile && max_cycles >= 1) {
+    c->r32_valid = &H == &c->h64;
+    rc = solve_mixed(c, P, prog_mixed, rtol, max_cycles, k, vcycles, r0, rel);
+  } else {
+    resnorm(c, P, P.u, mixed ? c->h32.lv[0].rhs : nullptr, true);
+    HIPCHK(c, hipGetLastError());
     RCHK(publish_wait(c));
-    ++k;
-    rel_prev = rel;
-    rel = std::sqrt(c->hscal[1]) / r0;
-    if (!std::isfinite(rel)) {
-      rc = VLP2D_E_NONFINITE;
-      c->err = "residual became non-finite";
-      break;
+    r0 = std::sqrt(c->hscal[1]);
+    c->r32_valid = mixed && (&H == &c->h64);
+    if (!std::isfinite(r0)) rc = fail(c, VLP2D_E_NONFINITE, "initial residual is not finite");
+    rel = r0 > 0 ? 1.0 : 0.0;
+    double rel_prev = 1.0;
+    int m_prev = 1;
+    while (rc == VLP2D_OK && r0 > 0 && rel > rtol) {
+      if (k >= max_cycles) {
+        rc = fail(c, VLP2D_E_NOTCONV, "max_cycles reached before rtol");
+        break;
+      }
+      if (c->cancel.load()) {
+        rc = fail(c, VLP2D_E_CANCELLED, "cancelled");
+        break;
+      }
+      if (mixed) {
+        const int m = k == 0 ? c->o.max_inner : inner_cycles(rel, rel_prev, m_prev, c->o.max_inner, rtol);
+        RCHK(run_program(c, gkey(prog_mixed, P.par, c->h32.lv[0].rpar, m) + kPubKey,
+                         [&] { mixed_step(c, P, m, true); }));
+        swap_mixed(c, P);
+        vcycles += m;
+        m_prev = m;
+      } else {
+        RCHK(run_program(c, gkey(prog_fp64, P.par, 0) + kPubKey, [&] { cycle64(c, H, EPI_RESNORM, true); }));
+        ++vcycles;
+      }
+      RCHK(publish_wait(c));
+      ++k;
+      rel_prev = rel;
+      rel = std::sqrt(c->hscal[1]) / r0;
+      if (!std::isfinite(rel)) rc = fail(c, VLP2D_E_NONFINITE, "residual became non-finite");
     }
   }
+  if (rc != VLP2D_OK && rc != VLP2D_E_NOTCONV && rc != VLP2D_E_CANCELLED && rc != VLP2D_E_NONFINITE) return rc;
   HIPCHK(c, hipEventRecord(e1, c->st));
   HIPCHK(c, hipEventSynchronize(e1));
   float ms = 0.f;
@@ -1000,8 +1113,6 @@ int solve_impl(Ctx *c, Hier<double> &H, int prog_fp64, int prog_mixed, double rt
     double unknowns = (double)(c->nx - 2) * (double)(c->ny - 2);
     st->munknowns_per_s = st->seconds > 0 ? unknowns / st->seconds / 1e6 : 0.0;
   }
-  if (rc == VLP2D_E_NOTCONV) c->err = "max_cycles reached before rtol";
-  if (rc == VLP2D_E_CANCELLED) c->err = "cancelled";
   return rc;
 }
 
@@ -1253,12 +1364,13 @@ static int create_impl(int segX, int segY, double hx, double hy, const double *f
     c->err = "hipHostMalloc failed";
     return bail(VLP2D_E_OOM);
   }
-  if (hipHostMalloc((void **)&c->hpub, sizeof(HostPub), hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess ||
+  if (hipHostMalloc((void **)&c->hpub, kPubRing * sizeof(HostPub), hipHostMallocMapped | hipHostMallocCoherent) !=
+          hipSuccess ||
       hipHostGetDevicePointer((void **)&c->hpub_dev, c->hpub, 0) != hipSuccess) {
     c->err = "hipHostMalloc (mapped result buffer) failed";
     return bail(VLP2D_E_OOM);
   }
-  std::memset(c->hpub, 0, sizeof(HostPub));
+  std::memset(c->hpub, 0, kPubRing * sizeof(HostPub));
   if ((rc = dalloc(c, &c->dseq, 1)) != VLP2D_OK) return bail(rc);
   if (hipMemsetAsync(c->dseq, 0, sizeof(unsigned long long), c->st) != hipSuccess) {
     c->err = "hipMemsetAsync failed";
