@@ -44,6 +44,11 @@ extern "C" {
 #define VLP2D_PREC_FP64 0  /* plain FP64 multigrid                            */
 #define VLP2D_PREC_MIXED 1 /* FP32 V-cycles inside FP64 iterative refinement  */
 
+#define VLP2D_SMOOTH_RBGS 0    /* red-black Gauss-Seidel, red first (the reference's) */
+#define VLP2D_SMOOTH_WJACOBI 1 /* weighted Jacobi u += omega (g - u), g the same
+                                  point formula; no counterpart in the reference;
+                                  runs one operator per launch                  */
+
 /* Option flags (vlp2d_mg_opts.flags). */
 #define VLP2D_FLAG_NO_GRAPH 1u   /* launch kernels directly, no hipGraph replay   */
 #define VLP2D_FLAG_PROFILE 2u    /* time every kernel class with hipEvents        */
@@ -80,6 +85,8 @@ typedef struct {
                       runs the fewest cycles that the contraction per cycle
                       measured on step k-1 predicts reach rtol, capped at
                       max_inner (step 0 runs max_inner).                    */
+  int smoother;    /* VLP2D_SMOOTH_* (default RBGS)                         */
+  double omega;    /* weighted-Jacobi weight in (0, 1] (default 0.8)        */
 } vlp2d_mg_opts;
 
 typedef struct {

@@ -166,6 +166,11 @@ template <class T> struct Level {
 template <class T> struct MG {
   int L = 0;
   std::vector<Level<T>> lv;
+  // smoother: 0 red-black Gauss-Seidel (the reference's), 1 weighted Jacobi
+  // with weight omega (an addition: the reference has no Jacobi smoother)
+  int smoother = 0;
+  double omega = 0.8;
+  std::vector<T> tmp;  // Jacobi: the iterate before the sweep
 
   // fn: scheme right-hand side (= -f) at all nodes, or nullptr for Laplace.
   MG(int segX, int segY, T hx, T hy, const T *fn) {
@@ -227,8 +232,39 @@ template <class T> struct MG {
       }
     }
   }
+  // --- weighted Jacobi sweep: every interior node from the previous
+  //     iterate, u = u + omega (g - u), g the same point formula as the
+  //     Gauss-Seidel update (MultiGridSlidingIteration.cs:53-57)
+  void jacobi_sweep(int l) {
+    Level<T> &v = lv[l];
+    const int nx = v.nx, ny = v.ny;
+    tmp.assign(v.u.begin(), v.u.end());
+    const T *o = tmp.data();
+    T *u = v.u.data();
+    const T *f = v.rhs.data();
+    const T q = Num<T>::from(0.25), w = Num<T>::from(omega);
+#pragma omp parallel for schedule(static) if (v.par)
+    for (int i = 1; i < nx - 1; ++i)
+      for (int j = 1; j < ny - 1; ++j) {
+        size_t c = (size_t)i * ny + j;
+        T uim = o[c - ny], uip = o[c + ny], ujm = o[c - 1], ujp = o[c + 1];
+        T g;
+        if (v.eq) {
+          if (v.lap) g = (uim + uip + ujm + ujp) * q;
+          else g = (uim + uip + ujm + ujp - f[c] * v.hx2) * q;
+        } else {
+          if (v.lap) g = v.coef * ((uim + uip) / v.hx2 + (ujm + ujp) / v.hy2);
+          else g = v.coef * ((uim + uip) / v.hx2 + (ujm + ujp) / v.hy2 - f[c]);
+        }
+        u[c] = o[c] + w * (g - o[c]);
+      }
+  }
   void smooth(int l, int n) {
     for (int s = 0; s < n; ++s) {
+      if (smoother == 1) {
+        jacobi_sweep(l);
+        continue;
+      }
       half_sweep(l, 0);
       half_sweep(l, 1);
     }
@@ -485,6 +521,17 @@ void *orc_create(int prec, int segX, int segY, double hx, double hy, const doubl
   return h;
 }
 
+void orc_set_smoother(void *p, int kind, double omega) {
+  OrcHandle *h = (OrcHandle *)p;
+  auto set = [&](auto *mg) {
+    mg->smoother = kind;
+    mg->omega = omega;
+  };
+  if (h->prec == 0) set((MG<double> *)h->mg);
+  else if (h->prec == 1) set((MG<float> *)h->mg);
+  else set((MG<DD> *)h->mg);
+}
+
 void orc_destroy(void *p) {
   OrcHandle *h = (OrcHandle *)p;
   if (!h) return;
@@ -573,7 +620,8 @@ int orc_plain_solve(void *p, int nu1, int nu2, int nuc, double rtol, int max_cyc
 // Stops when ||r||_2 / ||r_0||_2 <= rtol. u (all nodes, FP64) is updated in
 // place. hist[k] = relative residual after k outer steps. Returns steps run.
 int orc_mixed_solve(int segX, int segY, double hx, double hy, const double *f, double *u, int nu1, int nu2,
-                    int nuc, double rtol, int max_outer, double *hist, int nthreads, int inner) {
+                    int nuc, double rtol, int max_outer, double *hist, int nthreads, int inner, int smoother,
+                    double omega) {
   set_threads(nthreads);
   void *p64 = orc_create(0, segX, segY, hx, hy, f, nthreads);
   MG<double> &d = *(MG<double> *)((OrcHandle *)p64)->mg;
@@ -582,6 +630,8 @@ int orc_mixed_solve(int segX, int segY, double hx, double hy, const double *f, d
   const size_t n = (size_t)nx * ny;
   std::vector<float> dummy(n, 0.0f);
   MG<float> s(segX, segY, (float)hx, (float)hy, dummy.data());  // level 0 in Poisson form
+  s.smoother = smoother;
+  s.omega = omega;
   std::vector<double> r(n, 0.0);
   auto resid = [&]() {
     std::fill(r.begin(), r.end(), 0.0);
@@ -695,8 +745,10 @@ void orc_op_residual(int segX, int segY, double hx, double hy, const double *f, 
   orc_destroy(p);
 }
 // n red-black sweeps in place on u (level 0 formulas; f NULL -> Laplace).
-void orc_op_smooth(int segX, int segY, double hx, double hy, const double *f, double *u, int n) {
+void orc_op_smooth(int segX, int segY, double hx, double hy, const double *f, double *u, int n, int smoother,
+                   double omega) {
   void *p = orc_create(0, segX, segY, hx, hy, f, 1);
+  orc_set_smoother(p, smoother, omega);
   MG<double> &mg = *(MG<double> *)((OrcHandle *)p)->mg;
   std::copy(u, u + mg.lv[0].u.size(), mg.lv[0].u.begin());
   mg.smooth(0, n);

@@ -48,7 +48,9 @@ def load():
     lib.orc_mixed_solve.restype = ctypes.c_int
     lib.orc_mixed_solve.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp, _dp,
                                     ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, _dp,
-                                    ctypes.c_int, ctypes.c_int]
+                                    ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double]
+    lib.orc_set_smoother.restype = None
+    lib.orc_set_smoother.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double]
     lib.orc_forward_cycles.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp, _dp,
                                        ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
     lib.orc_adjoint_cycles.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
@@ -58,7 +60,7 @@ def load():
     lib.orc_op_prolong.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp]
     lib.orc_op_residual.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp, _dp, _dp]
     lib.orc_op_smooth.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp, _dp,
-                                  ctypes.c_int]
+                                  ctypes.c_int, ctypes.c_int, ctypes.c_double]
     lib.orc_dd_op.argtypes = [ctypes.c_int, _dp, _dp, _dp]
     _lib = lib
     return lib
@@ -95,6 +97,11 @@ class Oracle:
         self.lib.orc_get_u(self.h, _ptr(hi), _ptr(lo))
         return (hi, lo) if with_lo else hi
 
+    def set_smoother(self, kind, omega=0.8):
+        """kind 0: red-black Gauss-Seidel (the reference's); 1: weighted
+        Jacobi u += omega (g - u) with the same point formula g."""
+        self.lib.orc_set_smoother(self.h, int(kind), float(omega))
+
     def vcycle(self, nu1=3, nu2=3, nuc=3):
         return self.lib.orc_vcycle(self.h, nu1, nu2, nuc, self.nthreads)
 
@@ -118,8 +125,10 @@ class Oracle:
             pass
 
 
-def plain_solve(seg_x, seg_y, hx, hy, f, u, nu1=2, nu2=2, nuc=3, rtol=1e-10, max_cycles=100, nthreads=0):
+def plain_solve(seg_x, seg_y, hx, hy, f, u, nu1=2, nu2=2, nuc=3, rtol=1e-10, max_cycles=100, nthreads=0,
+                smoother=0, omega=0.8):
     o = Oracle(seg_x, seg_y, hx, hy, f, "double", nthreads)
+    o.set_smoother(smoother, omega)
     o.set_u(u)
     k, hist = o.plain_solve(nu1, nu2, nuc, rtol, max_cycles)
     out = o.get_u()
@@ -127,13 +136,14 @@ def plain_solve(seg_x, seg_y, hx, hy, f, u, nu1=2, nu2=2, nuc=3, rtol=1e-10, max
     return out, k, hist
 
 
-def mixed_solve(seg_x, seg_y, hx, hy, f, u, nu1=2, nu2=2, nuc=3, rtol=1e-10, max_outer=100, nthreads=0, inner=1):
+def mixed_solve(seg_x, seg_y, hx, hy, f, u, nu1=2, nu2=2, nuc=3, rtol=1e-10, max_outer=100, nthreads=0, inner=1,
+                smoother=0, omega=0.8):
     lib = load()
     u = np.array(u, dtype=np.float64, order="C").reshape(seg_x + 1, seg_y + 1)
     fa = None if f is None else np.ascontiguousarray(f, dtype=np.float64)
     hist = np.zeros(max_outer + 1)
     k = lib.orc_mixed_solve(seg_x, seg_y, hx, hy, _ptr(fa), _ptr(u), nu1, nu2, nuc, rtol, max_outer, _ptr(hist),
-                            nthreads, inner)
+                            nthreads, inner, smoother, omega)
     return u, k, hist[: k + 1]
 
 
@@ -181,11 +191,11 @@ def op_residual(seg_x, seg_y, hx, hy, f, u):
     return out
 
 
-def op_smooth(seg_x, seg_y, hx, hy, f, u, sweeps=1):
+def op_smooth(seg_x, seg_y, hx, hy, f, u, sweeps=1, smoother=0, omega=0.8):
     lib = load()
     fa = None if f is None else np.ascontiguousarray(f, dtype=np.float64)
     ua = np.array(u, dtype=np.float64, order="C")
-    lib.orc_op_smooth(seg_x, seg_y, hx, hy, _ptr(fa), _ptr(ua), sweeps)
+    lib.orc_op_smooth(seg_x, seg_y, hx, hy, _ptr(fa), _ptr(ua), sweeps, smoother, omega)
     return ua
 
 

@@ -32,7 +32,8 @@ def test_default_opts_and_version():
     lib = load_library()
     o = _lib.Opts()
     lib.vlp2d_mg_default_opts(ctypes.byref(o))
-    assert (o.nu1, o.nu2, o.nu_coarse, o.precision, o.flags, o.coarse_nodes, o.max_inner) == (3, 3, 3, 0, 0, 0, 1)
+    assert (o.nu1, o.nu2, o.nu_coarse, o.precision, o.flags, o.coarse_nodes, o.max_inner, o.smoother, o.omega) == \
+        (3, 3, 3, 0, 0, 0, 1, 0, 0.8)
     assert lib.vlp2d_mg_version() >> 16 == 1
 
 
@@ -49,14 +50,18 @@ def test_create_rejects_bad_steps_and_options():
     lib = load_library()
     h = ctypes.c_void_p()
     assert lib.vlp2d_mg_create(8, 8, -1.0, 0.1, None, 1e-13, 0, None, ctypes.byref(h)) == _lib.E_ARG
-    o = _lib.Opts(2, 2, 0, 0, 0, 0, 1)
+    o = _lib.Opts(2, 2, 0, 0, 0, 0, 1, 0, 0.8)
     assert lib.vlp2d_mg_create(8, 8, 0.1, 0.1, None, 1e-13, 0, ctypes.byref(o), ctypes.byref(h)) == _lib.E_ARG
-    o = _lib.Opts(2, 2, 1, 7, 0, 0, 1)
+    o = _lib.Opts(2, 2, 1, 7, 0, 0, 1, 0, 0.8)
     assert lib.vlp2d_mg_create(8, 8, 0.1, 0.1, None, 1e-13, 0, ctypes.byref(o), ctypes.byref(h)) == _lib.E_ARG
     for inner in (0, 9, -1):  # FP32 cycles per refinement step: 1..8
-        o = _lib.Opts(2, 2, 3, 1, 0, 0, inner)
+        o = _lib.Opts(2, 2, 3, 1, 0, 0, inner, 0, 0.8)
         assert lib.vlp2d_mg_create(8, 8, 0.1, 0.1, None, 1e-13, 0, ctypes.byref(o), ctypes.byref(h)) == _lib.E_ARG
         assert b"max_inner" in lib.vlp2d_mg_last_error(None)
+    for sm, w in ((2, 0.8), (1, 0.0), (1, 1.5), (1, float("nan"))):  # smoother 0/1, omega in (0, 1]
+        o = _lib.Opts(2, 2, 3, 0, 0, 0, 1, sm, w)
+        assert lib.vlp2d_mg_create(8, 8, 0.1, 0.1, None, 1e-13, 0, ctypes.byref(o), ctypes.byref(h)) == _lib.E_ARG
+        assert b"omega" in lib.vlp2d_mg_last_error(None)
     assert lib.vlp2d_mg_create(8, 8, 0.1, 0.1, None, 1e-13, 0, None, None) == _lib.E_ARG
 
 

@@ -200,6 +200,44 @@ def test_speculative_step_dropped_when_already_solved():
     s.cleanup()
 
 
+@pytest.mark.parametrize("omega", [0.8, 2 / 3])
+@pytest.mark.parametrize("seg_x,seg_y,hx,hy,laplace", [(64, 32, 1 / 64, 1 / 64, False),
+                                                       (128, 256, np.e / 128, np.pi / 256, False),
+                                                       (128, 128, 1 / 128, 1 / 128, True)])
+def test_weighted_jacobi_bitwise(seg_x, seg_y, hx, hy, laplace, omega):
+    """Weighted-Jacobi smoother (an addition; the reference smooths with
+    red-black Gauss-Seidel only): parity-mode cycles with the update norm,
+    and FP64 and mixed V(2,2) solves to 1e-10, bit for bit the oracle's."""
+    _say(f"jacobi {seg_x}x{seg_y} omega={omega:.3f}")
+    f, g = _random(seg_x, seg_y, 11 + seg_x, laplace)
+    u0 = _mean_init(g)
+    o = Oracle(seg_x, seg_y, hx, hy, f)
+    o.set_smoother(1, omega)
+    o.set_u(u0)
+    s = MultiGridSchemeHIP(seg_x, seg_y, hx, hy, f, eps=1e-13, options=Options(smoother=_lib.SMOOTH_WJACOBI, omega=omega))
+    s.un0[:] = u0
+    s.init_after_boundaries_and_initial_iteration_inited()
+    for k in range(4):
+        m_ref = o.vcycle(3, 3, 3)
+        s.do_iteration(k)
+        assert np.array_equal(s.get_array(), o.get_u()), (k, np.abs(s.get_array() - o.get_u()).max())
+        assert s.last_metric == m_ref
+    s.cleanup()
+    z = np.zeros_like(g) if laplace else f
+    for mixed in (False, True):
+        solve = mixed_solve if mixed else plain_solve
+        # (point Jacobi smooths the anisotropic grid, (hx/hy)^2 = 3, slowly: 109 cycles at omega = 2/3)
+        ur, kr, _ = solve(seg_x, seg_y, hx, hy, z, g, 2, 2, 3, 1e-10, 150, smoother=1, omega=omega)
+        s = MultiGridSchemeHIP(seg_x, seg_y, hx, hy, z,
+                               options=Options.performance(mixed, smoother=_lib.SMOOTH_WJACOBI, omega=omega))
+        s.un0[:] = g
+        s.init_after_boundaries_and_initial_iteration_inited()
+        st = s.solve(1e-10, 150)
+        assert st["cycles"] == kr, (mixed, st["cycles"], kr)
+        assert np.array_equal(s.get_array(), ur), (mixed, np.abs(s.get_array() - ur).max())
+        s.cleanup()
+
+
 def test_config1_reference_stop_rule_cycle_count():
     """Config 1 (257^2 Laplace, g = x^2 - y^2) through the reference loop:
     cycles until the update norm <= eps, same count as the oracle, and the
@@ -370,5 +408,13 @@ def test_errors_are_reported_not_faulted():
     assert s.solve(1e-10, 50)["rel_residual"] <= 1e-10
     with pytest.raises(VLP2DError) as e:
         s.adjoint_cycles(g)
+    assert e.value.code == _lib.E_STATE
+    s.cleanup()
+    # the truncated-cycle adjoint transposes the Gauss-Seidel cycle only
+    s = MultiGridSchemeHIP(16, 16, 1 / 16, 1 / 16, f, options=Options.performance(smoother=_lib.SMOOTH_WJACOBI))
+    s.un0[:] = g
+    s.init_after_boundaries_and_initial_iteration_inited()
+    with pytest.raises(VLP2DError) as e:
+        s.forward_cycles(2)
     assert e.value.code == _lib.E_STATE
     s.cleanup()

@@ -147,6 +147,57 @@ def test_gauss_seidel_matches_dense_splitting():
     np.testing.assert_allclose(got, want, rtol=0, atol=1e-12)
 
 
+@pytest.mark.parametrize("omega", [0.8, 2 / 3, 1.0])
+def test_weighted_jacobi_matches_dense_splitting(omega):
+    # One weighted-Jacobi sweep is x + omega D^-1 (b - A x) of the dense
+    # 5-point system (unequal steps); a discrete-harmonic quadratic is a fixed
+    # point, and omega = 1 on a single interior node equals one Gauss-Seidel
+    # update of it.
+    seg_x, seg_y, hx, hy = 6, 4, 0.25, 0.4
+    f, g = _random_problem(seg_x, seg_y, hx, hy, 7)
+    u0 = g.copy()
+    u0[1:-1, 1:-1] = np.random.default_rng(8).standard_normal((seg_x - 1, seg_y - 1))
+    got = op_smooth(seg_x, seg_y, hx, hy, f, u0, 1, smoother=1, omega=omega)
+    idx = [(i, j) for i in range(1, seg_x) for j in range(1, seg_y)]
+    pos = {p: k for k, p in enumerate(idx)}
+    n = len(idx)
+    A = np.zeros((n, n))
+    b = np.zeros(n)
+    for k, (i, j) in enumerate(idx):
+        A[k, k] = 2 / hx**2 + 2 / hy**2
+        b[k] = f[i, j]
+        for (di, dj, h2) in ((-1, 0, hx**2), (1, 0, hx**2), (0, -1, hy**2), (0, 1, hy**2)):
+            q = (i + di, j + dj)
+            if q in pos:
+                A[k, pos[q]] = -1 / h2
+            else:
+                b[k] += u0[q] / h2
+    x = np.array([u0[p] for p in idx])
+    x_new = x + omega * (b - A @ x) / np.diag(A)
+    want = u0.copy()
+    for k, p in enumerate(idx):
+        want[p] = x_new[k]
+    np.testing.assert_allclose(got, want, rtol=0, atol=1e-12)
+    lp = problems.laplace1(16)
+    u = op_smooth(lp.seg_x, lp.seg_y, lp.hx, lp.hy, None, lp.exact, 3, smoother=1, omega=omega)
+    np.testing.assert_allclose(u, lp.exact, rtol=0, atol=1e-13)
+    one = np.zeros((3, 3))
+    one[0, :], one[2, :], one[:, 0], one[:, 2] = 1.0, 2.0, 3.0, 4.0
+    gs = op_smooth(2, 2, 0.5, 0.5, None, one, 1)
+    jc = op_smooth(2, 2, 0.5, 0.5, None, one, 1, smoother=1, omega=1.0)
+    assert jc[1, 1] == gs[1, 1]
+
+
+def test_weighted_jacobi_vcycle_converges_to_dst_solution():
+    p = problems.config(2, 128)
+    u, k, hist = plain_solve(p.seg_x, p.seg_y, p.hx, p.hy, p.f, p.initial("zero"), 2, 2, 3, 1e-10, 80,
+                             smoother=1, omega=0.8)
+    assert hist[-1] <= 1e-10 and k < 40
+    assert np.median(hist[1:] / hist[:-1]) < 0.5
+    ex = dst_solve(p.f, p.g, p.hx, p.hy)
+    assert np.abs(u - ex).max() < 1e-8 * np.abs(ex).max()
+
+
 # ------------------------------------------------------------------ V-cycles
 @pytest.mark.parametrize("seg_x,seg_y,hx,hy", [(16, 16, 1 / 16, 1 / 16), (32, 8, 0.1, 0.3), (64, 128, 0.02, 0.01)])
 def test_vcycle_converges_to_dst_solution(seg_x, seg_y, hx, hy):

@@ -135,6 +135,22 @@ __global__ void __launch_bounds__(BX *BY)
   u[c] = rb_point<T, EQ, LAP>(u[c - pitch], u[c + pitch], u[c - 1], u[c + 1], LAP ? T(0) : f[c], hx2, hy2, coef);
 }
 
+// Weighted-Jacobi sweep (an addition: the reference smooths with red-black
+// Gauss-Seidel only): every interior node from the previous iterate,
+// out = u + w (g - u) with g the Gauss-Seidel point formula.
+template <typename T, bool EQ, bool LAP>
+__global__ void __launch_bounds__(BX *BY) k_wjacobi(T *__restrict__ out, const T *__restrict__ u,
+                                                    const T *__restrict__ f, int nx, int ny, int pitch, T hx2, T hy2,
+                                                    T coef, T w) {
+  const int j = blockIdx.x * BX + threadIdx.x + 1;
+  const int i = blockIdx.y * BY + threadIdx.y + 1;
+  if (i > nx - 2 || j > ny - 2) return;
+  const int c = i * pitch + j;
+  const T o = u[c];
+  const T g = rb_point<T, EQ, LAP>(u[c - pitch], u[c + pitch], u[c - 1], u[c + 1], LAP ? T(0) : f[c], hx2, hy2, coef);
+  out[c] = o + w * (g - o);
+}
+
 // ---------------------------------------------------------------------------
 // Residual res = rhs - ((uW + uE - 2u)/hx2 + (uS + uN - 2u)/hy2), interior.
 // ---------------------------------------------------------------------------

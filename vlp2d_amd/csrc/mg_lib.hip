@@ -265,8 +265,39 @@ template <typename T> void half_sweep(Ctx *c, Lvl<T> &v, int lvl, int color) {
   });
 }
 
+// Weighted-Jacobi sweep: u -> res (the level's scratch, free while it
+// smooths), then the interior back into u.
+template <typename T> void jacobi_sweep(Ctx *c, Lvl<T> &v, int lvl) {
+  const dim3 g = grid2(v.ny - 2, v.nx - 2);
+  const T w = (T)c->o.omega;
+  launch(c, KC_SMOOTH, lvl, 3.0 * sizeof(T) * nodes(v), [&] {
+    if (v.eq) {
+      if (v.lap)
+        hipLaunchKernelGGL((k_wjacobi<T, true, true>), g, kBlock, 0, c->st, v.res, v.u, v.rhs, v.nx, v.ny, v.pitch, v.hx2,
+                           v.hy2, v.coef, w);
+      else
+        hipLaunchKernelGGL((k_wjacobi<T, true, false>), g, kBlock, 0, c->st, v.res, v.u, v.rhs, v.nx, v.ny, v.pitch,
+                           v.hx2, v.hy2, v.coef, w);
+    } else {
+      if (v.lap)
+        hipLaunchKernelGGL((k_wjacobi<T, false, true>), g, kBlock, 0, c->st, v.res, v.u, v.rhs, v.nx, v.ny, v.pitch,
+                           v.hx2, v.hy2, v.coef, w);
+      else
+        hipLaunchKernelGGL((k_wjacobi<T, false, false>), g, kBlock, 0, c->st, v.res, v.u, v.rhs, v.nx, v.ny, v.pitch,
+                           v.hx2, v.hy2, v.coef, w);
+    }
+  });
+  launch(c, KC_COPY, lvl, 2.0 * sizeof(T) * nodes(v), [&] {
+    hipLaunchKernelGGL(k_copy_interior<T>, g, kBlock, 0, c->st, v.u, v.res, v.nx, v.ny, v.pitch);
+  });
+}
+
 template <typename T> void smooth(Ctx *c, Lvl<T> &v, int lvl, int n) {
   for (int s = 0; s < n; ++s) {
+    if (c->o.smoother == VLP2D_SMOOTH_WJACOBI) {
+      jacobi_sweep(c, v, lvl);
+      continue;
+    }
     half_sweep(c, v, lvl, 0);
     half_sweep(c, v, lvl, 1);
   }
@@ -862,7 +893,7 @@ template <typename T> int complete_hier(Ctx *c, Hier<T> &H) {
       if (l == 0) HIPCHK(c, hipMemcpyAsync(v.u2, v.u, v.elems() * sizeof(T), hipMemcpyDeviceToDevice, c->st));
       else fill(c, v.u2, v.elems(), T(0));
     }
-    if (c->simple && l < c->L - 1 && !v.res) {
+    if (c->simple && (l < c->L - 1 || c->o.smoother == VLP2D_SMOOTH_WJACOBI) && !v.res) {
       RCHK(dalloc(c, &v.res, v.elems()));
       fill(c, v.res, v.elems(), T(0));
     }
@@ -1258,9 +1289,11 @@ void vlp2d_mg_default_opts(vlp2d_mg_opts *o) {
   o->flags = 0;
   o->coarse_nodes = 0;
   o->max_inner = 1;
+  o->smoother = VLP2D_SMOOTH_RBGS;
+  o->omega = 0.8;
 }
 
-int vlp2d_mg_version(void) { return (1 << 16) | 2; }
+int vlp2d_mg_version(void) { return (1 << 16) | 3; }
 
 static int create_impl(int segX, int segY, double hx, double hy, const double *f, const Expr *fe, double eps,
                        int device, const vlp2d_mg_opts *opts, vlp2d_mg_handle *out) {
@@ -1282,9 +1315,10 @@ static int create_impl(int segX, int segY, double hx, double hy, const double *f
   if (opts) o = *opts;
   if (o.nu1 < 0 || o.nu2 < 0 || o.nu_coarse < 1 || o.nu1 > 32 || o.nu2 > 32 || o.nu_coarse > 1024 ||
       (o.precision != VLP2D_PREC_FP64 && o.precision != VLP2D_PREC_MIXED) || o.coarse_nodes < -1 ||
-      o.max_inner < 1 || o.max_inner > 8) {
+      o.max_inner < 1 || o.max_inner > 8 || (o.smoother != VLP2D_SMOOTH_RBGS && o.smoother != VLP2D_SMOOTH_WJACOBI) ||
+      !(o.omega > 0.0 && o.omega <= 1.0)) {
     g_create_error = "invalid options (nu1, nu2 in [0,32], nu_coarse in [1,1024], precision 0 or 1, coarse_nodes >= -1, "
-                     "max_inner in [1,8])";
+                     "max_inner in [1,8], smoother 0 or 1, omega in (0,1])";
     return VLP2D_E_ARG;
   }
   const int nx = segX + 1, ny = segY + 1;
@@ -1312,7 +1346,8 @@ static int create_impl(int segX, int segY, double hx, double hy, const double *f
   c->o = o;
   c->profile = (o.flags & VLP2D_FLAG_PROFILE) != 0;
   // fused kernels are instantiated for 1..3 sweeps per pass
-  c->simple = (o.flags & VLP2D_FLAG_SIMPLE) != 0 || o.nu1 < 1 || o.nu2 < 1 || o.nu1 > 3 || o.nu2 > 3;
+  c->simple = (o.flags & VLP2D_FLAG_SIMPLE) != 0 || o.nu1 < 1 || o.nu2 < 1 || o.nu1 > 3 || o.nu2 > 3 ||
+              o.smoother == VLP2D_SMOOTH_WJACOBI;
   c->L = std::min((int)std::log2((double)segX + 0.5), (int)std::log2((double)segY + 0.5));
   auto bail = [&](int rc) {
     g_create_error = c->err;
@@ -1513,6 +1548,8 @@ int vlp2d_mg_adjoint_exact(vlp2d_mg_handle c, const double *dL_du, double rtol, 
 int vlp2d_mg_forward_cycles(vlp2d_mg_handle c, int K) {
   if (!c) return VLP2D_E_ARG;
   if (K < 1 || K > 4096) return fail(c, VLP2D_E_ARG, "K must be in [1, 4096]");
+  if (c->o.smoother != VLP2D_SMOOTH_RBGS)
+    return fail(c, VLP2D_E_STATE, "the truncated-cycle adjoint transposes the red-black Gauss-Seidel cycle only");
   HIPCHK(c, hipSetDevice(c->dev));
   RCHK(complete_hier(c, c->h64));
   Lvl<double> &P = c->h64.lv[0];

@@ -31,6 +31,8 @@ class Options:
     flags: int = 0
     coarse_nodes: int = 0
     max_inner: int = 1  # mixed: FP32 V-cycles per FP64 refinement step, at most
+    smoother: int = _lib.SMOOTH_RBGS  # or SMOOTH_WJACOBI (one operator per launch)
+    omega: float = 0.8  # weighted-Jacobi weight
 
     @staticmethod
     def performance(mixed=False, **kw):
@@ -42,7 +44,7 @@ class Options:
 
     def to_c(self):
         return _lib.Opts(self.nu1, self.nu2, self.nu_coarse, self.precision, self.flags, self.coarse_nodes,
-                         self.max_inner)
+                         self.max_inner, self.smoother, self.omega)
 
 
 def _sample_rhs(f_ksi, nx, ny, hx, hy):
