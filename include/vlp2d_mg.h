@@ -49,6 +49,18 @@ extern "C" {
                                   point formula; no counterpart in the reference;
                                   runs one operator per launch                  */
 
+#define VLP2D_RESTRICT_HALF 0 /* half weighting 1/8 [0 1 0; 1 4 1; 0 1 0] (the
+                                 reference's, MultiGridScheme.cs:145-169)      */
+#define VLP2D_RESTRICT_FULL 1 /* full weighting 1/16 [1 2 1; 2 4 2; 1 2 1]; no
+                                 counterpart in the reference; runs one
+                                 operator per launch                           */
+
+#define VLP2D_CYCLE_V 0   /* V-cycle per iteration (the reference's doIteration,
+                             MultiGridScheme.cs:65)                          */
+#define VLP2D_CYCLE_FMG 1 /* FMG per iteration: the reference's unused FMG()
+                             (MultiGridScheme.cs:117-132); FP64, one operator
+                             per launch                                      */
+
 /* Option flags (vlp2d_mg_opts.flags). */
 #define VLP2D_FLAG_NO_GRAPH 1u   /* launch kernels directly, no hipGraph replay   */
 #define VLP2D_FLAG_PROFILE 2u    /* time every kernel class with hipEvents        */
@@ -87,6 +99,8 @@ typedef struct {
                       max_inner (step 0 runs max_inner).                    */
   int smoother;    /* VLP2D_SMOOTH_* (default RBGS)                         */
   double omega;    /* weighted-Jacobi weight in (0, 1] (default 0.8)        */
+  int restriction; /* VLP2D_RESTRICT_* (default HALF)                       */
+  int cycle;       /* VLP2D_CYCLE_* (default V)                             */
 } vlp2d_mg_opts;
 
 typedef struct {
@@ -96,7 +110,8 @@ typedef struct {
   double rel_residual;    /* final ||r||_2 / r0                               */
   double seconds;         /* device time of the solve, hipEvents              */
   double munknowns_per_s; /* interior unknowns / seconds / 1e6                */
-  int vcycles;            /* V-cycles run (mixed: over all refinement steps) */
+  int vcycles;            /* V-cycles run from level 0 (mixed: over all
+                             refinement steps; FMG: two per iteration)       */
 } vlp2d_mg_stats;
 
 typedef struct {

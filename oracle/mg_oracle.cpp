@@ -170,6 +170,12 @@ template <class T> struct MG {
   // with weight omega (an addition: the reference has no Jacobi smoother)
   int smoother = 0;
   double omega = 0.8;
+  // restriction: 0 half weighting (the reference's), 1 full weighting (an
+  // addition: the reference's full-weighting branch is dead code)
+  int restriction = 0;
+  // cycle: 0 V-cycle (MultiGridScheme.cs:65), 1 full multigrid (the
+  // reference's FMG(), :117-132, which its doIteration leaves commented out)
+  int cycle = 0;
   std::vector<T> tmp;  // Jacobi: the iterate before the sweep
 
   // fn: scheme right-hand side (= -f) at all nodes, or nullptr for Laplace.
@@ -285,6 +291,24 @@ template <class T> struct MG {
       }
   }
 
+  // --- full-weighting restriction (1/16)[1 2 1; 2 4 2; 1 2 1]
+  void restrict_fw(int l) {
+    Level<T> &f = lv[l];
+    Level<T> &c = lv[l + 1];
+    const T *re = f.res.data();
+    T *rh = c.rhs.data();
+    const T w = Num<T>::from(1.0) / Num<T>::from(16.0);
+    const T four = Num<T>::from(4.0), two = Num<T>::from(2.0);
+    const int nyf = f.ny;
+#pragma omp parallel for schedule(static) if (c.par)
+    for (int i = 1; i < c.nx - 1; ++i)
+      for (int j = 1; j < c.ny - 1; ++j) {
+        size_t k = (size_t)(2 * i) * nyf + 2 * j;
+        rh[c.at(i, j)] = w * (re[k] * four + (re[k + nyf] + re[k - nyf] + re[k + 1] + re[k - 1]) * two +
+                              (re[k + nyf + 1] + re[k + nyf - 1] + re[k - nyf + 1] + re[k - nyf - 1]));
+      }
+  }
+
   // --- half-weighting restriction (MultiGridScheme.cs:145-169)
   void restrict_hw(int l) {
     Level<T> &f = lv[l];
@@ -323,17 +347,41 @@ template <class T> struct MG {
       }
   }
 
-  // --- V-cycle (MultiGridScheme.cs:93-115). Returns the update norm
+  // --- One iteration: a V-cycle or FMG. Returns the update norm
   //     max|u - snapshot| over interior nodes of level 0 (:69, :111).
-  double vcycle(int nu1, int nu2, int nuc) {
+  double vcycle(int nu1, int nu2, int nuc) { return cycle == 1 ? fmg(nu1, nu2, nuc) : vcycle_from(nu1, nu2, nuc, 0); }
+
+  // --- FMG (MultiGridScheme.cs:117-132): the level-0 residual restricted to
+  //     every level without smoothing, one sweep on the coarsest, then from
+  //     coarse to fine: prolongation and a V-cycle starting at that level;
+  //     a last V-cycle from level 0 gives the update norm.
+  double fmg(int nu1, int nu2, int nuc) {
     for (int l = 0; l <= L - 2; ++l) {
+      residual_into(l, lv[l].res.data());
+      if (restriction == 1) restrict_fw(l);
+      else restrict_hw(l);
+      std::fill(lv[l + 1].u.begin(), lv[l + 1].u.end(), Num<T>::from(0.0));
+    }
+    smooth(L - 1, 1);
+    for (int l = L - 2; l >= 0; --l) {
+      interpolate(l);
+      vcycle_from(nu1, nu2, nuc, l);
+    }
+    return vcycle_from(nu1, nu2, nuc, 0);
+  }
+
+  // --- V-cycle over levels s..L-1 (MultiGridScheme.cs:93-115, startLevel
+  //     s); the level-0 snapshot is taken only when s == 0.
+  double vcycle_from(int nu1, int nu2, int nuc, int s) {
+    for (int l = s; l <= L - 2; ++l) {
       smooth(l, nu1);
       residual_into(l, lv[l].res.data());
-      restrict_hw(l);
+      if (restriction == 1) restrict_fw(l);
+      else restrict_hw(l);
       std::fill(lv[l + 1].u.begin(), lv[l + 1].u.end(), Num<T>::from(0.0));
     }
     smooth(L - 1, nuc);
-    for (int l = L - 2; l >= 0; --l) {
+    for (int l = L - 2; l >= s; --l) {
       interpolate(l);
       if (l == 0) {
         Level<T> &v = lv[0];
@@ -342,7 +390,7 @@ template <class T> struct MG {
       }
       smooth(l, nu2);
     }
-    if (L < 2) return 0.0;
+    if (L < 2 || s > 0) return 0.0;
     Level<T> &v = lv[0];
     double m = 0.0;
     for (int i = 1; i < v.nx - 1; ++i)
@@ -532,6 +580,20 @@ void orc_set_smoother(void *p, int kind, double omega) {
   else set((MG<DD> *)h->mg);
 }
 
+void orc_set_restriction(void *p, int kind) {
+  OrcHandle *h = (OrcHandle *)p;
+  if (h->prec == 0) ((MG<double> *)h->mg)->restriction = kind;
+  else if (h->prec == 1) ((MG<float> *)h->mg)->restriction = kind;
+  else ((MG<DD> *)h->mg)->restriction = kind;
+}
+
+void orc_set_cycle(void *p, int kind) {
+  OrcHandle *h = (OrcHandle *)p;
+  if (h->prec == 0) ((MG<double> *)h->mg)->cycle = kind;
+  else if (h->prec == 1) ((MG<float> *)h->mg)->cycle = kind;
+  else ((MG<DD> *)h->mg)->cycle = kind;
+}
+
 void orc_destroy(void *p) {
   OrcHandle *h = (OrcHandle *)p;
   if (!h) return;
@@ -621,7 +683,7 @@ int orc_plain_solve(void *p, int nu1, int nu2, int nuc, double rtol, int max_cyc
 // place. hist[k] = relative residual after k outer steps. Returns steps run.
 int orc_mixed_solve(int segX, int segY, double hx, double hy, const double *f, double *u, int nu1, int nu2,
                     int nuc, double rtol, int max_outer, double *hist, int nthreads, int inner, int smoother,
-                    double omega) {
+                    double omega, int restriction) {
   set_threads(nthreads);
   void *p64 = orc_create(0, segX, segY, hx, hy, f, nthreads);
   MG<double> &d = *(MG<double> *)((OrcHandle *)p64)->mg;
@@ -632,6 +694,7 @@ int orc_mixed_solve(int segX, int segY, double hx, double hy, const double *f, d
   MG<float> s(segX, segY, (float)hx, (float)hy, dummy.data());  // level 0 in Poisson form
   s.smoother = smoother;
   s.omega = omega;
+  s.restriction = restriction;
   std::vector<double> r(n, 0.0);
   auto resid = [&]() {
     std::fill(r.begin(), r.end(), 0.0);
@@ -718,13 +781,14 @@ void orc_adjoint_cycles(int segX, int segY, double hx, double hy, int K, int nu1
 }
 
 // ---- operator-level entry points (FP64), for pinning each operator alone --
-// Half-weighting restriction of a fine residual array (nx*ny, fine grid of
+// Restriction (half weighting, or full weighting when full != 0) of a fine residual array (nx*ny, fine grid of
 // segX x segY) to the coarse right-hand side (coarse interior written, coarse
 // boundary 0).
-void orc_op_restrict(int segX, int segY, const double *resf, double *rhsc) {
+void orc_op_restrict(int segX, int segY, const double *resf, double *rhsc, int full) {
   MG<double> mg(segX, segY, 1.0, 1.0, nullptr);
   std::copy(resf, resf + mg.lv[0].res.size(), mg.lv[0].res.begin());
-  mg.restrict_hw(0);
+  if (full) mg.restrict_fw(0);
+  else mg.restrict_hw(0);
   std::copy(mg.lv[1].rhs.begin(), mg.lv[1].rhs.end(), rhsc);
 }
 // uf += P uc (additive bilinear prolongation onto the fine interior).

@@ -48,7 +48,11 @@ def load():
     lib.orc_mixed_solve.restype = ctypes.c_int
     lib.orc_mixed_solve.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp, _dp,
                                     ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, _dp,
-                                    ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double]
+                                    ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int]
+    lib.orc_set_restriction.restype = None
+    lib.orc_set_restriction.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    lib.orc_set_cycle.restype = None
+    lib.orc_set_cycle.argtypes = [ctypes.c_void_p, ctypes.c_int]
     lib.orc_set_smoother.restype = None
     lib.orc_set_smoother.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double]
     lib.orc_forward_cycles.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp, _dp,
@@ -56,7 +60,7 @@ def load():
     lib.orc_adjoint_cycles.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                        ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp,
                                        ctypes.c_int]
-    lib.orc_op_restrict.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp]
+    lib.orc_op_restrict.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.c_int]
     lib.orc_op_prolong.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp]
     lib.orc_op_residual.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp, _dp, _dp]
     lib.orc_op_smooth.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp, _dp,
@@ -102,6 +106,15 @@ class Oracle:
         Jacobi u += omega (g - u) with the same point formula g."""
         self.lib.orc_set_smoother(self.h, int(kind), float(omega))
 
+    def set_restriction(self, kind):
+        """kind 0: half weighting (the reference's); 1: full weighting."""
+        self.lib.orc_set_restriction(self.h, int(kind))
+
+    def set_cycle(self, kind):
+        """kind 0: V-cycle (the reference's); 1: FMG, the reference's unused
+        FMG() driver (MultiGridScheme.cs:117-132)."""
+        self.lib.orc_set_cycle(self.h, int(kind))
+
     def vcycle(self, nu1=3, nu2=3, nuc=3):
         return self.lib.orc_vcycle(self.h, nu1, nu2, nuc, self.nthreads)
 
@@ -126,9 +139,11 @@ class Oracle:
 
 
 def plain_solve(seg_x, seg_y, hx, hy, f, u, nu1=2, nu2=2, nuc=3, rtol=1e-10, max_cycles=100, nthreads=0,
-                smoother=0, omega=0.8):
+                smoother=0, omega=0.8, restriction=0, cycle=0):
     o = Oracle(seg_x, seg_y, hx, hy, f, "double", nthreads)
     o.set_smoother(smoother, omega)
+    o.set_restriction(restriction)
+    o.set_cycle(cycle)
     o.set_u(u)
     k, hist = o.plain_solve(nu1, nu2, nuc, rtol, max_cycles)
     out = o.get_u()
@@ -137,13 +152,13 @@ def plain_solve(seg_x, seg_y, hx, hy, f, u, nu1=2, nu2=2, nuc=3, rtol=1e-10, max
 
 
 def mixed_solve(seg_x, seg_y, hx, hy, f, u, nu1=2, nu2=2, nuc=3, rtol=1e-10, max_outer=100, nthreads=0, inner=1,
-                smoother=0, omega=0.8):
+                smoother=0, omega=0.8, restriction=0):
     lib = load()
     u = np.array(u, dtype=np.float64, order="C").reshape(seg_x + 1, seg_y + 1)
     fa = None if f is None else np.ascontiguousarray(f, dtype=np.float64)
     hist = np.zeros(max_outer + 1)
     k = lib.orc_mixed_solve(seg_x, seg_y, hx, hy, _ptr(fa), _ptr(u), nu1, nu2, nuc, rtol, max_outer, _ptr(hist),
-                            nthreads, inner, smoother, omega)
+                            nthreads, inner, smoother, omega, restriction)
     return u, k, hist[: k + 1]
 
 
@@ -166,11 +181,11 @@ def adjoint_cycles(seg_x, seg_y, hx, hy, ubar, K, nu1=2, nu2=2, nuc=3, nthreads=
 
 
 # ---- single operators (FP64) -------------------------------------------------
-def op_restrict(seg_x, seg_y, res_fine):
+def op_restrict(seg_x, seg_y, res_fine, full=False):
     lib = load()
     r = np.ascontiguousarray(res_fine, dtype=np.float64)
     out = np.zeros((seg_x // 2 + 1, seg_y // 2 + 1))
-    lib.orc_op_restrict(seg_x, seg_y, _ptr(r), _ptr(out))
+    lib.orc_op_restrict(seg_x, seg_y, _ptr(r), _ptr(out), 1 if full else 0)
     return out
 
 

@@ -32,8 +32,8 @@ def test_default_opts_and_version():
     lib = load_library()
     o = _lib.Opts()
     lib.vlp2d_mg_default_opts(ctypes.byref(o))
-    assert (o.nu1, o.nu2, o.nu_coarse, o.precision, o.flags, o.coarse_nodes, o.max_inner, o.smoother, o.omega) == \
-        (3, 3, 3, 0, 0, 0, 1, 0, 0.8)
+    assert (o.nu1, o.nu2, o.nu_coarse, o.precision, o.flags, o.coarse_nodes, o.max_inner, o.smoother, o.omega,
+            o.restriction, o.cycle) == (3, 3, 3, 0, 0, 0, 1, 0, 0.8, 0, 0)
     assert lib.vlp2d_mg_version() >> 16 == 1
 
 
@@ -62,6 +62,14 @@ def test_create_rejects_bad_steps_and_options():
         o = _lib.Opts(2, 2, 3, 0, 0, 0, 1, sm, w)
         assert lib.vlp2d_mg_create(8, 8, 0.1, 0.1, None, 1e-13, 0, ctypes.byref(o), ctypes.byref(h)) == _lib.E_ARG
         assert b"omega" in lib.vlp2d_mg_last_error(None)
+    for rs in (2, -1):  # restriction 0/1
+        o = _lib.Opts(2, 2, 3, 0, 0, 0, 1, 0, 0.8, rs)
+        assert lib.vlp2d_mg_create(8, 8, 0.1, 0.1, None, 1e-13, 0, ctypes.byref(o), ctypes.byref(h)) == _lib.E_ARG
+        assert b"restriction" in lib.vlp2d_mg_last_error(None)
+    for prec, cyc in ((0, 2), (0, -1), (1, 1)):  # cycle 0/1; FMG in FP64 only
+        o = _lib.Opts(2, 2, 3, prec, 0, 0, 1, 0, 0.8, 0, cyc)
+        assert lib.vlp2d_mg_create(8, 8, 0.1, 0.1, None, 1e-13, 0, ctypes.byref(o), ctypes.byref(h)) == _lib.E_ARG
+        assert b"cycle" in lib.vlp2d_mg_last_error(None)
     assert lib.vlp2d_mg_create(8, 8, 0.1, 0.1, None, 1e-13, 0, None, None) == _lib.E_ARG
 
 

@@ -238,6 +238,82 @@ def test_weighted_jacobi_bitwise(seg_x, seg_y, hx, hy, laplace, omega):
         s.cleanup()
 
 
+@pytest.mark.parametrize("smoother", [_lib.SMOOTH_RBGS, _lib.SMOOTH_WJACOBI])
+@pytest.mark.parametrize("seg_x,seg_y,hx,hy,laplace", [(64, 32, 1 / 64, 1 / 64, False),
+                                                       (256, 128, np.e / 256, np.pi / 128, False),
+                                                       (128, 128, 1 / 128, 1 / 128, True)])
+def test_full_weighting_bitwise(seg_x, seg_y, hx, hy, laplace, smoother):
+    """Full-weighting restriction (an addition; the reference restricts by
+    half weighting): parity-mode cycles with the update norm, and FP64 and
+    mixed V(2,2) solves to 1e-10, bit for bit the oracle's."""
+    _say(f"full weighting {seg_x}x{seg_y} smoother={smoother}")
+    f, g = _random(seg_x, seg_y, 23 + seg_x, laplace)
+    u0 = _mean_init(g)
+    o = Oracle(seg_x, seg_y, hx, hy, f)
+    o.set_restriction(1)
+    o.set_smoother(smoother)
+    o.set_u(u0)
+    s = MultiGridSchemeHIP(seg_x, seg_y, hx, hy, f, eps=1e-13,
+                           options=Options(restriction=_lib.RESTRICT_FULL, smoother=smoother))
+    s.un0[:] = u0
+    s.init_after_boundaries_and_initial_iteration_inited()
+    for k in range(4):
+        m_ref = o.vcycle(3, 3, 3)
+        s.do_iteration(k)
+        assert np.array_equal(s.get_array(), o.get_u()), (k, np.abs(s.get_array() - o.get_u()).max())
+        assert s.last_metric == m_ref
+    s.cleanup()
+    z = np.zeros_like(g) if laplace else f
+    for mixed in (False, True):
+        solve = mixed_solve if mixed else plain_solve
+        ur, kr, _ = solve(seg_x, seg_y, hx, hy, z, g, 2, 2, 3, 1e-10, 150, smoother=smoother, restriction=1)
+        s = MultiGridSchemeHIP(seg_x, seg_y, hx, hy, z,
+                               options=Options.performance(mixed, smoother=smoother, restriction=_lib.RESTRICT_FULL))
+        s.un0[:] = g
+        s.init_after_boundaries_and_initial_iteration_inited()
+        st = s.solve(1e-10, 150)
+        assert st["cycles"] == kr, (mixed, st["cycles"], kr)
+        assert np.array_equal(s.get_array(), ur), (mixed, np.abs(s.get_array() - ur).max())
+        s.cleanup()
+
+
+@pytest.mark.parametrize("restriction", [_lib.RESTRICT_HALF, _lib.RESTRICT_FULL])
+@pytest.mark.parametrize("seg_x,seg_y,hx,hy,laplace", [(64, 32, 1 / 64, 1 / 64, False),
+                                                       (256, 128, np.e / 256, np.pi / 128, False),
+                                                       (128, 128, 1 / 128, 1 / 128, True)])
+def test_fmg_bitwise(seg_x, seg_y, hx, hy, laplace, restriction):
+    """FMG iterations (the reference's unused FMG(), MultiGridScheme.cs:117-132):
+    parity-mode iterations with the update norm, and an FP64 solve to 1e-10,
+    bit for bit the oracle's."""
+    _say(f"fmg {seg_x}x{seg_y} restriction={restriction}")
+    f, g = _random(seg_x, seg_y, 31 + seg_x, laplace)
+    u0 = _mean_init(g)
+    o = Oracle(seg_x, seg_y, hx, hy, f)
+    o.set_cycle(1)
+    o.set_restriction(restriction)
+    o.set_u(u0)
+    s = MultiGridSchemeHIP(seg_x, seg_y, hx, hy, f, eps=1e-13,
+                           options=Options(cycle=_lib.CYCLE_FMG, restriction=restriction))
+    s.un0[:] = u0
+    s.init_after_boundaries_and_initial_iteration_inited()
+    for k in range(3):
+        m_ref = o.vcycle(3, 3, 3)
+        s.do_iteration(k)
+        assert np.array_equal(s.get_array(), o.get_u()), (k, np.abs(s.get_array() - o.get_u()).max())
+        assert s.last_metric == m_ref
+    s.cleanup()
+    z = np.zeros_like(g) if laplace else f
+    ur, kr, _ = plain_solve(seg_x, seg_y, hx, hy, z, g, 2, 2, 3, 1e-10, 60, restriction=restriction, cycle=1)
+    s = MultiGridSchemeHIP(seg_x, seg_y, hx, hy, z,
+                           options=Options.performance(cycle=_lib.CYCLE_FMG, restriction=restriction))
+    s.un0[:] = g
+    s.init_after_boundaries_and_initial_iteration_inited()
+    st = s.solve(1e-10, 60)
+    assert st["cycles"] == kr and st["vcycles"] == 2 * kr, (st["cycles"], kr)
+    assert np.array_equal(s.get_array(), ur), np.abs(s.get_array() - ur).max()
+    s.cleanup()
+
+
 def test_config1_reference_stop_rule_cycle_count():
     """Config 1 (257^2 Laplace, g = x^2 - y^2) through the reference loop:
     cycles until the update norm <= eps, same count as the oracle, and the
@@ -410,11 +486,12 @@ def test_errors_are_reported_not_faulted():
         s.adjoint_cycles(g)
     assert e.value.code == _lib.E_STATE
     s.cleanup()
-    # the truncated-cycle adjoint transposes the Gauss-Seidel cycle only
-    s = MultiGridSchemeHIP(16, 16, 1 / 16, 1 / 16, f, options=Options.performance(smoother=_lib.SMOOTH_WJACOBI))
-    s.un0[:] = g
-    s.init_after_boundaries_and_initial_iteration_inited()
-    with pytest.raises(VLP2DError) as e:
-        s.forward_cycles(2)
-    assert e.value.code == _lib.E_STATE
-    s.cleanup()
+    # the truncated-cycle adjoint transposes the Gauss-Seidel, half-weighting cycle only
+    for kw in (dict(smoother=_lib.SMOOTH_WJACOBI), dict(restriction=_lib.RESTRICT_FULL), dict(cycle=_lib.CYCLE_FMG)):
+        s = MultiGridSchemeHIP(16, 16, 1 / 16, 1 / 16, f, options=Options.performance(**kw))
+        s.un0[:] = g
+        s.init_after_boundaries_and_initial_iteration_inited()
+        with pytest.raises(VLP2DError) as e:
+            s.forward_cycles(2)
+        assert e.value.code == _lib.E_STATE
+        s.cleanup()

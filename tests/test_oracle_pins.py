@@ -62,6 +62,98 @@ def test_restriction_weights_and_constants():
     assert np.count_nonzero(op_restrict(seg_x, seg_y, d)) == 0
 
 
+def test_full_weighting_weights_and_transpose():
+    # full weighting (an addition): constants stay constants, the exact
+    # 4/16, 2/16, 1/16 delta responses, and FW = P^T / 4 (P the bilinear
+    # prolongation) on interior data
+    seg_x, seg_y = 16, 8
+    c = op_restrict(seg_x, seg_y, np.full((seg_x + 1, seg_y + 1), 3.0), full=True)
+    assert np.all(c[1:-1, 1:-1] == 3.0) and np.all(c[0, :] == 0)
+    d = np.zeros((seg_x + 1, seg_y + 1))
+    d[6, 4] = 1.0
+    c = op_restrict(seg_x, seg_y, d, full=True)
+    assert c[3, 2] == 0.25 and np.count_nonzero(c) == 1
+    d = np.zeros((seg_x + 1, seg_y + 1))
+    d[7, 4] = 1.0
+    c = op_restrict(seg_x, seg_y, d, full=True)
+    assert c[3, 2] == 0.125 and c[4, 2] == 0.125 and np.count_nonzero(c) == 2
+    d = np.zeros((seg_x + 1, seg_y + 1))
+    d[7, 5] = 1.0
+    c = op_restrict(seg_x, seg_y, d, full=True)
+    assert c[3, 2] == c[4, 2] == c[3, 3] == c[4, 3] == 0.0625 and np.count_nonzero(c) == 4
+    rng = np.random.default_rng(9)
+    y = np.zeros((seg_x + 1, seg_y + 1))
+    y[1:-1, 1:-1] = rng.standard_normal((seg_x - 1, seg_y - 1))
+    xc = np.zeros((seg_x // 2 + 1, seg_y // 2 + 1))
+    xc[1:-1, 1:-1] = rng.standard_normal((seg_x // 2 - 1, seg_y // 2 - 1))
+    Px = op_prolong(seg_x, seg_y, xc, np.zeros_like(y))
+    Ry = op_restrict(seg_x, seg_y, y, full=True)
+    assert abs(np.sum(Px * y) - 4 * np.sum(xc * Ry)) < 1e-12
+
+
+def test_full_weighting_vcycle_converges_to_dst_solution():
+    p = problems.config(3, 128)
+    u, k, hist = plain_solve(p.seg_x, p.seg_y, p.hx, p.hy, p.f, p.initial("zero"), 2, 2, 3, 1e-10, 60, restriction=1)
+    assert hist[-1] <= 1e-10 and k <= 12
+    ex = dst_solve(p.f, p.g, p.hx, p.hy)
+    assert np.abs(u - ex).max() < 1e-8 * np.abs(ex).max()
+
+
+def _vcycle_on_level(seg_x, seg_y, hx, hy, rhs, e, nu1, nu2, nuc):
+    """A V-cycle started at a coarse level (the reference's VCycle(level)):
+    the same as a V-cycle of a grid of that level's size and steps whose
+    scheme right-hand side -f is that level's rhs, with zero borders."""
+    o = Oracle(seg_x, seg_y, hx, hy, -rhs)
+    o.set_u(e)
+    o.vcycle(nu1, nu2, nuc)
+    out = o.get_u()
+    o.close()
+    return out
+
+
+@pytest.mark.parametrize("seg_x,seg_y", [(32, 4), (32, 8), (64, 16)])
+def test_fmg_is_the_reference_composition(seg_x, seg_y):
+    # FMG (MultiGridScheme.cs:117-132) rebuilt from the separately pinned
+    # operators: restrict the level-0 residual down the hierarchy (coarse
+    # iterates zero, so each coarse residual is its rhs), one sweep on the
+    # coarsest level, then prolong + V-cycle from each level up, prolong
+    # into level 0 and two V-cycles there; bit for bit, update norm included
+    hx, hy = 0.03, 0.07
+    rng = np.random.default_rng(seg_x + seg_y)
+    f = rng.standard_normal((seg_x + 1, seg_y + 1))
+    u0 = rng.standard_normal((seg_x + 1, seg_y + 1))
+    o = Oracle(seg_x, seg_y, hx, hy, f)
+    o.set_cycle(1)
+    o.set_u(u0)
+    m = o.vcycle(2, 2, 3)
+    L = int(np.log2(min(seg_x, seg_y)))
+    segs = [(seg_x >> l, seg_y >> l) for l in range(L)]
+    steps = [(hx * 2 ** l, hy * 2 ** l) for l in range(L)]
+    rhs = [None] * L
+    rhs[1] = op_restrict(seg_x, seg_y, op_residual(seg_x, seg_y, hx, hy, f, u0))
+    for l in range(1, L - 1):
+        rhs[l + 1] = op_restrict(*segs[l], rhs[l])
+    e = op_smooth(*segs[L - 1], *steps[L - 1], -rhs[L - 1], np.zeros_like(rhs[L - 1]), 1)
+    for l in range(L - 2, 0, -1):
+        e = op_prolong(*segs[l], e, np.zeros_like(rhs[l]))
+        e = _vcycle_on_level(*segs[l], *steps[l], rhs[l], e, 2, 2, 3)
+    u1 = op_prolong(seg_x, seg_y, e, u0)
+    o2 = Oracle(seg_x, seg_y, hx, hy, f)
+    o2.set_u(u1)
+    o2.vcycle(2, 2, 3)
+    m2 = o2.vcycle(2, 2, 3)
+    assert np.array_equal(o.get_u(), o2.get_u()) and m == m2
+
+
+def test_fmg_converges_to_dst_solution_faster_than_vcycles():
+    p = problems.config(3, 128)
+    u, k, hist = plain_solve(p.seg_x, p.seg_y, p.hx, p.hy, p.f, p.initial("zero"), 2, 2, 3, 1e-10, 20, cycle=1)
+    _, kv, _ = plain_solve(p.seg_x, p.seg_y, p.hx, p.hy, p.f, p.initial("zero"), 2, 2, 3, 1e-10, 20)
+    assert hist[-1] <= 1e-10 and k < kv and hist[1] < 1e-3
+    ex = dst_solve(p.f, p.g, p.hx, p.hy)
+    assert np.abs(u - ex).max() < 1e-8 * np.abs(ex).max()
+
+
 def test_prolongation_reproduces_bilinear():
     seg_x, seg_y = 16, 8
     xc = np.arange(seg_x // 2 + 1)[:, None] * 2.0

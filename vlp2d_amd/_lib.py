@@ -23,6 +23,8 @@ FLAG_STREAM_SMALL = 64
 FLAG_NO_CYCLE_FUSION = 128
 FLAG_GRAPH_SOLVE = 256
 SMOOTH_RBGS, SMOOTH_WJACOBI = 0, 1
+RESTRICT_HALF, RESTRICT_FULL = 0, 1
+CYCLE_V, CYCLE_FMG = 0, 1
 
 # Every entry point declared in include/vlp2d_mg.h.
 EXPORTS = [
@@ -45,7 +47,8 @@ class VLP2DError(RuntimeError):
 class Opts(ctypes.Structure):
     _fields_ = [("nu1", ctypes.c_int), ("nu2", ctypes.c_int), ("nu_coarse", ctypes.c_int),
                 ("precision", ctypes.c_int), ("flags", ctypes.c_uint), ("coarse_nodes", ctypes.c_int),
-                ("max_inner", ctypes.c_int), ("smoother", ctypes.c_int), ("omega", ctypes.c_double)]
+                ("max_inner", ctypes.c_int), ("smoother", ctypes.c_int), ("omega", ctypes.c_double),
+                ("restriction", ctypes.c_int), ("cycle", ctypes.c_int)]
 
 
 class Stats(ctypes.Structure):

@@ -185,6 +185,19 @@ __global__ void __launch_bounds__(BX *BY) k_restrict(T *__restrict__ rhsc, const
   rhsc[i * pc + j] = T(0.125) * (res[k] * T(4) + (res[k + pf] + res[k - pf] + res[k + 1] + res[k - 1]));
 }
 
+// Full-weighting restriction (no counterpart in the reference):
+// rhs_c = 1/16 (4 r + 2 (N + S + E + W) + (NE + NW + SE + SW)) around r[2i,2j]
+template <typename T>
+__global__ void __launch_bounds__(BX *BY) k_restrict_fw(T *__restrict__ rhsc, const T *__restrict__ res, int ncx,
+                                                        int ncy, int pc, int pf) {
+  const int j = blockIdx.x * BX + threadIdx.x + 1;
+  const int i = blockIdx.y * BY + threadIdx.y + 1;
+  if (i > ncx - 2 || j > ncy - 2) return;
+  const int k = 2 * i * pf + 2 * j;
+  rhsc[i * pc + j] = T(0.0625) * (res[k] * T(4) + (res[k + pf] + res[k - pf] + res[k + 1] + res[k - 1]) * T(2) +
+                                  (res[k + pf + 1] + res[k + pf - 1] + res[k - pf + 1] + res[k - pf - 1]));
+}
+
 // p[0..n) = v
 template <typename T>
 __global__ void k_fill(T *__restrict__ p, long long n, T v) {

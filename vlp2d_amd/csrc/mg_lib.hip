@@ -303,32 +303,45 @@ template <typename T> void smooth(Ctx *c, Lvl<T> &v, int lvl, int n) {
   }
 }
 
-// V-cycle, one operator per launch (MultiGridScheme.cs:93-115).  If snap is
-// given the level-0 iterate is copied there before post-smoothing.  zero0:
-// level 0 starts from u = 0 (correction cycle of a mixed step).
-template <typename T> void vcycle_simple(Ctx *c, Hier<T> &H, bool zero0, double *snap) {
+// Residual of level l, restricted into level l + 1's right-hand side; the
+// coarse iterate set to zero.
+template <typename T> void resid_restrict(Ctx *c, Lvl<T> &f, Lvl<T> &g, int l) {
+  const double s = sizeof(T);
+  launch(c, KC_RESID, l, 3 * s * nodes(f), [&] {
+    hipLaunchKernelGGL(k_residual<T>, grid2(f.ny - 2, f.nx - 2), kBlock, 0, c->st, f.res, f.rhs, f.u, f.nx, f.ny, f.pitch, f.hx2, f.hy2);
+  });
+  launch(c, KC_RESTRICT, l, s * (nodes(f) + nodes(g)), [&] {
+    if (c->o.restriction == VLP2D_RESTRICT_FULL)
+      hipLaunchKernelGGL(k_restrict_fw<T>, grid2(g.ny - 2, g.nx - 2), kBlock, 0, c->st, g.rhs, f.res, g.nx, g.ny, g.pitch, f.pitch);
+    else
+      hipLaunchKernelGGL(k_restrict<T>, grid2(g.ny - 2, g.nx - 2), kBlock, 0, c->st, g.rhs, f.res, g.nx, g.ny, g.pitch, f.pitch);
+  });
+  fill(c, g.u, g.elems(), T(0), l + 1);
+}
+
+// Additive prolongation of level l + 1's iterate into level l.
+template <typename T> void prolong(Ctx *c, Lvl<T> &f, Lvl<T> &g, int l) {
+  launch(c, KC_PROLONG, l, (double)sizeof(T) * (2 * nodes(f) + nodes(g)), [&] {
+    hipLaunchKernelGGL(k_prolong<T>, grid2(g.ny - 1, g.nx - 1), kBlock, 0, c->st, f.u, g.u, g.nx, g.ny, f.pitch, g.pitch);
+  });
+}
+
+// V-cycle over levels from..L-1, one operator per launch
+// (MultiGridScheme.cs:93-115, startLevel = from).  If snap is given (from ==
+// 0) the level-0 iterate is copied there before post-smoothing.  zero0: level
+// 0 starts from u = 0 (correction cycle of a mixed step).
+template <typename T> void vcycle_simple(Ctx *c, Hier<T> &H, bool zero0, double *snap, int from = 0) {
   const int L = (int)H.lv.size();
   const double s = sizeof(T);
   if (zero0) fill(c, H.lv[0].u, H.lv[0].elems(), T(0));
-  for (int l = 0; l <= L - 2; ++l) {
-    Lvl<T> &f = H.lv[l];
-    Lvl<T> &g = H.lv[l + 1];
-    smooth(c, f, l, c->o.nu1);
-    launch(c, KC_RESID, l, 3 * s * nodes(f), [&] {
-      hipLaunchKernelGGL(k_residual<T>, grid2(f.ny - 2, f.nx - 2), kBlock, 0, c->st, f.res, f.rhs, f.u, f.nx, f.ny, f.pitch, f.hx2, f.hy2);
-    });
-    launch(c, KC_RESTRICT, l, s * (nodes(f) + nodes(g)), [&] {
-      hipLaunchKernelGGL(k_restrict<T>, grid2(g.ny - 2, g.nx - 2), kBlock, 0, c->st, g.rhs, f.res, g.nx, g.ny, g.pitch, f.pitch);
-    });
-    fill(c, g.u, g.elems(), T(0), l + 1);
+  for (int l = from; l <= L - 2; ++l) {
+    smooth(c, H.lv[l], l, c->o.nu1);
+    resid_restrict(c, H.lv[l], H.lv[l + 1], l);
   }
   smooth(c, H.lv[L - 1], L - 1, c->o.nu_coarse);
-  for (int l = L - 2; l >= 0; --l) {
+  for (int l = L - 2; l >= from; --l) {
     Lvl<T> &f = H.lv[l];
-    Lvl<T> &g = H.lv[l + 1];
-    launch(c, KC_PROLONG, l, s * (2 * nodes(f) + nodes(g)), [&] {
-      hipLaunchKernelGGL(k_prolong<T>, grid2(g.ny - 1, g.nx - 1), kBlock, 0, c->st, f.u, g.u, g.nx, g.ny, f.pitch, g.pitch);
-    });
+    prolong(c, f, H.lv[l + 1], l);
     if (l == 0 && snap) {
       launch(c, KC_COPY, l, 2 * s * nodes(f), [&] {
         hipLaunchKernelGGL(k_copy_interior<T>, grid2(f.ny - 2, f.nx - 2), kBlock, 0, c->st, (T *)(void *)snap, f.u, f.nx, f.ny, f.pitch);
@@ -336,6 +349,23 @@ template <typename T> void vcycle_simple(Ctx *c, Hier<T> &H, bool zero0, double 
     }
     smooth(c, f, l, c->o.nu2);
   }
+}
+
+// FMG iteration (MultiGridScheme.cs:117-132), one operator per launch: the
+// level-0 residual restricted down every level without smoothing (each coarse
+// iterate is zero, so its residual is its right-hand side), one sweep on the
+// coarsest level, then from coarse to fine a prolongation and a V-cycle
+// started at that level, and a last V-cycle from level 0 (snapshot taken
+// there).
+void fmg_simple(Ctx *c, Hier<double> &H, double *snap) {
+  const int L = (int)H.lv.size();
+  for (int l = 0; l <= L - 2; ++l) resid_restrict(c, H.lv[l], H.lv[l + 1], l);
+  smooth(c, H.lv[L - 1], L - 1, 1);
+  for (int l = L - 2; l >= 0; --l) {
+    prolong(c, H.lv[l], H.lv[l + 1], l);
+    vcycle_simple(c, H, false, nullptr, l);
+  }
+  vcycle_simple(c, H, false, snap, 0);
 }
 
 // ---- fused kernels -------------------------------------------------------------
@@ -633,7 +663,8 @@ void cycle64(Ctx *c, Hier<double> &H, int epi, bool pub = false) {
   MixView mx{};
   if (c->simple || H.lc == 0) {
     double *snap = epi == EPI_METRIC ? c->snap : nullptr;
-    if (c->simple) vcycle_simple(c, H, false, snap);
+    if (c->simple && c->o.cycle == VLP2D_CYCLE_FMG) fmg_simple(c, H, snap);
+    else if (c->simple) vcycle_simple(c, H, false, snap);
     else vcycle_fused(c, H, false, EPI_NONE, mx, snap);
     if (epi == EPI_RESNORM) resnorm(c, P, P.u, nullptr, pub);
     if (epi == EPI_METRIC) maxdiff(c, P, c->snap, 0);
@@ -1118,7 +1149,7 @@ int solve_impl(Ctx *c, Hier<double> &H, int prog_fp64, int prog_mixed, double rt
         m_prev = m;
       } else {
         RCHK(run_program(c, gkey(prog_fp64, P.par, 0) + kPubKey, [&] { cycle64(c, H, EPI_RESNORM, true); }));
-        ++vcycles;
+        vcycles += c->o.cycle == VLP2D_CYCLE_FMG ? 2 : 1;
       }
       RCHK(publish_wait(c));
       ++k;
@@ -1291,9 +1322,11 @@ void vlp2d_mg_default_opts(vlp2d_mg_opts *o) {
   o->max_inner = 1;
   o->smoother = VLP2D_SMOOTH_RBGS;
   o->omega = 0.8;
+  o->restriction = VLP2D_RESTRICT_HALF;
+  o->cycle = VLP2D_CYCLE_V;
 }
 
-int vlp2d_mg_version(void) { return (1 << 16) | 3; }
+int vlp2d_mg_version(void) { return (1 << 16) | 4; }
 
 static int create_impl(int segX, int segY, double hx, double hy, const double *f, const Expr *fe, double eps,
                        int device, const vlp2d_mg_opts *opts, vlp2d_mg_handle *out) {
@@ -1316,9 +1349,13 @@ static int create_impl(int segX, int segY, double hx, double hy, const double *f
   if (o.nu1 < 0 || o.nu2 < 0 || o.nu_coarse < 1 || o.nu1 > 32 || o.nu2 > 32 || o.nu_coarse > 1024 ||
       (o.precision != VLP2D_PREC_FP64 && o.precision != VLP2D_PREC_MIXED) || o.coarse_nodes < -1 ||
       o.max_inner < 1 || o.max_inner > 8 || (o.smoother != VLP2D_SMOOTH_RBGS && o.smoother != VLP2D_SMOOTH_WJACOBI) ||
-      !(o.omega > 0.0 && o.omega <= 1.0)) {
+      !(o.omega > 0.0 && o.omega <= 1.0) ||
+      (o.restriction != VLP2D_RESTRICT_HALF && o.restriction != VLP2D_RESTRICT_FULL) ||
+      (o.cycle != VLP2D_CYCLE_V && o.cycle != VLP2D_CYCLE_FMG) ||
+      (o.cycle == VLP2D_CYCLE_FMG && o.precision != VLP2D_PREC_FP64)) {
     g_create_error = "invalid options (nu1, nu2 in [0,32], nu_coarse in [1,1024], precision 0 or 1, coarse_nodes >= -1, "
-                     "max_inner in [1,8], smoother 0 or 1, omega in (0,1])";
+                     "max_inner in [1,8], smoother 0 or 1, omega in (0,1], restriction 0 or 1, cycle 0 or 1, "
+                     "cycle FMG with precision FP64 only)";
     return VLP2D_E_ARG;
   }
   const int nx = segX + 1, ny = segY + 1;
@@ -1347,7 +1384,7 @@ static int create_impl(int segX, int segY, double hx, double hy, const double *f
   c->profile = (o.flags & VLP2D_FLAG_PROFILE) != 0;
   // fused kernels are instantiated for 1..3 sweeps per pass
   c->simple = (o.flags & VLP2D_FLAG_SIMPLE) != 0 || o.nu1 < 1 || o.nu2 < 1 || o.nu1 > 3 || o.nu2 > 3 ||
-              o.smoother == VLP2D_SMOOTH_WJACOBI;
+              o.smoother == VLP2D_SMOOTH_WJACOBI || o.restriction == VLP2D_RESTRICT_FULL || o.cycle == VLP2D_CYCLE_FMG;
   c->L = std::min((int)std::log2((double)segX + 0.5), (int)std::log2((double)segY + 0.5));
   auto bail = [&](int rc) {
     g_create_error = c->err;
@@ -1548,8 +1585,9 @@ int vlp2d_mg_adjoint_exact(vlp2d_mg_handle c, const double *dL_du, double rtol, 
 int vlp2d_mg_forward_cycles(vlp2d_mg_handle c, int K) {
   if (!c) return VLP2D_E_ARG;
   if (K < 1 || K > 4096) return fail(c, VLP2D_E_ARG, "K must be in [1, 4096]");
-  if (c->o.smoother != VLP2D_SMOOTH_RBGS)
-    return fail(c, VLP2D_E_STATE, "the truncated-cycle adjoint transposes the red-black Gauss-Seidel cycle only");
+  if (c->o.smoother != VLP2D_SMOOTH_RBGS || c->o.restriction != VLP2D_RESTRICT_HALF || c->o.cycle != VLP2D_CYCLE_V)
+    return fail(c, VLP2D_E_STATE,
+                "the truncated-cycle adjoint transposes the red-black Gauss-Seidel, half-weighting V-cycle only");
   HIPCHK(c, hipSetDevice(c->dev));
   RCHK(complete_hier(c, c->h64));
   Lvl<double> &P = c->h64.lv[0];

@@ -33,6 +33,8 @@ class Options:
     max_inner: int = 1  # mixed: FP32 V-cycles per FP64 refinement step, at most
     smoother: int = _lib.SMOOTH_RBGS  # or SMOOTH_WJACOBI (one operator per launch)
     omega: float = 0.8  # weighted-Jacobi weight
+    restriction: int = _lib.RESTRICT_HALF  # or RESTRICT_FULL (one operator per launch)
+    cycle: int = _lib.CYCLE_V  # or CYCLE_FMG (FP64, one operator per launch)
 
     @staticmethod
     def performance(mixed=False, **kw):
@@ -44,7 +46,8 @@ class Options:
 
     def to_c(self):
         return _lib.Opts(self.nu1, self.nu2, self.nu_coarse, self.precision, self.flags, self.coarse_nodes,
-                         self.max_inner, self.smoother, self.omega)
+                         self.max_inner, self.smoother, self.omega, self.restriction,
+                         self.cycle)
 
 
 def _sample_rhs(f_ksi, nx, ny, hx, hy):
