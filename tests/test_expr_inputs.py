@@ -130,3 +130,17 @@ def test_pow2_sizing_of_the_multigrid_method():
     # the usage line needs no GPU
     r = subprocess.run([host_lib.SOLVE_PATH], capture_output=True, text=True)
     assert r.returncode == 2 and "usage" in r.stderr
+
+
+@pytest.mark.parametrize("args,msg", [(["--smoother", "sor"], "unknown smoother"),
+                                      (["--smoother", "jacobi:0"], "omega"),
+                                      (["--smoother", "jacobi:1.5"], "omega"),
+                                      (["--smoother", "jacobi:0.8x"], "omega"),
+                                      (["--restriction", "quarter"], "unknown restriction"),
+                                      (["--cycle", "2"], "nu1,nu2"),
+                                      (["--fmg", "--bogus"], "unknown option")])
+def test_solve_cli_rejects_bad_options_before_touching_a_device(args, msg):
+    import subprocess
+    r = subprocess.run([host_lib.SOLVE_PATH, os.path.join(INPUTS, "Poisson1.txt")] + args,
+                       capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and msg in r.stderr, r.stderr

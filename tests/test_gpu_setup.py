@@ -152,3 +152,20 @@ def test_solve_cli_on_poisson1():
     rec = json.loads(out.stdout)
     _say(out.stdout.strip())
     assert rec["converged"] and rec["segments"] == [128, 256] and rec["cycles"] <= 10
+
+
+@pytest.mark.parametrize("args", [["--fmg"], ["--restriction", "full"], ["--smoother", "jacobi:0.8"],
+                                  ["--fmg", "--restriction", "full", "--smoother", "jacobi"]])
+def test_solve_cli_scheme_options(args):
+    """The additions (FMG iterations, full weighting, weighted Jacobi) through
+    the executable: the reference's update-norm loop converges (oracle
+    counts from the mean guess: FMG 5, full weighting 11, Jacobi 23, all
+    three 11, against the reference's 9) and the solution equals the
+    quadratic analytic one to the solver's tolerance."""
+    out = subprocess.run([host_lib.SOLVE_PATH, os.path.join(INPUTS, "Poisson1.txt"), "--init", "wlinear",
+                          "--max-cycles", "100"] + args, capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    rec = json.loads(out.stdout)
+    _say(" ".join(args) + " " + out.stdout.strip())
+    assert rec["converged"] and rec["segments"] == [128, 256] and rec["cycles"] <= 40
+    assert max(abs(rec["diff_min"]), abs(rec["diff_max"])) <= 1e-9 * max(abs(rec["min"]), abs(rec["max"]))

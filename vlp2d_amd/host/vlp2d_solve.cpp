@@ -3,9 +3,13 @@
 //
 //   vlp2d_solve <input.txt> [--init mean|amean|linear|wlinear] [--device N]
 //               [--mixed] [--cycle nu1,nu2] [--max-cycles K]
+//               [--smoother rbgs|jacobi[:omega]] [--restriction half|full] [--fmg]
 //
 // Without --cycle the scheme runs the reference's V(3,3) cycle with its update
-// norm stop (epsilonDouble of the file).
+// norm stop (epsilonDouble of the file).  --smoother jacobi (weighted Jacobi,
+// omega default 0.8), --restriction full (full weighting) and --fmg (each
+// iteration the reference's FMG, FP64) are additions to the reference's
+// scheme.
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -16,7 +20,8 @@
 int main(int argc, char **argv) {
   if (argc < 2) {
     std::fprintf(stderr, "usage: %s <input.txt> [--init mean|amean|linear|wlinear] [--device N] [--mixed] "
-                         "[--cycle nu1,nu2] [--max-cycles K]\n", argv[0]);
+                         "[--cycle nu1,nu2] [--max-cycles K] [--smoother rbgs|jacobi[:omega]] "
+                         "[--restriction half|full] [--fmg]\n", argv[0]);
     return 2;
   }
   int init = VLP2D_INIT_MEAN, device = 0, max_cycles = 0;
@@ -50,6 +55,33 @@ int main(int argc, char **argv) {
       }
     } else if (a == "--max-cycles") {
       max_cycles = std::atoi(next());
+    } else if (a == "--smoother") {
+      const std::string v = next();
+      if (v == "rbgs") {
+        o.smoother = VLP2D_SMOOTH_RBGS;
+      } else if (v == "jacobi" || v.rfind("jacobi:", 0) == 0) {
+        o.smoother = VLP2D_SMOOTH_WJACOBI;
+        if (v.size() > 7) {
+          char *end = nullptr;
+          o.omega = std::strtod(v.c_str() + 7, &end);
+          if (!end || *end || !(o.omega > 0.0 && o.omega <= 1.0)) {
+            std::fprintf(stderr, "jacobi omega must be in (0, 1]: %s\n", v.c_str());
+            return 2;
+          }
+        }
+      } else {
+        std::fprintf(stderr, "unknown smoother %s\n", v.c_str());
+        return 2;
+      }
+    } else if (a == "--restriction") {
+      const std::string v = next();
+      if (v != "half" && v != "full") {
+        std::fprintf(stderr, "unknown restriction %s\n", v.c_str());
+        return 2;
+      }
+      o.restriction = v == "full" ? VLP2D_RESTRICT_FULL : VLP2D_RESTRICT_HALF;
+    } else if (a == "--fmg") {
+      o.cycle = VLP2D_CYCLE_FMG;
     } else {
       std::fprintf(stderr, "unknown option %s\n", a.c_str());
       return 2;
