@@ -449,13 +449,18 @@ def test_adjoint_cycles_match_oracle(seg_x, seg_y, hx, hy, K):
     s.cleanup()
 
 
-def test_adjoint_exact_matches_dst_gradient():
+@pytest.mark.parametrize("kw", [{}, dict(smoother=_lib.SMOOTH_WJACOBI), dict(restriction=_lib.RESTRICT_FULL),
+                                dict(cycle=_lib.CYCLE_FMG),
+                                dict(cycle=_lib.CYCLE_FMG, restriction=_lib.RESTRICT_FULL, smoother=_lib.SMOOTH_WJACOBI)])
+def test_adjoint_exact_matches_dst_gradient(kw):
+    """The exact adjoint solves A lambda = dL/du with the handle's own cycle
+    (any smoother, restriction or FMG): the gradient is the DST solution's."""
     seg, h = 128, 1 / 128
-    _say("adjoint exact 129^2")
+    _say(f"adjoint exact 129^2 {kw}")
     rng = np.random.default_rng(8)
     ubar = np.zeros((seg + 1, seg + 1))
     ubar[1:-1, 1:-1] = rng.standard_normal((seg - 1, seg - 1))
-    s = MultiGridSchemeHIP(seg, seg, h, h, np.zeros((seg + 1, seg + 1)), options=Options.performance())
+    s = MultiGridSchemeHIP(seg, seg, h, h, np.zeros((seg + 1, seg + 1)), options=Options.performance(**kw))
     fb, gb = s.adjoint_exact(ubar, 1e-12, 60)
     lam, gex = dst_adjoint(ubar, h, h)
     assert np.abs(fb - lam).max() <= 1e-9 * np.abs(lam).max()
