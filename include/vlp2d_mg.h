@@ -21,7 +21,10 @@
  *     VisualLaplacePoisson2D/Model/CPU/StepsRecalculator.cs:168-172).
  *   - Calls on one handle are serialised by the caller (one worker thread,
  *     VisualLaplacePoisson2D/Model/CPU/VLPRectangleModel.cs:494-502), except
- *     vlp2d_mg_cancel(), which may come from any thread.
+ *     vlp2d_mg_cancel(), which may come from any thread.  Handles share no
+ *     state: different handles may be driven from different threads at once
+ *     (each has its own non-blocking stream; hipGraph capture is
+ *     thread-local).
  */
 #ifndef VLP2D_MG_H
 #define VLP2D_MG_H

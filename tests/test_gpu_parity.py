@@ -468,6 +468,91 @@ def test_adjoint_exact_matches_dst_gradient(kw):
     s.cleanup()
 
 
+# ------------------------------------------------------------ independent handles
+def _handle_cases():
+    """(name, seg_x, seg_y, hx, hy, f, g, mixed, parity_cycles) with the
+    oracle's result of each: a parity-mode run (doIteration, graph replay),
+    an FP64 solve and a two-cycle mixed solve of different shapes."""
+    cases = []
+    for name, sx, sy, seed, mixed, parity in (("parity", 64, 128, 41, False, 3), ("fp64", 128, 256, 42, False, 0),
+                                              ("mixed", 256, 128, 43, True, 0)):
+        f, g = _random(sx, sy, seed)
+        hx = hy = 1 / 256  # equal steps: point Gauss-Seidel V(2,2) diverges on strongly anisotropic grids
+        if parity:
+            o = Oracle(sx, sy, hx, hy, f)
+            o.set_u(_mean_init(g))
+            for _ in range(parity):
+                o.vcycle(3, 3, 3)
+            ref = o.get_u()
+            o.close()
+        elif mixed:
+            ref, _, _ = mixed_solve(sx, sy, hx, hy, f, g, 2, 2, 3, 1e-10, 40, inner=2)
+        else:
+            ref, _, _ = plain_solve(sx, sy, hx, hy, f, g, 2, 2, 3, 1e-10, 40)
+        cases.append((name, sx, sy, hx, hy, f, g, mixed, parity, ref))
+    return cases
+
+
+def _run_case(case):
+    name, sx, sy, hx, hy, f, g, mixed, parity, ref = case
+    if parity:
+        s = MultiGridSchemeHIP(sx, sy, hx, hy, f, eps=1e-13)
+        s.un0[:] = _mean_init(g)
+        s.init_after_boundaries_and_initial_iteration_inited()
+        for k in range(parity):
+            s.do_iteration(k)
+    else:
+        s = MultiGridSchemeHIP(sx, sy, hx, hy, f, options=Options.performance(mixed, max_inner=2 if mixed else 1))
+        s.un0[:] = g
+        s.init_after_boundaries_and_initial_iteration_inited()
+        s.solve(1e-10, 40)
+    u = s.get_array().copy()
+    s.cleanup()
+    return np.array_equal(u, ref)
+
+
+def test_independent_handles_interleaved_and_threaded():
+    """Handles share no state: three live handles with their calls
+    interleaved, then two host threads each running its own handles at the
+    same time (one in graph-replayed parity mode), all bit for bit the
+    oracle's results."""
+    import threading
+    cases = _handle_cases()
+    _say("handles interleaved")
+    hs = []
+    for name, sx, sy, hx, hy, f, g, mixed, parity, ref in cases:
+        opts = Options() if parity else Options.performance(mixed, max_inner=2 if mixed else 1)
+        s = MultiGridSchemeHIP(sx, sy, hx, hy, f, eps=1e-13, options=opts)
+        s.un0[:] = _mean_init(g) if parity else g
+        s.init_after_boundaries_and_initial_iteration_inited()
+        hs.append(s)
+    for k in range(3):  # parity cycles between the other handles' calls
+        hs[0].do_iteration(k)
+        if k == 0:
+            hs[1].solve(1e-10, 40)
+        if k == 1:
+            hs[2].solve(1e-10, 40)
+    for s, case in zip(hs, cases):
+        assert np.array_equal(s.get_array(), case[-1]), case[0]
+        s.cleanup()
+    _say("handles on two threads")
+    ok = {}
+
+    def worker(tid):
+        try:
+            ok[tid] = [_run_case(cases[(tid + r) % len(cases)]) for r in range(4)]
+        except Exception as e:  # reported by the main thread
+            ok[tid] = [repr(e)]
+
+    ts = [threading.Thread(target=worker, args=(t,)) for t in range(2)]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join(timeout=100)
+    assert not any(t.is_alive() for t in ts), "worker thread did not finish"
+    assert ok == {0: [True] * 4, 1: [True] * 4}, ok
+
+
 # ------------------------------------------------------------ error handling
 def test_errors_are_reported_not_faulted():
     f, g = _random(16, 16, 3)
