@@ -553,6 +553,46 @@ def test_independent_handles_interleaved_and_threaded():
     assert ok == {0: [True] * 4, 1: [True] * 4}, ok
 
 
+@pytest.mark.parametrize("mixed", [False, True])
+def test_cancel_from_another_thread_ends_a_running_solve(mixed):
+    """vlp2d_mg_cancel from another thread while vlp2d_mg_solve runs (rtol 0,
+    so only the cancel can end it before max_cycles): the solve returns
+    VLP2D_E_CANCELLED at its next step boundary, and after a new upload the
+    handle solves normally."""
+    import threading
+    import time
+    p = problems.config(3, 4096)
+    _say(f"cancel from a thread, 4097^2 mixed={mixed}")
+    s = MultiGridSchemeHIP(p.seg_x, p.seg_y, p.hx, p.hy, p.f, options=Options.performance(mixed, max_inner=2))
+    s.un0[:] = p.initial("zero")
+    s.init_after_boundaries_and_initial_iteration_inited()
+    s.solve(1e-10, 50)  # warm: programs and scratch set up
+    s.init_after_boundaries_and_initial_iteration_inited()
+    out = {}
+
+    def run():
+        t0 = time.perf_counter()
+        try:
+            out["st"] = s.solve(0.0, 20000, raise_on_notconv=False)
+        except VLP2DError as e:
+            out["code"] = e.code
+        out["t"] = time.perf_counter() - t0
+
+    th = threading.Thread(target=run)
+    th.start()
+    time.sleep(0.3)
+    t_cancel = time.perf_counter()
+    s.cancel_iterations()
+    th.join(timeout=60)
+    assert not th.is_alive()
+    assert out.get("code") == _lib.E_CANCELLED, out
+    assert 0.25 < out["t"] < 0.3 + (time.perf_counter() - t_cancel) + 0.5, out
+    s.init_after_boundaries_and_initial_iteration_inited()  # upload clears the request
+    st = s.solve(1e-10, 50)
+    assert st["rel_residual"] <= 1e-10 and st["cycles"] <= 8
+    s.cleanup()
+
+
 # ------------------------------------------------------------ error handling
 def test_errors_are_reported_not_faulted():
     f, g = _random(16, 16, 3)
