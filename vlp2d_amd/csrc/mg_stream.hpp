@@ -31,8 +31,10 @@
 // reference's multigrid.  Rows and columns within distance s of the strip
 // edge hold partial results after half-sweep s; the halo H covers them.
 //
-// A tile whose staged nodes are all interior nodes of the domain (most of
-// them) runs a body without boundary predicates.
+// Every tile runs a body that predicates its loads and keeps Dirichlet nodes
+// fixed.  Only the plain ascent (EPI 0) also holds a body without boundary
+// predicates, for tiles whose staged nodes are all interior nodes of the
+// domain; in the other kernels the predicated body alone is faster.
 //
 // Kernels:
 //   k_sdown : [zero or staged iterate] -> S = 2*NS half-sweeps -> residual
