@@ -119,6 +119,34 @@ def test_points_min_max_and_difference():
     s.cleanup()
 
 
+@pytest.mark.parametrize("mixed", [False, True])
+def test_maximum_size_32769_on_device(mixed):
+    """The largest grid the ABI accepts (32768 x 32768 segments, 1.07e9
+    nodes): boundary, initial guess, solve and the deviation from the analytic
+    solution all on the GPU (no host copy of the grid).  The boundary is a
+    discrete-harmonic polynomial, so the discrete solution is the polynomial
+    and the deviation is the solver's error.  The mean initial guess puts the
+    initial residual in the boundary layer (||r0|| ~ 3e12), far above the FP64
+    rounding floor of the 5-point residual at h = 2^-15, so rel. residual 1e-12
+    is reachable (measured: FP64 8 cycles, deviation 1e-9; mixed 6 steps,
+    8e-11; 34 and 47 GB on the device)."""
+    seg = 32768
+    _say(f"maximum size 32769^2 mixed={mixed}")
+    s = MultiGridSchemeHIP(seg, seg, 1 / seg, 1 / seg, None,
+                           options=Options.performance(mixed, nu1=1, nu2=3, max_inner=2))
+    s.set_boundary_expr(boundary=POLY)
+    s.init_guess(_lib.INIT_MEAN)
+    st = s.solve(1e-12, 40)
+    dmin, dmax = s.difference(POLY)
+    inf = s.info()
+    _say(f"  cycles {st['cycles']} V-cycles {st['vcycles']} rel {st['rel_residual']:.2e} "
+         f"{st['seconds'] * 1e3:.1f} ms deviation [{dmin:.2e}, {dmax:.2e}] peak {inf['device_peak_bytes'] / 1e9:.1f} GB")
+    assert st["rel_residual"] <= 1e-12 and st["cycles"] <= (10 if mixed else 12)
+    assert max(abs(dmin), abs(dmax)) <= 1e-8
+    assert (inf["nx"], inf["ny"]) == (32769, 32769) and inf["device_peak_bytes"] < 64e9
+    s.cleanup()
+
+
 @pytest.mark.parametrize("name,kinds", [("Laplace1", [0, 1, 2, 3]), ("Poisson1", [0, 1, 2, 3]),
                                          ("Laplace1_01K", [0, 1, 2, 3])])
 def test_reference_input_files(name, kinds):
