@@ -468,6 +468,42 @@ def test_adjoint_exact_matches_dst_gradient(kw):
     s.cleanup()
 
 
+@pytest.mark.parametrize("seg_x,seg_y", [(4, 2048), (2048, 4), (4, 8192)])
+def test_extreme_aspect_ratios_bitwise(seg_x, seg_y):
+    """Two-level hierarchies of extreme shape: one column strip or one row
+    tile at level 0, and a coarsest level of 3 x 1025 (LDS above 64 KB), 1025
+    x 3 or 3 x 4097 nodes (too large for LDS: the global-memory coarse
+    kernel).  Parity-mode cycles, and 4 FP64 cycles and 3 mixed steps at
+    rtol 0, bit for bit the oracle's."""
+    _say(f"aspect {seg_x}x{seg_y}")
+    hx, hy = 1 / seg_x, 1 / seg_y
+    f, g = _random(seg_x, seg_y, 51 + seg_x)
+    u0 = _mean_init(g)
+    o = Oracle(seg_x, seg_y, hx, hy, f)
+    o.set_u(u0)
+    s = MultiGridSchemeHIP(seg_x, seg_y, hx, hy, f, eps=1e-13)
+    s.un0[:] = u0
+    s.init_after_boundaries_and_initial_iteration_inited()
+    for k in range(3):
+        m_ref = o.vcycle(3, 3, 3)
+        s.do_iteration(k)
+        assert np.array_equal(s.get_array(), o.get_u()), (k, np.abs(s.get_array() - o.get_u()).max())
+        assert s.last_metric == m_ref
+    s.cleanup()
+    for mixed, n in ((False, 4), (True, 3)):
+        if mixed:
+            ur, kr, _ = mixed_solve(seg_x, seg_y, hx, hy, f, g, 2, 2, 3, 0.0, n, inner=2)
+        else:
+            ur, kr, _ = plain_solve(seg_x, seg_y, hx, hy, f, g, 2, 2, 3, 0.0, n)
+        s = MultiGridSchemeHIP(seg_x, seg_y, hx, hy, f, options=Options.performance(mixed, max_inner=2))
+        s.un0[:] = g
+        s.init_after_boundaries_and_initial_iteration_inited()
+        st = s.solve(0.0, n, raise_on_notconv=False)
+        assert st["rc"] == _lib.E_NOTCONV and st["cycles"] == kr == n
+        assert np.array_equal(s.get_array(), ur), (mixed, np.abs(s.get_array() - ur).max())
+        s.cleanup()
+
+
 # ------------------------------------------------------------ independent handles
 def _handle_cases():
     """(name, seg_x, seg_y, hx, hy, f, g, mixed, parity_cycles) with the
