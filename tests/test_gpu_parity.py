@@ -504,6 +504,24 @@ def test_extreme_aspect_ratios_bitwise(seg_x, seg_y):
         s.cleanup()
 
 
+@pytest.mark.parametrize("mixed", [False, True])
+def test_divergent_solve_reports_nonfinite(mixed):
+    """Point Gauss-Seidel V(2,2) diverges on a strongly anisotropic grid
+    ((hy/hx)^2 = 16; the oracle diverges as well, FP64 by about 4x per cycle,
+    the mixed steps faster until FP32 overflows at step 30): the solve ends
+    with VLP2D_E_NONFINITE once the residual is no longer finite, rather than
+    running on or returning garbage."""
+    _say(f"divergent solve mixed={mixed}")
+    f, g = _random(256, 128, 43)
+    s = MultiGridSchemeHIP(256, 128, 1 / 256, 2 / 128, f, options=Options.performance(mixed, max_inner=2))
+    s.un0[:] = g
+    s.init_after_boundaries_and_initial_iteration_inited()
+    with pytest.raises(VLP2DError) as e:
+        s.solve(1e-10, 1000)
+    assert e.value.code == _lib.E_NONFINITE
+    s.cleanup()
+
+
 # ------------------------------------------------------------ independent handles
 def _handle_cases():
     """(name, seg_x, seg_y, hx, hy, f, g, mixed, parity_cycles) with the
