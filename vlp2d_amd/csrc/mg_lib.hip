@@ -647,17 +647,21 @@ void reduce_part(Ctx *c, const double *part, int np, int slot, bool mx) {
   });
 }
 
-void maxdiff(Ctx *c, Lvl<double> &v, const double *snap, int slot) {
+// max |u - snap| over interior nodes -> dscal[0]; pub: published for the
+// host (publish_wait)
+void maxdiff(Ctx *c, Lvl<double> &v, const double *snap, bool pub) {
   dim3 g = grid_red(v.ny, v.nx);
   int np = (int)(g.x * g.y);
   launch(c, KC_MAXDIFF, 0, 16.0 * nodes(v), [&] {
     hipLaunchKernelGGL(k_absdiff_max<double>, g, kBlock, 0, c->st, v.u, snap, v.nx, v.ny, v.pitch, c->part);
   });
-  reduce_part(c, c->part, np, slot, true);
+  if (pub) finish(c, c->part, np, nullptr, 0, true);
+  else reduce_part(c, c->part, np, 0, true);
 }
 
 // FP64 cycle on H; epi: EPI_NONE, EPI_RESNORM (||r||^2 -> dscal[1]) or
-// EPI_METRIC (update norm -> dscal[0]).
+// EPI_METRIC (update norm -> dscal[0]); pub: the result is published for the
+// host (publish_wait).
 void cycle64(Ctx *c, Hier<double> &H, int epi, bool pub = false) {
   Lvl<double> &P = H.lv[0];
   MixView mx{};
@@ -667,12 +671,15 @@ void cycle64(Ctx *c, Hier<double> &H, int epi, bool pub = false) {
     else if (c->simple) vcycle_simple(c, H, false, snap);
     else vcycle_fused(c, H, false, EPI_NONE, mx, snap);
     if (epi == EPI_RESNORM) resnorm(c, P, P.u, nullptr, pub);
-    if (epi == EPI_METRIC) maxdiff(c, P, c->snap, 0);
+    if (epi == EPI_METRIC) maxdiff(c, P, c->snap, pub);
     return;
   }
   const int np = vcycle_fused(c, H, false, epi, mx, nullptr);
   if (epi == EPI_RESNORM) finish(c, nullptr, 0, c->part, np, pub);
-  if (epi == EPI_METRIC) reduce_part(c, c->part, np, 0, true);
+  if (epi == EPI_METRIC) {
+    if (pub) finish(c, c->part, np, nullptr, 0, true);
+    else reduce_part(c, c->part, np, 0, true);
+  }
 }
 
 // Level 0 between the two FP32 cycles of a mixed step, one launch: the first
@@ -777,7 +784,7 @@ void swap_mixed(Ctx *c, Lvl<double> &P) {
 }
 
 // ---- graph cache ---------------------------------------------------------
-constexpr int kPubKey = 1 << 20;  // keys of the solve loop's programs, which end by publishing their result
+constexpr int kPubKey = 1 << 20;  // added to the keys of the solve loop's programs
 // Programs of the solve loop (key >= kPubKey) launch their kernels directly
 // unless VLP2D_FLAG_GRAPH_SOLVE: the loop keeps one step launched ahead, and
 // graph replay measured slower there (a graph's launch stalls the queue in
@@ -1497,14 +1504,16 @@ int vlp2d_mg_cycle(vlp2d_mg_handle c, double *stop_metric) {
       HIPCHK(c, hipGetLastError());
       c->r32_valid = true;
     }
-    RCHK(run_program(c, gkey(PG_MIXED, P.par, c->h32.lv[0].rpar), [&] { mixed_step(c, P); }));
+    RCHK(run_program(c, gkey(PG_MIXED, P.par, c->h32.lv[0].rpar), [&] { mixed_step(c, P, 1, true); }));
     swap_mixed(c, P);
   } else {
     RCHK(complete_hier(c, c->h64));
     RCHK(ensure_aux(c));
-    RCHK(run_program(c, gkey(PG_CYCLE_METRIC, P.par, 0), [&] { cycle64(c, c->h64, EPI_METRIC); }));
+    RCHK(run_program(c, gkey(PG_CYCLE_METRIC, P.par, 0), [&] { cycle64(c, c->h64, EPI_METRIC, true); }));
   }
-  RCHK(sync_read(c, 2));
+  // the program's last kernel publishes the metric in mapped host memory:
+  // no copy and no stream synchronisation per call
+  RCHK(publish_wait(c));
   if (stop_metric) *stop_metric = c->hscal[0];
   if (std::isnan(c->hscal[0])) return fail(c, VLP2D_E_NONFINITE, "iterate became non-finite");
   return VLP2D_OK;
