@@ -1428,7 +1428,7 @@ static int create_impl(int segX, int segY, double hx, double hy, const double *f
     // partials: one per tile of the largest launch.  Streaming tiles number
     // at most max(resident wavefronts, column strips), the strips being
     // narrowest for the widest halo (ascent with a recomputed pre-sweep, 3
-    // post-sweeps and a residual epilogue); refinement tiles are 512 x 64.
+    // post-sweeps and a residual epilogue); refinement tiles are 512 x RS_ROWS.
     constexpr int kMinStripCols = SCOLS - 2 * s_up_halo(4, 3);
     const long long strips = (ny - 2) / kMinStripCols + 1;
     long long stiles = std::max<long long>(strips, 64LL * std::max(1, c->num_cu));
