@@ -2,21 +2,27 @@
 
     python -m vlp2d_amd.build
 
-lib/libvlp2d_mg.so: the gfx950 solver, one hipcc translation unit
-(csrc/mg_lib.hip and the headers it includes).  -ffp-contract=off keeps every
-product and sum separately rounded, as written, so device results are
-reproducible operation by operation; packed-FP32 instructions are disabled.
+lib/libvlp2d_mg.so: the gfx950 solver.  csrc/mg_lib.hip holds the C ABI and
+the small kernels; the streaming kernels are instantiated in one translation
+unit per precision and sweep count (csrc/mg_k_down_*.hip, csrc/mg_k_up_*.hip),
+all compiled in parallel with hipcc -c and linked into one shared library.
+-ffp-contract=off keeps every product and sum separately rounded, as written,
+so device results are reproducible operation by operation; packed-FP32
+instructions are disabled.
 
 lib/libvlp2d_host.so and lib/vlp2d_solve: the C++ host layer (host/: input
 files, scheme object, model loop) over the solver's C ABI, built with g++.
 """
+import glob
 import os
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-SRC = os.path.join(HERE, "csrc", "mg_lib.hip")
+CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "lib")
+OBJ_DIR = os.path.join(OUT_DIR, "obj")
 OUT = os.path.join(OUT_DIR, "libvlp2d_mg.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
@@ -25,7 +31,6 @@ FLAGS = [
     "-O3",
     "-std=c++17",
     "-fPIC",
-    "-shared",
     "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall",
@@ -63,10 +68,35 @@ def build_host(verbose=False):
     return HOST_OUT
 
 
+def sources():
+    """Translation units of the solver library (the ABI unit first)."""
+    tus = sorted(glob.glob(os.path.join(CSRC, "mg_k_*.hip")))
+    return [os.path.join(CSRC, "mg_lib.hip")] + tus
+
+
+def _jobs():
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 4
+    return max(1, min(n, 16))
+
+
 def build(verbose=False):
-    os.makedirs(OUT_DIR, exist_ok=True)
+    """Compiles every translation unit (in parallel) and links the library."""
+    os.makedirs(OBJ_DIR, exist_ok=True)
+    srcs = sources()
+    objs = [os.path.join(OBJ_DIR, os.path.basename(p)[:-4] + ".o") for p in srcs]
+
+    def one(pair):
+        src, obj = pair
+        _run([HIPCC] + FLAGS + ["-c", "-o", obj + ".tmp", src], verbose)
+        os.replace(obj + ".tmp", obj)
+
+    with ThreadPoolExecutor(max_workers=_jobs()) as ex:
+        list(ex.map(one, zip(srcs, objs)))
     tmp = OUT + ".tmp"
-    _run([HIPCC] + FLAGS + ["-o", tmp, SRC], verbose)
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs, verbose)
     os.replace(tmp, OUT)
     build_host(verbose)
     return OUT

@@ -28,12 +28,11 @@
 #include <vector>
 
 #include "../../include/vlp2d_mg.h"
-#include "mg_kernels.hpp"
-#include "mg_tiles.hpp"
+#include "mg_ctx.hpp"
 #include "mg_coarse.hpp"
 #include "mg_refine.hpp"
-#include "mg_stream.hpp"
 #include "mg_ltile.hpp"
+#include "mg_misc_kernels.hpp"
 #include "mg_setup.hpp"
 
 
@@ -43,111 +42,10 @@ namespace {
 
 thread_local std::string g_create_error;
 
-enum KClass {
-  KC_SMOOTH,
-  KC_RESID,
-  KC_RESTRICT,
-  KC_FILL,
-  KC_PROLONG,
-  KC_COPY,
-  KC_MAXDIFF,
-  KC_RESNORM,
-  KC_REDUCE,
-  KC_UPDATE,
-  KC_ADJ,
-  KC_DOWN,
-  KC_UP,
-  KC_COARSE,
-  KC_SETUP,
-  KC_UP_LDS,
-  KC_UP_REFINE,
-  KC_UP_DOWN,
-  KC_COUNT
-};
 const char *kKClassName[KC_COUNT] = {"rb_half",   "residual",     "restrict", "fill",    "prolong",
                                      "copy_interior", "absdiff_max", "resnorm64", "reduce", "update_mixed",
                                      "adjoint",   "down_fused",   "up_fused", "coarse_vcycle", "setup",
                                      "up_lds",    "up_fused_refine", "up_down_fused"};
-
-// Epilogues of the level-0 ascent.
-enum Epi { EPI_NONE = 0, EPI_RESNORM = 1, EPI_METRIC = 2, EPI_MIXED = 3 };
-
-template <typename T> struct Lvl {
-  T *u = nullptr, *u2 = nullptr;      // iterate and its second buffer
-  T *rhs = nullptr, *rhs2 = nullptr;  // right-hand side (rhs2: mixed level 0)
-  T *res = nullptr;                   // residual (one-operator-per-launch path)
-  int nx = 0, ny = 0, pitch = 0;
-  T hx2 = 0, hy2 = 0, coef = 0;
-  bool eq = true, lap = false;
-  T ihx2 = 0, ihy2 = 0;          // 1/hx^2, 1/hy^2 in T (exact when pw2)
-  bool pw2 = false;              // hx^2 and hy^2 are powers of two
-  double ix2 = 0, iy2 = 0;       // 1/hx^2, 1/hy^2 in double (adjoint)
-  double ax = 0, ay = 0, b = 0;  // smoother coefficients in double (adjoint)
-  int par = 0, rpar = 0;         // swap parity of u/u2 and rhs/rhs2
-  int tiles_i = 0, tiles_j = 0, chunk = 0;
-  size_t elems() const { return (size_t)nx * pitch; }
-};
-
-template <typename T> struct Hier {
-  std::vector<Lvl<T>> lv;
-  bool full = false;  // every buffer of every level allocated
-  int lc = 0;         // first level handled by the one-workgroup kernel
-};
-
-struct PendingEv {
-  int kc, lvl;
-  hipEvent_t a, b;
-};
-constexpr int kMaxLv = 16;
-
-}  // namespace
-
-struct vlp2d_mg_ctx {
-  int dev = 0;
-  hipStream_t st = nullptr;
-  int segX = 0, segY = 0, nx = 0, ny = 0, L = 0;
-  double hx = 0, hy = 0, eps = 0;
-  bool laplace = false;
-  bool simple = false;  // one operator per launch
-  vlp2d_mg_opts o{};
-  Hier<double> h64;  // the problem: level 0 u (with boundary), rhs = fn
-  Hier<float> h32;   // FP32 correction hierarchy (mixed precision)
-  Hier<double> ha;   // adjoint-solve hierarchy (exact adjoint)
-  double *snap = nullptr;
-  double *saved = nullptr;
-  double *part = nullptr, *part2 = nullptr;
-  int part_cap = 0;
-  double *dscal = nullptr;  // device scalars
-  double *hscal = nullptr;  // pinned host scalars
-  HostPub *hpub = nullptr;  // mapped pinned ring of kPubRing results the solve loop polls
-  HostPub *hpub_dev = nullptr;
-  unsigned long long *dseq = nullptr;  // device count of published results
-  unsigned long long pub_seq = 0;      // results published so far
-  bool r32_valid = false;   // h32 level-0 rhs holds (float)(fn - Lap u64)
-  // transposed-cycle buffers
-  std::vector<double *> aub, arb, aresb;
-  std::vector<double *> ckpt;
-  int K = 0;
-  std::map<int, std::pair<hipGraph_t, hipGraphExec_t>> graphs;
-  int num_cu = 0;                    // compute units of the device
-  int max_lds = 65536;               // dynamic LDS bytes one workgroup may use
-  ExprOp *dprog = nullptr;           // device copies of compiled expressions (kProgSlots programs)
-  double bmin = NAN, bmax = NAN;     // min / max of the last boundary sampled on the device
-  std::map<const void *, int> occ;   // resident 64-thread workgroups per GPU, per kernel
-  // profiling
-  bool profile = false;
-  std::vector<PendingEv> pend;
-  long long kc_launch[KC_COUNT][kMaxLv] = {};
-  double kc_ms[KC_COUNT][kMaxLv] = {};
-  double kc_bytes[KC_COUNT][kMaxLv] = {};
-  std::atomic<int> cancel{0};
-  std::string err;
-  long long bytes = 0, peak = 0;
-};
-
-namespace {
-
-using Ctx = vlp2d_mg_ctx;
 
 #define HIPCHK(ctx, call)                                                \
   do {                                                                   \
@@ -368,62 +266,10 @@ void fmg_simple(Ctx *c, Hier<double> &H, double *snap) {
   vcycle_simple(c, H, false, snap, 0);
 }
 
-// ---- fused kernels -------------------------------------------------------------
-// Operands of one fused launch (the tile grid is set by sview).
-template <typename T> LevelView<T> view(const Lvl<T> &v, const T *uin, T *uout, int H) {
-  LevelView<T> a;
-  a.uin = uin;
-  a.uout = uout;
-  a.f = v.rhs;
-  a.nx = v.nx;
-  a.ny = v.ny;
-  a.pitch = v.pitch;
-  a.hx2 = v.hx2;
-  a.hy2 = v.hy2;
-  a.coef = v.coef;
-  a.eq = v.eq ? 1 : 0;
-  a.lap = v.lap ? 1 : 0;
-  a.ihx2 = v.ihx2;
-  a.ihy2 = v.ihy2;
-  a.pw2 = v.pw2 ? 1 : 0;
-  a.rc = nullptr;
-  a.tiles_i = a.tiles_j = a.chunk = a.rows = 0;
-  return a;
-}
 
 bool is_pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
 
-// Resident 64-thread workgroups (wavefronts) of a kernel on the whole GPU.
-int resident_waves(Ctx *c, const void *fn) {
-  auto it = c->occ.find(fn);
-  if (it != c->occ.end()) return it->second;
-  int nb = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, 64 * kWaves, 0) != hipSuccess || nb < 1) {
-    (void)hipGetLastError();
-    nb = 4;
-  }
-  const int q = nb * kWaves * std::max(1, c->num_cu);
-  c->occ[fn] = q;
-  return q;
-}
 
-// Streaming-kernel view: tiles of R output rows by 128 - 2H columns, R
-// chosen so that the tiles fill the resident wavefront slots about once (one
-// wave of tiles: each staged halo row is re-read once per tile boundary, and
-// no tail of late tiles).  The staged rows R + 2H are a multiple of P.
-template <typename T>
-LevelView<T> sview(Ctx *c, const void *fn, const Lvl<T> &v, const T *uin, T *uout, int H, int P) {
-  LevelView<T> a = view(v, uin, uout, H);
-  const int tj = (v.ny - 2) / (SCOLS - 2 * H) + 1;
-  const int ti = std::max(1, resident_waves(c, fn) / tj);
-  const int R = s_rows_at_least((v.nx - 2) / ti + 1, H, P);
-  const STileGrid g = s_grid(v.nx, v.ny, R, H);
-  a.rows = R;
-  a.tiles_i = g.tiles_i;
-  a.tiles_j = g.tiles_j;
-  a.chunk = g.chunk;
-  return a;
-}
 
 // A zero-initialised level pre-smoothed with one sweep is not stored by the
 // descent; the ascent recomputes it from the right-hand side (k_sup N1 = 1).
@@ -449,18 +295,6 @@ template <typename T> bool lds_ascent(const Ctx *c, const Hier<T> &H, int l) {
          !(c->o.flags & VLP2D_FLAG_STREAM_SMALL);
 }
 
-template <typename T, int NS>
-void launch_down_ns(Ctx *c, Lvl<T> &f, Lvl<T> &g, bool zi, bool store) {
-  constexpr int H = s_down_halo(NS), P = s_period(NS);
-  using K = void (*)(LevelView<T>, T *, int, int, int);
-  K k;
-  if (!zi && store) k = f.eq ? (K)k_sdown<T, NS, false, true> : (K)k_sdown<T, NS, false, false>;
-  else if (!zi) k = f.eq ? (K)k_sdown<T, NS, false, true, false> : (K)k_sdown<T, NS, false, false, false>;
-  else if (store) k = f.eq ? (K)k_sdown<T, NS, true, true> : (K)k_sdown<T, NS, true, false>;
-  else k = f.eq ? (K)k_sdown<T, NS, true, true, false> : (K)k_sdown<T, NS, true, false, false>;
-  const LevelView<T> a = sview(c, (const void *)k, f, f.u, f.u2, H, P);
-  hipLaunchKernelGGL(k, dim3(8 * a.chunk), dim3(64 * kWaves), 0, c->st, a, g.rhs, g.nx, g.ny, g.pitch);
-}
 
 template <typename T> void down(Ctx *c, Lvl<T> &f, Lvl<T> &g, int lvl, bool zi, bool store) {
   const double s = sizeof(T);
@@ -474,37 +308,6 @@ template <typename T> void down(Ctx *c, Lvl<T> &f, Lvl<T> &g, int lvl, bool zi, 
   });
 }
 
-template <typename T, int NS, int EPI, int N1, bool ZS = true>
-int launch_sup(Ctx *c, Lvl<T> &f, const Lvl<T> &g, const MixView &mx) {
-  constexpr int H = s_up_halo(N1 + NS, EPI), P = s_period(N1 + NS);
-  using K = void (*)(LevelView<T>, const T *, int, int, int, double *, MixView);
-  const K k = f.eq ? (K)k_sup<T, NS, EPI, true, N1, ZS> : (K)k_sup<T, NS, EPI, false, N1, ZS>;
-  // a recomputed non-zero start is staged from the descent's input f.u
-  const LevelView<T> a = sview(c, (const void *)k, f, ZS ? f.u2 : f.u, f.u, H, P);
-  hipLaunchKernelGGL(k, dim3(8 * a.chunk), dim3(64 * kWaves), 0, c->st, a, g.u, g.nx, g.ny, g.pitch, c->part, mx);
-  return a.tiles_i * a.tiles_j;
-}
-
-// FP64 levels run epilogues none / residual norm / update norm; FP32 levels
-// (the correction cycle of a mixed step) none / fused refinement.  rz: the
-// level's pre-smoothed iterate is recomputed (one pre-smoothing sweep).
-template <typename T, int NS>
-int launch_up_ns(Ctx *c, Lvl<T> &f, const Lvl<T> &g, int epi, bool rz, bool zi, const MixView &mx) {
-  if constexpr (sizeof(T) == 8) {
-    switch (epi) {
-      case EPI_RESNORM:
-        return rz ? launch_sup<T, NS, EPI_RESNORM, 1>(c, f, g, mx) : launch_sup<T, NS, EPI_RESNORM, 0>(c, f, g, mx);
-      case EPI_METRIC: return launch_sup<T, NS, EPI_METRIC, 0>(c, f, g, mx);
-      default: return rz ? launch_sup<T, NS, EPI_NONE, 1>(c, f, g, mx) : launch_sup<T, NS, EPI_NONE, 0>(c, f, g, mx);
-    }
-  } else {
-    if (epi == EPI_MIXED) {
-      if (rz && !zi) return launch_sup<T, NS, EPI_MIXED, 1, false>(c, f, g, mx);
-      return rz ? launch_sup<T, NS, EPI_MIXED, 1>(c, f, g, mx) : launch_sup<T, NS, EPI_MIXED, 0>(c, f, g, mx);
-    }
-    return rz ? launch_sup<T, NS, EPI_NONE, 1>(c, f, g, mx) : launch_sup<T, NS, EPI_NONE, 0>(c, f, g, mx);
-  }
-}
 
 // returns the number of partials the epilogue writes
 template <typename T>
@@ -682,18 +485,6 @@ void cycle64(Ctx *c, Hier<double> &H, int epi, bool pub = false) {
   }
 }
 
-// Level 0 between the two FP32 cycles of a mixed step, one launch: the first
-// cycle's ascent (pre-smoothed iterate recomputed from zero, prolongation,
-// nu2 sweeps, e -> z.u) continues into the second cycle's descent (one sweep
-// on e, residual, restriction into level 1's right-hand side).
-template <int NS> void launch_up_down0_ns(Ctx *c, Lvl<float> &f, Lvl<float> &g) {
-  constexpr int NB = 1, H = s_up_halo(1 + NS + NB, 4), P = s_period(1 + NS + NB);
-  using K = void (*)(LevelView<float>, const float *, int, int, int, double *, MixView);
-  const K k = f.eq ? (K)k_sup<float, NS, 4, true, 1, true, NB> : (K)k_sup<float, NS, 4, false, 1, true, NB>;
-  LevelView<float> a = sview(c, (const void *)k, f, f.u2, f.u, H, P);
-  a.rc = g.rhs;
-  hipLaunchKernelGGL(k, dim3(8 * a.chunk), dim3(64 * kWaves), 0, c->st, a, g.u, g.nx, g.ny, g.pitch, c->part, MixView{});
-}
 void up_down0(Ctx *c, Lvl<float> &f, Lvl<float> &g) {
   launch(c, KC_UP_DOWN, 0, 4.0 * (2.0 * nodes(f) + 2.0 * nodes(g)), [&] {
     switch (c->o.nu2) {
