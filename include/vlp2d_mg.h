@@ -173,9 +173,11 @@ int vlp2d_mg_solve(vlp2d_mg_handle h, double rtol, int max_cycles, vlp2d_mg_stat
 int vlp2d_mg_adjoint_exact(vlp2d_mg_handle h, const double *dL_du, double rtol, int max_cycles, double *dL_df,
                            double *dL_dg);
 
-/* Runs K FP64 V(nu1,nu2) cycles from the current iterate in Poisson form,
- * storing one device checkpoint of u per cycle, so that
- * vlp2d_mg_adjoint_cycles can differentiate u_K = Phi^K(u_0; f, g). */
+/* Runs K FP64 V(nu1,nu2) cycles from the current iterate in Poisson form and
+ * records K, so that vlp2d_mg_adjoint_cycles can differentiate
+ * u_K = Phi^K(u_0; f, g).  One V-cycle is an affine map of (u, f, g), so the
+ * transposed sweep needs no forward state: no iterate is stored per cycle
+ * and the call allocates nothing beyond the solver's own hierarchy. */
 int vlp2d_mg_forward_cycles(vlp2d_mg_handle h, int K);
 
 /* Transposed sweep through the K cycles recorded by vlp2d_mg_forward_cycles.

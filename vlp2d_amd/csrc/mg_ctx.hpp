@@ -101,8 +101,7 @@ struct vlp2d_mg_ctx {
   bool r32_valid = false;   // h32 level-0 rhs holds (float)(fn - Lap u64)
   // transposed-cycle buffers
   std::vector<double *> aub, arb, aresb;
-  std::vector<double *> ckpt;
-  int K = 0;
+  int K = 0;  // forward cycles recorded for the transposed sweep
   std::map<int, std::pair<hipGraph_t, hipGraphExec_t>> graphs;
   int num_cu = 0;                    // compute units of the device
   int max_lds = 65536;               // dynamic LDS bytes one workgroup may use

@@ -1306,7 +1306,9 @@ int vlp2d_mg_cycle(vlp2d_mg_handle c, double *stop_metric) {
   // no copy and no stream synchronisation per call
   RCHK(publish_wait(c));
   if (stop_metric) *stop_metric = c->hscal[0];
-  if (std::isnan(c->hscal[0])) return fail(c, VLP2D_E_NONFINITE, "iterate became non-finite");
+  // the metric is a max over the iterate's update: NaN or +Inf in the
+  // iterate reaches it (the max reduction propagates NaN)
+  if (!std::isfinite(c->hscal[0])) return fail(c, VLP2D_E_NONFINITE, "iterate became non-finite");
   return VLP2D_OK;
 }
 
@@ -1390,15 +1392,11 @@ int vlp2d_mg_forward_cycles(vlp2d_mg_handle c, int K) {
                 "the truncated-cycle adjoint transposes the red-black Gauss-Seidel, half-weighting V-cycle only");
   HIPCHK(c, hipSetDevice(c->dev));
   RCHK(complete_hier(c, c->h64));
-  Lvl<double> &P = c->h64.lv[0];
-  while ((int)c->ckpt.size() < K) {
-    double *p = nullptr;
-    RCHK(dalloc(c, &p, P.elems()));
-    c->ckpt.push_back(p);
-  }
+  // The V-cycle is an affine map of (u, fn, g), so its transpose reads no
+  // forward state: nothing is stored per cycle, only the cycle count K that
+  // the reverse sweep transposes.
   for (int k = 0; k < K; ++k) {
     if (c->cancel.load()) return fail(c, VLP2D_E_CANCELLED, "cancelled");
-    HIPCHK(c, hipMemcpyAsync(c->ckpt[k], P.u, P.elems() * sizeof(double), hipMemcpyDeviceToDevice, c->st));
     cycle64(c, c->h64, EPI_NONE);
     HIPCHK(c, hipGetLastError());
   }
@@ -1530,7 +1528,6 @@ void vlp2d_mg_destroy(vlp2d_mg_handle c) {
     dfree(c, c->arb[l], n);
     dfree(c, c->aresb[l], n);
   }
-  for (auto &p : c->ckpt) dfree(c, p, n0);
   dfree(c, c->part, (size_t)c->part_cap);
   dfree(c, c->part2, (size_t)c->part_cap);
   dfree(c, c->dscal, 8);
