@@ -273,6 +273,37 @@ int vlp2d_mg_difference(vlp2d_mg_handle h, const char *analytic, double *dmin, d
  * vlp2d_mg_last_error(NULL). */
 int vlp2d_expr_eval(const char *expr, const double *x, const double *y, long long n, double *out);
 
+/* ---- allocation-free solve ----------------------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code run between hipGraph captures, tight serving loops, harnesses
+ * that forbid hipMalloc).  The solve of vlp2d_mg_create + vlp2d_mg_upload +
+ * vlp2d_mg_solve + vlp2d_mg_download on the same inputs and options (the
+ * same iterate, bit for bit, and the same cycle count), on caller-owned
+ * device arrays.  Every other buffer (the level hierarchy, partial sums,
+ * result scalars) is carved from one static device array of
+ * VLP2D_NOALLOC_SCRATCH_BYTES per GPU, which the library holds from its first
+ * use on that GPU until the process ends; the call itself calls no hipMalloc,
+ * hipFree or hipHostMalloc and captures no graph.
+ *   f_dev  device, PDE right-hand side f (as vlp2d_mg_create), NULL = Laplace
+ *   u_dev  device, in: Dirichlet boundary values + initial guess; out: the
+ *          iterate at the end of the solve
+ *   pitch  elements per row of both arrays: ny rounded up to a multiple of 32
+ *          (the library's row layout); rows i = 0..nx-1, 16-byte aligned
+ *   stream a hipStream_t, NULL = the null stream; the call returns after the
+ *          solve has finished on it
+ * Calls on one GPU share its scratch array, which keeps the last call's
+ * contents (every buffer is cleared before use); the library serialises
+ * them, so overlapping calls on one GPU wait for each other rather than
+ * race.  A grid whose hierarchy does not fit returns VLP2D_E_SIZE before any
+ * kernel runs (64 MiB holds 1025^2 in either precision and 2049 x 1025 in
+ * FP64).  opts as for vlp2d_mg_create (NULL = defaults); VLP2D_FLAG_PROFILE
+ * is rejected.  Returns as vlp2d_mg_solve (stats may be NULL); the message of
+ * a failure in vlp2d_mg_last_error(NULL). */
+#define VLP2D_NOALLOC_SCRATCH_BYTES (64u << 20)
+int vlp2d_mg_solve_noalloc(int segX, int segY, double hx, double hy, const double *f_dev, double *u_dev, int pitch,
+                           double rtol, int max_cycles, const vlp2d_mg_opts *opts, int device, void *stream,
+                           vlp2d_mg_stats *stats);
+
 /* Message of the last failure on h; with h == NULL, of the last failed
  * vlp2d_mg_create / vlp2d_mg_create_expr / vlp2d_expr_eval on the calling
  * thread.  Never NULL. */

@@ -89,3 +89,22 @@ def test_null_handle_calls_are_rejected():
     assert lib.vlp2d_mg_solve(None, 1e-10, 10, None) == _lib.E_ARG
     lib.vlp2d_mg_destroy(None)
     lib.vlp2d_mg_cancel(None)
+
+
+def test_noalloc_solve_checks_arguments_before_touching_a_device():
+    lib = load_library()
+    st = _lib.Stats()
+    fake = 1 << 20  # never dereferenced: every case fails before a device call
+    call = lambda sx, sy, h, u, pitch, rtol=1e-10, o=None: lib.vlp2d_mg_solve_noalloc(
+        sx, sy, h, h, None, u, pitch, rtol, 10, o, 0, None, ctypes.byref(st))
+    assert call(6, 8, 0.1, fake, 32) == _lib.E_SIZE
+    assert call(8, 8, -0.1, fake, 32) == _lib.E_ARG
+    assert call(8, 8, 0.1, None, 32) == _lib.E_ARG
+    assert call(8, 8, 0.1, fake, 9) == _lib.E_ARG and b"pitch" in lib.vlp2d_mg_last_error(None)
+    assert call(8, 8, 0.1, fake + 8, 32) == _lib.E_ARG and b"aligned" in lib.vlp2d_mg_last_error(None)
+    assert call(8, 8, 0.1, fake, 32, rtol=-1.0) == _lib.E_ARG
+    o = _lib.Opts()
+    lib.vlp2d_mg_default_opts(ctypes.byref(o))
+    o.flags = _lib.FLAG_PROFILE
+    assert call(8, 8, 0.1, fake, 32, o=ctypes.byref(o)) == _lib.E_ARG
+    assert b"PROFILE" in lib.vlp2d_mg_last_error(None)

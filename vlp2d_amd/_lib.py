@@ -34,7 +34,9 @@ EXPORTS = [
     "vlp2d_mg_reset_kernel_times", "vlp2d_mg_save_iterate", "vlp2d_mg_restore_iterate", "vlp2d_mg_cancel",
     "vlp2d_mg_destroy", "vlp2d_mg_last_error", "vlp2d_mg_create_expr", "vlp2d_mg_set_boundary_expr",
     "vlp2d_mg_init_guess", "vlp2d_mg_points_min_max", "vlp2d_mg_difference", "vlp2d_expr_eval",
+    "vlp2d_mg_solve_noalloc",
 ]
+NOALLOC_SCRATCH_BYTES = 64 << 20
 INIT_MEAN, INIT_ARITHMETIC_MEAN, INIT_LINEAR, INIT_WEIGHT_LINEAR = 0, 1, 2, 3
 
 
@@ -109,6 +111,10 @@ def load_library(path=LIB_PATH):
         "vlp2d_mg_points_min_max": (ctypes.c_int, [_h, _dp, _dp]),
         "vlp2d_mg_difference": (ctypes.c_int, [_h, ctypes.c_char_p, _dp, _dp, _dp]),
         "vlp2d_expr_eval": (ctypes.c_int, [ctypes.c_char_p, _dp, _dp, ctypes.c_longlong, _dp]),
+        "vlp2d_mg_solve_noalloc": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                                  ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_double,
+                                                  ctypes.c_int, ctypes.POINTER(Opts), ctypes.c_int, ctypes.c_void_p,
+                                                  ctypes.POINTER(Stats)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

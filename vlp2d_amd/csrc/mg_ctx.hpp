@@ -114,6 +114,15 @@ struct vlp2d_mg_ctx {
   long long kc_launch[vlp2d::KC_COUNT][vlp2d::kMaxLv] = {};
   double kc_ms[vlp2d::KC_COUNT][vlp2d::kMaxLv] = {};
   double kc_bytes[vlp2d::KC_COUNT][vlp2d::kMaxLv] = {};
+  hipEvent_t ev[2] = {nullptr, nullptr};  // timing events of solves
+  // Allocation-free calls (vlp2d_mg_solve_noalloc): device buffers are carved
+  // from the library's static device array (arena) instead of hipMalloc, and
+  // results are copied back with a stream synchronisation (nomap: no mapped
+  // host memory; hscal points at hloc).
+  char *arena = nullptr;
+  size_t arena_cap = 0, arena_used = 0;
+  bool nomap = false;
+  double hloc[8] = {};
   std::atomic<int> cancel{0};
   std::string err;
   long long bytes = 0, peak = 0;

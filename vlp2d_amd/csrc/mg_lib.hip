@@ -24,6 +24,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -68,6 +69,21 @@ int fail(Ctx *c, int code, const std::string &msg) {
 }
 
 template <typename T> int dalloc(Ctx *c, T **p, size_t n) {
+  if (c->arena) {
+    // allocation-free calls: carve 256-byte aligned pieces of the static array
+    const size_t off = (c->arena_used + 255) / 256 * 256, need = n * sizeof(T);
+    if (off + need > c->arena_cap) {
+      *p = nullptr;
+      c->err = "the grid needs more than the " + std::to_string(c->arena_cap >> 20) +
+               " MiB of device scratch the allocation-free entry point holds";
+      return VLP2D_E_SIZE;
+    }
+    *p = reinterpret_cast<T *>(c->arena + off);
+    c->arena_used = off + need;
+    c->bytes += (long long)need;
+    c->peak = std::max(c->peak, c->bytes);
+    return VLP2D_OK;
+  }
   hipError_t e = hipMalloc((void **)p, n * sizeof(T));
   if (e != hipSuccess) {
     *p = nullptr;
@@ -81,7 +97,7 @@ template <typename T> int dalloc(Ctx *c, T **p, size_t n) {
 }
 template <typename T> void dfree(Ctx *c, T *&p, size_t n) {
   if (p) {
-    (void)hipFree(p);
+    if (!c->arena) (void)hipFree(p);
     c->bytes -= (long long)(n * sizeof(T));
     p = nullptr;
   }
@@ -415,7 +431,7 @@ int vcycle_fused(Ctx *c, Hier<T> &H, bool zero0, int epi, const MixView &mx, dou
 // dscal[1] = sum of psum[0..nsum); with pmax also dscal[0] = max of
 // pmax[0..nmax); pub: then publish both for the solve loop (publish_wait).
 void finish(Ctx *c, const double *pmax, int nmax, const double *psum, int nsum, bool pub, int sum_slot = 1) {
-  HostPub *hp = pub && !c->profile ? c->hpub_dev : nullptr;
+  HostPub *hp = pub && !c->profile && !c->nomap ? c->hpub_dev : nullptr;
   launch(c, KC_REDUCE, 0, 8.0 * (nmax + nsum), [&] {
     if (pmax)
       hipLaunchKernelGGL(k_finish<true>, dim3(1), dim3(RED_THREADS), 0, c->st, pmax, nmax, psum, nsum, c->dscal,
@@ -581,7 +597,7 @@ constexpr int kPubKey = 1 << 20;  // added to the keys of the solve loop's progr
 // graph replay measured slower there (a graph's launch stalls the queue in
 // the middle of a step); single cycles replay their captured graph.
 template <typename F> int run_program(Ctx *c, int key, F &&body) {
-  bool use_graph = !(c->o.flags & VLP2D_FLAG_NO_GRAPH) && !c->profile;
+  bool use_graph = !(c->o.flags & VLP2D_FLAG_NO_GRAPH) && !c->profile && !c->nomap;
   if (key >= kPubKey && !(c->o.flags & VLP2D_FLAG_GRAPH_SOLVE)) use_graph = false;
   if (!use_graph) {
     body();
@@ -643,7 +659,7 @@ int wait_result(Ctx *c, unsigned long long want) {
 // The result of the program just launched (profiling runs, which publish
 // nothing, synchronise instead).
 int publish_wait(Ctx *c) {
-  if (c->profile) return sync_read(c, 3);
+  if (c->profile || c->nomap) return sync_read(c, 3);
   return wait_result(c, ++c->pub_seq);
 }
 
@@ -705,27 +721,32 @@ template <typename T> void shape_hier(Ctx *c, Hier<T> &H, T hx0, T hy0) {
 }
 
 // Allocate every buffer the cycles of H need; level-0 u and rhs already exist.
+// All buffers are allocated before any is cleared, so a failed allocation
+// returns before anything runs on the device.
 template <typename T> int complete_hier(Ctx *c, Hier<T> &H) {
   if (H.full) return VLP2D_OK;
+  struct New {
+    T *p;
+    size_t n;
+    bool copy0;  // level-0 second iterate buffer: a copy of u (boundary values)
+  };
+  std::vector<New> made;
+  auto need = [&](T *&p, size_t n, bool copy0 = false) -> int {
+    if (p) return VLP2D_OK;
+    RCHK(dalloc(c, &p, n));
+    made.push_back(New{p, n, copy0});
+    return VLP2D_OK;
+  };
   for (int l = 0; l < c->L; ++l) {
     Lvl<T> &v = H.lv[l];
-    if (!v.u) {
-      RCHK(dalloc(c, &v.u, v.elems()));
-      fill(c, v.u, v.elems(), T(0));
-    }
-    if (!v.rhs) {
-      RCHK(dalloc(c, &v.rhs, v.elems()));
-      fill(c, v.rhs, v.elems(), T(0));
-    }
-    if (!v.u2 && (l == 0 || l < H.lc)) {
-      RCHK(dalloc(c, &v.u2, v.elems()));
-      if (l == 0) HIPCHK(c, hipMemcpyAsync(v.u2, v.u, v.elems() * sizeof(T), hipMemcpyDeviceToDevice, c->st));
-      else fill(c, v.u2, v.elems(), T(0));
-    }
-    if (c->simple && (l < c->L - 1 || c->o.smoother == VLP2D_SMOOTH_WJACOBI) && !v.res) {
-      RCHK(dalloc(c, &v.res, v.elems()));
-      fill(c, v.res, v.elems(), T(0));
-    }
+    RCHK(need(v.u, v.elems()));
+    RCHK(need(v.rhs, v.elems()));
+    if (l == 0 || l < H.lc) RCHK(need(v.u2, v.elems(), l == 0));
+    if (c->simple && (l < c->L - 1 || c->o.smoother == VLP2D_SMOOTH_WJACOBI)) RCHK(need(v.res, v.elems()));
+  }
+  for (const New &m : made) {
+    if (m.copy0) HIPCHK(c, hipMemcpyAsync(m.p, H.lv[0].u, m.n * sizeof(T), hipMemcpyDeviceToDevice, c->st));
+    else fill(c, m.p, m.n, T(0));
   }
   HIPCHK(c, hipStreamSynchronize(c->st));
   H.full = true;
@@ -750,8 +771,9 @@ int ensure_h32(Ctx *c) {
   Lvl<float> &z = c->h32.lv[0];
   z.lap = false;
   RCHK(dalloc(c, &z.rhs2, z.elems()));
+  RCHK(complete_hier(c, c->h32));
   fill(c, z.rhs2, z.elems(), 0.0f);
-  return complete_hier(c, c->h32);
+  return VLP2D_OK;
 }
 
 int ensure_aux(Ctx *c) {
@@ -909,14 +931,15 @@ int solve_impl(Ctx *c, Hier<double> &H, int prog_fp64, int prog_mixed, double rt
   const bool mixed = c->o.precision == VLP2D_PREC_MIXED;
   if (mixed) RCHK(ensure_h32(c));
   else RCHK(complete_hier(c, H));  // mixed steps use only level 0 of H
-  hipEvent_t e0, e1;
-  HIPCHK(c, hipEventCreate(&e0));
-  HIPCHK(c, hipEventCreate(&e1));
+  // the handle's two timing events, created on its first solve
+  for (hipEvent_t &e : c->ev)
+    if (!e) HIPCHK(c, hipEventCreate(&e));
+  hipEvent_t e0 = c->ev[0], e1 = c->ev[1];
   HIPCHK(c, hipEventRecord(e0, c->st));
   double r0 = 0.0, rel = 0.0;
   int k = 0, vcycles = 0;
   int rc = VLP2D_OK;
-  if (mixed && !c->profile && max_cycles >= 1) {
+  if (mixed && !c->profile && !c->nomap && max_cycles >= 1) {
     c->r32_valid = &H == &c->h64;
     rc = solve_mixed(c, P, prog_mixed, rtol, max_cycles, k, vcycles, r0, rel);
   } else {
@@ -961,8 +984,6 @@ int solve_impl(Ctx *c, Hier<double> &H, int prog_fp64, int prog_mixed, double rt
   HIPCHK(c, hipEventSynchronize(e1));
   float ms = 0.f;
   HIPCHK(c, hipEventElapsedTime(&ms, e0, e1));
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
   if (st) {
     st->cycles = k;
     st->vcycles = vcycles;
@@ -1107,6 +1128,70 @@ int iterate_replaced(Ctx *c) {
   return mirror_level0(c, c->h64.lv[0]);
 }
 
+// ---- allocation-free solve ----------------------------------------------------
+// The device array that every allocation-free call on a GPU carves its
+// buffers from (each GPU holds its own copy, at its own address).  In host
+// code the name is a host-side stand-in, not a device address: host code uses
+// only the address hipGetSymbolAddress returns for the current device.
+static __device__ __align__(256) unsigned char scratch[VLP2D_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int kNoAllocDevices = 64;
+struct NoAllocDevice {
+  std::mutex mu;        // serialises the calls on this device
+  Ctx *c = nullptr;     // host state reused by every call on the device
+  char *ptr = nullptr;  // device address of scratch on this device
+};
+NoAllocDevice g_noalloc[kNoAllocDevices];
+
+// Once per device, with that device current: the device address of scratch,
+// and the host context with its timing events (created here, never per call).
+int noalloc_setup(NoAllocDevice &d, int device) {
+  if (d.c) return VLP2D_OK;
+  void *p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p) {
+    (void)hipGetLastError();
+    g_create_error = std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e);
+    return VLP2D_E_DEVICE;
+  }
+  Ctx *c = new Ctx();
+  c->dev = device;
+  for (hipEvent_t &ev : c->ev)
+    if (hipEventCreate(&ev) != hipSuccess) {
+      (void)hipGetLastError();
+      for (hipEvent_t &x : c->ev)
+        if (x) (void)hipEventDestroy(x);
+      delete c;
+      g_create_error = "hipEventCreate failed";
+      return VLP2D_E_DEVICE;
+    }
+  if (hipDeviceGetAttribute(&c->max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess) {
+    (void)hipGetLastError();
+    c->max_lds = 65536;
+  }
+  if (hipDeviceGetAttribute(&c->num_cu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) {
+    (void)hipGetLastError();
+    c->num_cu = 0;
+  }
+  c->h64.lv.reserve(kMaxLv);
+  c->h32.lv.reserve(kMaxLv);
+  c->arena = static_cast<char *>(p);
+  c->arena_cap = VLP2D_NOALLOC_SCRATCH_BYTES;
+  c->nomap = true;
+  c->hscal = c->hloc;
+  d.ptr = static_cast<char *>(p);
+  d.c = c;
+  return VLP2D_OK;
+}
+
+// One solve on the device's context; caller holds d.mu with the device current.
+int noalloc_solve(NoAllocDevice &d, int segX, int segY, double hx, double hy, const double *f_dev, double *u_dev,
+                  double rtol, int max_cycles, const vlp2d_mg_opts &o, hipStream_t st, vlp2d_mg_stats *stats);
+
+}  // namespace
+
 extern "C" {
 
 void vlp2d_mg_default_opts(vlp2d_mg_opts *o) {
@@ -1126,22 +1211,9 @@ void vlp2d_mg_default_opts(vlp2d_mg_opts *o) {
 
 int vlp2d_mg_version(void) { return (1 << 16) | 4; }
 
-static int create_impl(int segX, int segY, double hx, double hy, const double *f, const Expr *fe, double eps,
-                       int device, const vlp2d_mg_opts *opts, vlp2d_mg_handle *out) {
-  if (!out) {
-    g_create_error = "out handle pointer is NULL";
-    return VLP2D_E_ARG;
-  }
-  *out = nullptr;
-  if (!is_pow2(segX) || !is_pow2(segY) || segX < 4 || segY < 4 || segX > 32768 || segY > 32768) {
-    g_create_error = "segX and segY must be powers of two in [4, 32768]";
-    return VLP2D_E_SIZE;
-  }
-  if (!(hx > 0) || !(hy > 0) || !std::isfinite(hx) || !std::isfinite(hy)) {
-    g_create_error = "hx and hy must be finite and positive";
-    return VLP2D_E_ARG;
-  }
-  vlp2d_mg_opts o;
+// Options with defaults for NULL; false (message in g_create_error) when a
+// value is out of range.
+static bool checked_opts(const vlp2d_mg_opts *opts, vlp2d_mg_opts &o) {
   vlp2d_mg_default_opts(&o);
   if (opts) o = *opts;
   if (o.nu1 < 0 || o.nu2 < 0 || o.nu_coarse < 1 || o.nu1 > 32 || o.nu2 > 32 || o.nu_coarse > 1024 ||
@@ -1154,8 +1226,57 @@ static int create_impl(int segX, int segY, double hx, double hy, const double *f
     g_create_error = "invalid options (nu1, nu2 in [0,32], nu_coarse in [1,1024], precision 0 or 1, coarse_nodes >= -1, "
                      "max_inner in [1,8], smoother 0 or 1, omega in (0,1], restriction 0 or 1, cycle 0 or 1, "
                      "cycle FMG with precision FP64 only)";
+    return false;
+  }
+  return true;
+}
+
+// The fused kernels are instantiated for 1..3 sweeps per pass, red-black
+// Gauss-Seidel and half weighting; everything else runs one operator per launch.
+static bool one_operator_path(const vlp2d_mg_opts &o) {
+  return (o.flags & VLP2D_FLAG_SIMPLE) != 0 || o.nu1 < 1 || o.nu2 < 1 || o.nu1 > 3 || o.nu2 > 3 ||
+         o.smoother == VLP2D_SMOOTH_WJACOBI || o.restriction == VLP2D_RESTRICT_FULL || o.cycle == VLP2D_CYCLE_FMG;
+}
+
+// nLevels = min(log2 segX, log2 segY) (MultiGridScheme.cs:23-61)
+static int level_count(int segX, int segY) {
+  return std::min((int)std::log2((double)segX + 0.5), (int)std::log2((double)segY + 0.5));
+}
+
+static bool valid_segments(int segX, int segY) {
+  return is_pow2(segX) && is_pow2(segY) && segX >= 4 && segY >= 4 && segX <= 32768 && segY <= 32768;
+}
+
+// Partials of the tile reductions: one per tile of the largest launch.
+// Streaming tiles number at most max(resident wavefronts, column strips), the
+// strips being narrowest for the widest halo (ascent with a recomputed
+// pre-sweep, 3 post-sweeps and a residual epilogue); refinement tiles are
+// 512 x RS_ROWS.
+static int partial_capacity(int nx, int ny, int num_cu) {
+  constexpr int kMinStripCols = SCOLS - 2 * s_up_halo(4, 3);
+  const long long strips = (ny - 2) / kMinStripCols + 1;
+  const long long stiles = std::max<long long>(strips, 64LL * std::max(1, num_cu));
+  const long long rtiles = (long long)refine_strips(ny) * refine_row_blocks(nx);
+  return (int)std::max<long long>(1 << 16, std::max(stiles, rtiles) + 64);
+}
+
+static int create_impl(int segX, int segY, double hx, double hy, const double *f, const Expr *fe, double eps,
+                       int device, const vlp2d_mg_opts *opts, vlp2d_mg_handle *out) {
+  if (!out) {
+    g_create_error = "out handle pointer is NULL";
     return VLP2D_E_ARG;
   }
+  *out = nullptr;
+  if (!valid_segments(segX, segY)) {
+    g_create_error = "segX and segY must be powers of two in [4, 32768]";
+    return VLP2D_E_SIZE;
+  }
+  if (!(hx > 0) || !(hy > 0) || !std::isfinite(hx) || !std::isfinite(hy)) {
+    g_create_error = "hx and hy must be finite and positive";
+    return VLP2D_E_ARG;
+  }
+  vlp2d_mg_opts o;
+  if (!checked_opts(opts, o)) return VLP2D_E_ARG;
   const int nx = segX + 1, ny = segY + 1;
   if (f && !check_finite(f, (size_t)nx * ny)) {
     // only interior values are used, but a NaN anywhere is a caller error
@@ -1180,10 +1301,8 @@ static int create_impl(int segX, int segY, double hx, double hy, const double *f
   c->laplace = fe ? fe->empty() : (f == nullptr);
   c->o = o;
   c->profile = (o.flags & VLP2D_FLAG_PROFILE) != 0;
-  // fused kernels are instantiated for 1..3 sweeps per pass
-  c->simple = (o.flags & VLP2D_FLAG_SIMPLE) != 0 || o.nu1 < 1 || o.nu2 < 1 || o.nu1 > 3 || o.nu2 > 3 ||
-              o.smoother == VLP2D_SMOOTH_WJACOBI || o.restriction == VLP2D_RESTRICT_FULL || o.cycle == VLP2D_CYCLE_FMG;
-  c->L = std::min((int)std::log2((double)segX + 0.5), (int)std::log2((double)segY + 0.5));
+  c->simple = one_operator_path(o);
+  c->L = level_count(segX, segY);
   auto bail = [&](int rc) {
     g_create_error = c->err;
     vlp2d_mg_destroy(c);
@@ -1215,17 +1334,7 @@ static int create_impl(int segX, int segY, double hx, double hy, const double *f
   fill(c, v.u, v.elems(), 0.0);
   fill(c, v.u2, v.elems(), 0.0);
   fill(c, v.rhs, v.elems(), 0.0);
-  {
-    // partials: one per tile of the largest launch.  Streaming tiles number
-    // at most max(resident wavefronts, column strips), the strips being
-    // narrowest for the widest halo (ascent with a recomputed pre-sweep, 3
-    // post-sweeps and a residual epilogue); refinement tiles are 512 x RS_ROWS.
-    constexpr int kMinStripCols = SCOLS - 2 * s_up_halo(4, 3);
-    const long long strips = (ny - 2) / kMinStripCols + 1;
-    long long stiles = std::max<long long>(strips, 64LL * std::max(1, c->num_cu));
-    long long rtiles = (long long)refine_strips(ny) * refine_row_blocks(nx);
-    c->part_cap = (int)std::max<long long>(1 << 16, std::max(stiles, rtiles) + 64);
-  }
+  c->part_cap = partial_capacity(nx, ny, c->num_cu);
   if ((rc = dalloc(c, &c->part, (size_t)c->part_cap)) != VLP2D_OK) return bail(rc);
   if ((rc = dalloc(c, &c->part2, (size_t)c->part_cap)) != VLP2D_OK) return bail(rc);
   if ((rc = dalloc(c, &c->dscal, 8)) != VLP2D_OK) return bail(rc);
@@ -1535,6 +1644,8 @@ void vlp2d_mg_destroy(vlp2d_mg_handle c) {
   if (c->hscal) (void)hipHostFree(c->hscal);
   if (c->hpub) (void)hipHostFree(c->hpub);
   dfree(c, c->dseq, 1);
+  for (hipEvent_t &e : c->ev)
+    if (e) (void)hipEventDestroy(e);
   if (c->st) (void)hipStreamDestroy(c->st);
   delete c;
 }
@@ -1672,4 +1783,148 @@ const char *vlp2d_mg_last_error(vlp2d_mg_handle c) {
   return c->err.c_str();
 }
 
+int vlp2d_mg_solve_noalloc(int segX, int segY, double hx, double hy, const double *f_dev, double *u_dev, int pitch,
+                           double rtol, int max_cycles, const vlp2d_mg_opts *opts, int device, void *stream,
+                           vlp2d_mg_stats *stats) {
+  if (!valid_segments(segX, segY)) {
+    g_create_error = "segX and segY must be powers of two in [4, 32768]";
+    return VLP2D_E_SIZE;
+  }
+  if (!(hx > 0) || !(hy > 0) || !std::isfinite(hx) || !std::isfinite(hy)) {
+    g_create_error = "hx and hy must be finite and positive";
+    return VLP2D_E_ARG;
+  }
+  if (!u_dev) {
+    g_create_error = "u_dev is NULL";
+    return VLP2D_E_ARG;
+  }
+  if (pitch != round_pitch(segY + 1)) {
+    g_create_error = "pitch must be ny = segY + 1 rounded up to a multiple of 32 (" +
+                     std::to_string(round_pitch(segY + 1)) + ")";
+    return VLP2D_E_ARG;
+  }
+  if ((reinterpret_cast<uintptr_t>(u_dev) & 15) || (reinterpret_cast<uintptr_t>(f_dev) & 15)) {
+    g_create_error = "u_dev and f_dev must be 16-byte aligned";
+    return VLP2D_E_ARG;
+  }
+  if (!(rtol >= 0) || max_cycles < 0) {
+    g_create_error = "rtol must be >= 0 and max_cycles >= 0";
+    return VLP2D_E_ARG;
+  }
+  vlp2d_mg_opts o;
+  if (!checked_opts(opts, o)) return VLP2D_E_ARG;
+  if (o.flags & VLP2D_FLAG_PROFILE) {
+    g_create_error = "VLP2D_FLAG_PROFILE is not available without allocation";
+    return VLP2D_E_ARG;
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev || device >= kNoAllocDevices) {
+    (void)hipGetLastError();
+    g_create_error = "device " + std::to_string(device) + " not available (" + std::to_string(ndev) + " visible)";
+    return VLP2D_E_DEVICE;
+  }
+  int prev = -1;
+  if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+  if (hipSetDevice(device) != hipSuccess) {
+    (void)hipGetLastError();
+    g_create_error = "hipSetDevice failed";
+    return VLP2D_E_DEVICE;
+  }
+  NoAllocDevice &d = g_noalloc[device];
+  int rc;
+  {
+    std::lock_guard<std::mutex> lock(d.mu);
+    rc = noalloc_setup(d, device);
+    if (rc == VLP2D_OK)
+      rc = noalloc_solve(d, segX, segY, hx, hy, f_dev, u_dev, rtol, max_cycles, o, static_cast<hipStream_t>(stream),
+                         stats);
+  }
+  if (prev >= 0 && prev != device) (void)hipSetDevice(prev);
+  return rc;
+}
+
 }  // extern "C"
+
+namespace {
+
+int noalloc_solve(NoAllocDevice &d, int segX, int segY, double hx, double hy, const double *f_dev, double *u_dev,
+                  double rtol, int max_cycles, const vlp2d_mg_opts &o, hipStream_t st, vlp2d_mg_stats *stats) {
+  Ctx *c = d.c;
+  // the state of a fresh handle over the caller's arrays (no graphs, no
+  // mapped memory, every buffer from scratch, cleared before use)
+  c->segX = segX;
+  c->segY = segY;
+  c->nx = segX + 1;
+  c->ny = segY + 1;
+  c->hx = hx;
+  c->hy = hy;
+  c->eps = 0.0;
+  c->laplace = f_dev == nullptr;
+  c->o = o;
+  c->o.flags |= VLP2D_FLAG_NO_GRAPH;
+  c->profile = false;
+  c->simple = one_operator_path(o);
+  c->L = level_count(segX, segY);
+  c->st = st;
+  c->arena_used = 0;
+  c->bytes = c->peak = 0;
+  c->r32_valid = false;
+  c->K = 0;
+  c->cancel.store(0);
+  c->err.clear();
+  c->part = c->part2 = c->dscal = nullptr;
+  c->dseq = nullptr;
+  for (Hier<double> *H : {&c->h64, &c->ha}) {
+    H->lv.clear();
+    H->full = false;
+  }
+  c->h32.lv.clear();
+  c->h32.full = false;
+  shape_hier(c, c->h64, hx, hy);
+  Lvl<double> &v = c->h64.lv[0];
+  v.lap = c->laplace;
+  v.u = u_dev;
+  auto bail = [&](int rc) {
+    g_create_error = c->err;
+    return rc;
+  };
+  int rc;
+  c->part_cap = partial_capacity(c->nx, c->ny, c->num_cu);
+  if ((rc = dalloc(c, &v.u2, v.elems())) != VLP2D_OK) return bail(rc);
+  if ((rc = dalloc(c, &v.rhs, v.elems())) != VLP2D_OK) return bail(rc);
+  if ((rc = dalloc(c, &c->part, (size_t)c->part_cap)) != VLP2D_OK) return bail(rc);
+  if ((rc = dalloc(c, &c->part2, (size_t)c->part_cap)) != VLP2D_OK) return bail(rc);
+  if ((rc = dalloc(c, &c->dscal, 8)) != VLP2D_OK) return bail(rc);
+  // the rest of the hierarchy: allocated (from scratch) before anything is
+  // cleared or runs, so a grid that does not fit returns here
+  if (o.precision == VLP2D_PREC_MIXED) rc = ensure_h32(c);
+  else rc = complete_hier(c, c->h64);
+  if (rc != VLP2D_OK) return bail(rc);
+  fill(c, c->dscal, 8, 0.0);
+  fill(c, v.rhs, v.elems(), 0.0);
+  if (f_dev) {
+    // fn = -f on interior nodes, 0 on the boundary (MultiGridSchemeCU.cs:41-45)
+    launch(c, KC_COPY, 0, 16.0 * nodes(v), [&] {
+      hipLaunchKernelGGL(k_neg_interior, grid2(v.ny - 2, v.nx - 2), kBlock, 0, c->st, v.rhs, f_dev, v.nx, v.ny,
+                         v.pitch);
+    });
+  }
+  if (hipGetLastError() != hipSuccess) return bail(fail(c, VLP2D_E_DEVICE, "set-up kernels failed to launch"));
+  if ((rc = mirror_level0(c, v)) != VLP2D_OK) return bail(rc);
+  rc = solve_impl(c, c->h64, PG_CYCLE_NORM, PG_MIXED, rtol, max_cycles, stats);
+  if (rc == VLP2D_OK || rc == VLP2D_E_NOTCONV || rc == VLP2D_E_NONFINITE) {
+    // a mixed solve alternates the iterate between u_dev and its second buffer
+    if (v.u != u_dev &&
+        hipMemcpyAsync(u_dev, v.u, v.elems() * sizeof(double), hipMemcpyDeviceToDevice, c->st) != hipSuccess) {
+      (void)hipGetLastError();
+      rc = fail(c, VLP2D_E_DEVICE, "copy of the final iterate failed");
+    }
+  }
+  if (hipStreamSynchronize(c->st) != hipSuccess) {
+    (void)hipGetLastError();
+    if (rc == VLP2D_OK) rc = fail(c, VLP2D_E_DEVICE, "stream synchronisation failed");
+  }
+  return rc == VLP2D_OK ? rc : bail(rc);
+}
+
+}  // namespace
