@@ -10,11 +10,16 @@ Reported: the mixed-precision solver (FP32 V(1,3) cycles inside FP64
 iterative refinement, at most two FP32 cycles per refinement step; --cycle
 picks another V(nu1,nu2), --inner the cap) as ``value``; the
 plain FP64 multigrid with the same cycle on the same
-problem beside it (``fp64``); the dominant kernel's roofline from hipEvent
-timings of a profiled solve; and the oracle's CPU multigrid on a bounded
-sample of the same workload.
+problem beside it (``fp64``); the same two solves with config 3's literal
+V(2,2) cycle (``v22``); the dominant kernel's roofline from hipEvent timings
+of a profiled solve; the HBM traffic of one whole solve of each precision
+(``solve_traffic``, from the committed rocprofv3 PMC summary) against this
+run's solve times; and the oracle's CPU multigrid on the same workload on
+every CPU the job may use.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--seg 8192]
+    python bench.py --only mixed|fp64 --steps 1 --warmup 1 --no-cpu --no-profile --no-v22
+        (one precision only: the run rocprofv3 --pmc profiles for solve_traffic)
 """
 import argparse
 import json
@@ -29,6 +34,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
 PMC_SUMMARY = os.path.join(ROOT, "profiles", "poisson8193_level0_pmc.md")
+SOLVE_PMC_SUMMARY = os.path.join(ROOT, "profiles", "poisson8193_solve_pmc.md")
 # launch classes that run on the FP32 hierarchy in a mixed-precision solve
 FP32_CLASSES_IN_MIXED = {"down_fused", "up_fused", "up_fused_refine", "up_lds", "coarse_vcycle", "rb_half", "residual", "restrict",
                          "prolong", "fill"}
@@ -84,8 +90,8 @@ def cpu_baseline(seg, rtol, nu):
     from oracle import Oracle
     from vlp2d_amd import problems
     p = problems.config(3, seg)
-    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
-    cores = max(1, min(cores, 16))
+    host = host_cpus()
+    cores = host["usable"]
     o = Oracle(p.seg_x, p.seg_y, p.hx, p.hy, p.f, "double", nthreads=cores)
     o.set_u(p.initial("zero"))
     t0 = time.perf_counter()
@@ -95,8 +101,35 @@ def cpu_baseline(seg, rtol, nu):
     o.close()
     rec = {"value": round(p.unknowns / dt / 1e6, 2), "unit": "Munknowns/s", "cores": cores, "kind": "port",
            "sample": f"oracle FP64 V({nu[0]},{nu[1]}) RB-GS multigrid on the {seg + 1}x{seg + 1} workload, "
-                     f"{k} cycles to rel. residual {hist[-1]:.2e} in {dt:.2f} s"}
+                     f"{k} cycles to rel. residual {hist[-1]:.2e} in {dt:.2f} s, one OpenMP thread per usable CPU",
+           "nproc": host["nproc"], "affinity": host["affinity"], "cgroup_cpus": host["cgroup"],
+           "cpu_model": host["model"]}
     return rec, u
+
+
+def host_cpus():
+    """CPUs of the host and the share this job may use: the affinity mask,
+    capped by the cgroup CPU quota when one is set (a quota below the mask
+    makes more threads time-share the same CPU time)."""
+    nproc = os.cpu_count() or 1
+    aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else nproc
+    quota = None
+    try:
+        q, per = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if q != "max":
+            quota = int(q) // int(per)
+    except (OSError, ValueError):
+        pass
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    usable = max(1, min(aff, quota) if quota else aff)
+    return {"nproc": nproc, "affinity": aff, "cgroup": quota, "model": model, "usable": usable}
 
 
 def dd128_check(nu, seg=512, rtol=1e-10, inner=1):
@@ -142,6 +175,9 @@ def main():
                     help="nu1,nu2 of the V-cycle (V(1,3): fewest cycles to 1e-10 on this workload, measured)")
     ap.add_argument("--inner", type=int, default=2,
                     help="mixed: at most this many FP32 V-cycles per FP64 refinement step")
+    ap.add_argument("--only", choices=("mixed", "fp64"),
+                    help="time one precision only (the runs rocprofv3 --pmc profiles)")
+    ap.add_argument("--no-v22", action="store_true", help="skip the V(2,2) sub-record")
     args = ap.parse_args()
     nu = tuple(int(x) for x in args.cycle.split(","))
     if args.gpus != 1:
@@ -155,20 +191,28 @@ def main():
     p = problems.config(3, args.seg)
     log(f"problem {p.name}: {p.shape} nodes, {p.unknowns} unknowns ({time.perf_counter() - t0:.1f}s to generate)")
 
-    res = {}
-    for mixed in (True, False):
+    def timed(mixed, cyc_nu, keep_u=True):
         s = MultiGridSchemeHIP(p.seg_x, p.seg_y, p.hx, p.hy, p.f,
-                               options=Options.performance(mixed, nu1=nu[0], nu2=nu[1],
+                               options=Options.performance(mixed, nu1=cyc_nu[0], nu2=cyc_nu[1],
                                                            max_inner=args.inner if mixed else 1))
         s.un0[:] = p.initial("zero")
         s.init_after_boundaries_and_initial_iteration_inited()
         s.save_iterate()
         wall, dev, cyc, rel = solve_times(s, args.steps, args.warmup, args.rtol)
-        res[mixed] = {"wall": wall, "dev": dev, "cycles": cyc, "rel": rel, "info": s.info(),
-                      "vcycles": solve_times.vcycles[0], "u": s.get_array().copy()}
-        log(f"{'mixed' if mixed else 'fp64 '}: steps {cyc}, V-cycles {solve_times.vcycles[0]}, "
-            f"wall ms {[round(w * 1e3, 3) for w in wall]}, rel {max(rel):.2e}")
+        r = {"wall": wall, "dev": dev, "cycles": cyc, "rel": rel, "info": s.info(),
+             "vcycles": solve_times.vcycles[0], "u": s.get_array().copy() if keep_u else None}
+        log(f"{'mixed' if mixed else 'fp64 '} V({cyc_nu[0]},{cyc_nu[1]}): steps {cyc}, V-cycles "
+            f"{solve_times.vcycles[0]}, wall ms {[round(w * 1e3, 3) for w in wall]}, rel {max(rel):.2e}")
         s.cleanup()
+        return r
+
+    if args.only:
+        r = timed(args.only == "mixed", nu, keep_u=False)
+        print(json.dumps({"only": args.only, "ms_per_step": round(1e3 * float(np.mean(r["wall"])), 4),
+                          "cycles": r["cycles"][0], "vcycles": r["vcycles"]}), flush=True)
+        return
+
+    res = {mixed: timed(mixed, nu) for mixed in (True, False)}
 
     mixed_ms = 1e3 * float(np.mean(res[True]["wall"]))
     fp64_ms = 1e3 * float(np.mean(res[False]["wall"]))
@@ -191,17 +235,29 @@ def main():
                     "kernel": f"{key}/{prec}", "avg_launch_us": round(avg_s * 1e6, 3),
                     "alg_bytes_per_launch": per_launch_bytes, "share_of_solve": round(ms / tot, 3)}
         kernels = sorted(([e[0], e[1], e[2], round(e[3], 4), e[4]] for e in kt), key=lambda e: -e[3])[:12]
-    # whole-solve accounting with the per-operator byte model (unfused operators)
-    solve_bytes = {m: alg_bytes_per_operator(p, m, res[m]["cycles"][0], res[m]["vcycles"], nu[0], nu[1])
-                   for m in (True, False)}
-    solve_roofline = {
-        "model": "per-operator algorithmic bytes (each operator one streaming pass; V(nu1,nu2): "
-                 "(3(nu1+nu2)+6+3/4) s B/node/cycle, refinement 40 B/node/step, residual check 16 B/node)",
-        "mixed": {"bytes": solve_bytes[True], "effective_GBps": round(solve_bytes[True] / (mixed_ms * 1e-3) / 1e9, 1),
-                  "frac": round(solve_bytes[True] / (mixed_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
-        "fp64": {"bytes": solve_bytes[False], "effective_GBps": round(solve_bytes[False] / (fp64_ms * 1e-3) / 1e9, 1),
-                 "frac": round(solve_bytes[False] / (fp64_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
-    }
+    # whole solve: the HBM bytes one solve of each precision moves (rocprofv3
+    # FETCH_SIZE + WRITE_SIZE over all its dispatches, committed summary of a
+    # run of the same command) over this run's solve time
+    solve_traffic = None
+    if args.seg == 8192:
+        cyc = f"V({nu[0]},{nu[1]})"
+        solve_traffic = {"source": os.path.relpath(SOLVE_PMC_SUMMARY, ROOT), "cycle": cyc}
+        for key, ms in (("mixed", mixed_ms), ("fp64", fp64_ms)):
+            b = solve_traffic_bytes(f"{key}{cyc}")
+            solve_traffic[key] = None if b is None else {
+                "bytes": b, "GBps": round(b / (ms * 1e-3) / 1e9, 1), "frac": round(b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+
+    # config 3's literal cycle, V(2,2), for both precisions
+    v22 = None
+    if not args.no_v22 and nu != (2, 2):
+        r22 = {m: timed(m, (2, 2), keep_u=False) for m in (True, False)}
+        m22, f22 = (1e3 * float(np.mean(r22[m]["wall"])) for m in (True, False))
+        v22 = {"mixed": {"value": round(p.unknowns / (m22 * 1e-3) / 1e6, 2), "ms_per_step": round(m22, 4),
+                         "steps": r22[True]["cycles"][0], "vcycles": r22[True]["vcycles"],
+                         "rel_residual": max(r22[True]["rel"])},
+               "fp64": {"value": round(p.unknowns / (f22 * 1e-3) / 1e6, 2), "ms_per_step": round(f22, 4),
+                        "cycles": r22[False]["cycles"][0], "rel_residual": max(r22[False]["rel"])},
+               "speedup_mixed_vs_fp64": round(f22 / m22, 3)}
 
     accuracy = {"mixed_vs_fp64_gpu_rel": float(np.abs(res[True]["u"] - res[False]["u"]).max()
                                                / np.abs(res[False]["u"]).max())}
@@ -240,38 +296,13 @@ def main():
         "mixed_max_inner": args.inner,
         "speedup_mixed_vs_fp64": round(fp64_ms / mixed_ms, 3),
         "roofline": roofline,
-        "solve_roofline": solve_roofline,
+        "solve_traffic": solve_traffic,
+        "v22": v22,
         "accuracy": accuracy,
         "cpu_baseline": cpu,
         "kernels_top": kernels,
     }
     print(json.dumps(line), flush=True)
-
-
-def alg_bytes_per_operator(p, mixed, cycles, vcycles, nu1=2, nu2=2):
-    """Bytes a solve moves if every operator is one streaming pass: per
-    V-cycle and non-coarsest level (3(nu1+nu2) + 6) s N_l + 3 s N_{l+1},
-    coarsest 3 s N_c; mixed adds 40 B/node per refinement step (`cycles`
-    steps: FP64 residual + cast, FP64 update) and 20 B/node for the first
-    residual; FP64 adds a 16 B/node residual check per cycle and one
-    initial check."""
-    s = 4 if mixed else 8
-    nx, ny = p.seg_x, p.seg_y
-    levels = []
-    while True:
-        levels.append((nx + 1) * (ny + 1))
-        if nx % 2 or ny % 2 or min(nx, ny) <= 2:
-            break
-        nx //= 2
-        ny //= 2
-    L = min(int(np.log2(p.seg_x)), int(np.log2(p.seg_y)))
-    levels = levels[:L]
-    per_cycle = sum((3 * (nu1 + nu2) + 6) * s * levels[l] + 3 * s * levels[l + 1] for l in range(L - 1))
-    per_cycle += 3 * s * levels[-1]
-    n0 = levels[0]
-    if mixed:
-        return float(vcycles * per_cycle + cycles * 40 * n0 + 20 * n0)
-    return float(vcycles * per_cycle + cycles * 16 * n0 + 16 * n0)
 
 
 def measured_traffic(kernel_key):
@@ -285,6 +316,19 @@ def measured_traffic(kernel_key):
     for line in open(path):
         parts = line.split()
         if len(parts) == 3 and parts[0] == "TRAFFIC" and parts[1] == kernel_key:
+            return float(parts[2])
+    return None
+
+
+def solve_traffic_bytes(label):
+    """HBM bytes of one whole solve from the committed PMC summary (lines
+    'SOLVE_TRAFFIC <label> <bytes> <dispatches>' written by
+    vlp2d_amd.rocprof_summary solve), or None."""
+    if not os.path.exists(SOLVE_PMC_SUMMARY):
+        return None
+    for line in open(SOLVE_PMC_SUMMARY):
+        parts = line.split()
+        if len(parts) == 4 and parts[0] == "SOLVE_TRAFFIC" and parts[1] == label:
             return float(parts[2])
     return None
 

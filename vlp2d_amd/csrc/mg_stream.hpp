@@ -31,10 +31,12 @@
 // reference's multigrid.  Rows and columns within distance s of the strip
 // edge hold partial results after half-sweep s; the halo H covers them.
 //
-// Every tile runs a body that predicates its loads and keeps Dirichlet nodes
-// fixed.  Only the plain ascent (EPI 0) also holds a body without boundary
-// predicates, for tiles whose staged nodes are all interior nodes of the
-// domain; in the other kernels the predicated body alone is faster.
+// Every tile keeps Dirichlet nodes fixed with per-lane column predicates
+// (one select per half-sweep).  Row checks (Dirichlet rows, rows outside the
+// domain, predicated loads) are needed only in the loop bodies of tiles at
+// the top and bottom of the domain: each loop body of P rows runs in one of
+// two compiled forms, with row checks (ROWS) or, when every row it touches is
+// an interior row, without them and with unconditional loads.
 //
 // Kernels:
 //   k_sdown : [zero or staged iterate] -> S = 2*NS half-sweeps -> residual
@@ -274,7 +276,7 @@ template <typename T, int NS> struct SweepState {
 // Apply half-sweeps SA..SB (default 1..S) at step J (row q = q0 + J):
 // half-sweep s updates row q - s, using level s-1 rows q-s-1, q-s, q-s+1.
 // On edge tiles nodes outside the domain interior keep their value.
-template <typename T, int NS, bool EQ, bool PW2, bool EDGE, int J, int SA = 1, int SB = 2 * NS>
+template <typename T, int NS, bool EQ, bool PW2, bool EDGE, bool ROWS, int J, int SA = 1, int SB = 2 * NS>
 __device__ __forceinline__ void s_sweeps(SweepState<T, NS> &st, const LevelView<T> &a, const Lane &L, int gi0, int q) {
   constexpr int P = SweepState<T, NS>::P;
   static_for<SA, SB + 1>([&](auto sc) {
@@ -295,8 +297,11 @@ __device__ __forceinline__ void s_sweeps(SweepState<T, NS> &st, const LevelView<
     }
     const T g = s_gs<T, EQ, PW2>(above, below, ujm, ujp, st.F[md<P>(p)][e], a);
     if constexpr (EDGE) {
-      const int gi = gi0 + q - s;
-      const bool ok = gi >= 1 && gi <= a.nx - 2 && (e == 0 ? L.in0 : L.in1);
+      bool ok = e == 0 ? L.in0 : L.in1;
+      if constexpr (ROWS) {
+        const int gi = gi0 + q - s;
+        ok = ok && gi >= 1 && gi <= a.nx - 2;
+      }
       st.U[s + 1][md<6>(p)] = ok ? g : st.U[s - 1][md<6>(p)];
     } else {
       st.U[s + 1][md<6>(p)] = g;
@@ -306,15 +311,23 @@ __device__ __forceinline__ void s_sweeps(SweepState<T, NS> &st, const LevelView<
 
 // Load the raw pair of staged row q + kPf (u unless zero-initialised) and
 // its right-hand side.
-template <typename T, int NS, bool ZI, bool EDGE, int J>
+// ROWS = false: every row this loop body touches is an interior row of the
+// domain, so the load needs no predicate: lanes outside the row allocation
+// read its columns 0..1 (L.boff = 0), values that reach no interior node.
+template <typename T, int NS, bool ZI, bool EDGE, bool ROWS, int J>
 __device__ __forceinline__ void s_prefetch(SweepState<T, NS> &st, const LevelView<T> &a, const Lane &L, int gi0, int q,
                                            int LR) {
   constexpr int P = SweepState<T, NS>::P;
   const int qq = q + kPf, gi = gi0 + qq;
   Pair<T> u{T(0), T(0)}, f{T(0), T(0)};
-  bool go = qq < LR;
-  if constexpr (EDGE) go = go && gi >= 0 && gi <= a.nx - 1 && L.ld;
-  if (go) {
+  if constexpr (ROWS) {
+    bool go = qq < LR;
+    if constexpr (EDGE) go = go && gi >= 0 && gi <= a.nx - 1 && L.ld;
+    if (go) {
+      if constexpr (!ZI) u = ld_pair(row_of(a.uin, gi, a.pitch), L.boff);
+      if (!a.lap) f = ld_pair(row_of(a.f, gi, a.pitch), L.boff);
+    }
+  } else {
     if constexpr (!ZI) u = ld_pair(row_of(a.uin, gi, a.pitch), L.boff);
     if (!a.lap) f = ld_pair(row_of(a.f, gi, a.pitch), L.boff);
   }
@@ -338,7 +351,11 @@ template <int H> __device__ __forceinline__ bool s_tile(int R, int tiles_i, int 
   {
     // workgroup -> XCD as tile_of_block, kWaves consecutive tiles per workgroup
     const int b = blockIdx.x;
-    const int t = ((b & 7) * chunk + (b >> 3)) * kWaves + (int)(threadIdx.x >> 6);
+    // the wavefront's index in its workgroup is uniform across the wave:
+    // readfirstlane makes the tile, its rows and its row addresses scalar,
+    // so the choice of loop body below is a scalar branch
+    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int t = ((b & 7) * chunk + (b >> 3)) * kWaves + w;
     tl.t = t < tiles_i * tiles_j ? t : -1;
   }
   if (tl.t < 0) return false;
@@ -374,28 +391,33 @@ __device__ __forceinline__ void sdown_body(const LevelView<T> &a, const STile &t
   // rows 0 .. kPf - 1 (the loop prefetches q + kPf)
   static_for<-kPf, 0>([&](auto jc) {
     constexpr int J = decltype(jc)::value;
-    s_prefetch<T, NS, ZI, EDGE, J>(st, a, L, gi0, J, LR);
+    s_prefetch<T, NS, ZI, EDGE, true, J>(st, a, L, gi0, J, LR);
   });
   const int Jc = L.m + cb / 2;  // coarse column of element 0 (fine column 2 Jc)
   const bool oc_c = EDGE ? L.oc && Jc >= 1 && Jc <= ncy - 2 : L.oc;
   T *const rc = rhsc + Jc;
 #pragma unroll 1
   for (int q0 = 0; q0 < LR; q0 += P) {
+    // rows q0 - S - 3 .. q0 + P + kPf + 4 cover everything this loop body
+    // reads or writes; when all are interior rows it runs without row checks
+    const bool rows_in = gi0 + q0 - (S + 3) >= 1 && gi0 + q0 + P + kPf + 4 <= a.nx - 2;
+    auto body = [&](auto rows_c) {
+    constexpr bool ROWS = decltype(rows_c)::value;
     static_for<0, P>([&](auto jc) {
       constexpr int J = decltype(jc)::value;
       const int q = q0 + J;
       group_step();
-      s_prefetch<T, NS, ZI, EDGE, J>(st, a, L, gi0, q, LR);
+      s_prefetch<T, NS, ZI, EDGE, ROWS, J>(st, a, L, gi0, q, LR);
       // level 0 of row q: red element (J & 1), black element
       constexpr int er = J & 1;
       st.U[0][md<6>(J)] = st.RU[md<kNru>(J)][er];
       st.U[1][md<6>(J)] = st.RU[md<kNru>(J)][1 - er];
-      s_sweeps<T, NS, EQ, PW2, EDGE, J>(st, a, L, gi0, q);
+      s_sweeps<T, NS, EQ, PW2, EDGE, ROWS, J>(st, a, L, gi0, q);
       // store row q - S (unless the ascent recomputes it)
       if constexpr (STORE) {
         const int p = q - S, gi = gi0 + p;
         bool go = p >= H && p < H + R;
-        if constexpr (EDGE) go = go && gi >= 1 && gi <= a.nx - 2;
+        if constexpr (ROWS) go = go && gi >= 1 && gi <= a.nx - 2;
         if (go) s_store<T, EDGE>(a.uout, a.pitch, gi, L, st.template full<S, J, S, 0>(), st.template full<S, J, S, 1>());
       }
       // residual of row q - S - 1
@@ -421,11 +443,14 @@ __device__ __forceinline__ void sdown_body(const LevelView<T> &a, const STile &t
           const T rim = RR[md<3>(px - 1)][0], rip = RR[md<3>(px + 1)][0];
           const int I = (gi0 + p) / 2;
           bool go = p >= H && p < H + R;
-          if constexpr (EDGE) go = go && I >= 1 && I <= ncx - 2;
+          if constexpr (ROWS) go = go && I >= 1 && I <= ncx - 2;
           if (go && oc_c) rc[I * pc] = T(0.125) * (r0 * T(4) + (rip + rim + rjp + rjm));
         }
       }
     });
+    };
+    if (rows_in) body(std::false_type{});
+    else body(std::true_type{});
   }
 }
 
@@ -478,17 +503,26 @@ __device__ __forceinline__ void sup_body(const LevelView<T> &a, const STile &tl,
   const bool ldr = last && (EDGE ? Jc + 1 >= 0 && Jc + 1 <= ncy - 1 : true);
   const T *const ucl = uc + (ldc ? Jc : 0);
   const T *const ucr = uc + (ldr ? Jc + 1 : 0);
-  auto load_coarse = [&](int kk, T &v, T &vr) {
+  // rows_c (ROWS): the coarse row may lie outside the coarse grid; else it is
+  // a coarse-grid row, and the clamped columns are loaded unconditionally and
+  // masked (no divergent branch)
+  auto load_coarse = [&](int kk, T &v, T &vr, auto rows_c) {
     const int I = gi0 / 2 + kk;
-    v = T(0);
-    vr = T(0);
-    if (I >= 0 && I <= ncx - 1) {
-      if (ldc) v = ucl[I * pc];
-      if (ldr) vr = ucr[I * pc];
+    if constexpr (decltype(rows_c)::value) {
+      v = T(0);
+      vr = T(0);
+      if (I >= 0 && I <= ncx - 1) {
+        if (ldc) v = ucl[I * pc];
+        if (ldr) vr = ucr[I * pc];
+      }
+    } else {
+      const T x = ucl[I * pc], xr = ucr[I * pc];
+      v = ldc ? x : T(0);
+      vr = ldr ? xr : T(0);
     }
   };
-  load_coarse(0, C[0], CR[0]);
-  load_coarse(1, C[1], CR[1]);
+  load_coarse(0, C[0], CR[0], std::true_type{});
+  load_coarse(1, C[1], CR[1], std::true_type{});
   C[2] = CR[2] = T(0);
   // EPI 2 keeps the staged (post-prolongation) pair of each row until its
   // last half-sweep
@@ -515,28 +549,35 @@ __device__ __forceinline__ void sup_body(const LevelView<T> &a, const STile &tl,
   const bool oc_c = EDGE ? L.oc && Jc >= 1 && Jc <= ncy - 2 : L.oc;
   static_for<-kPf, 0>([&](auto jc) {
     constexpr int J = decltype(jc)::value;
-    s_prefetch<T, NT, ZS, EDGE, J>(st, a, L, gi0, J, LR);
+    s_prefetch<T, NT, ZS, EDGE, true, J>(st, a, L, gi0, J, LR);
   });
 #pragma unroll 1
   for (int q0 = 0; q0 < LR; q0 += P) {
+    // rows q0 - SE - 3 .. q0 + P + kPf + 4 cover everything this loop body
+    // reads or writes (coarse rows included); when all are interior rows it
+    // runs without row checks
+    const bool rows_in = gi0 + q0 - (SE + 3) >= 1 && gi0 + q0 + P + kPf + 4 <= a.nx - 2;
+    auto body = [&](auto rows_c) {
+    constexpr bool ROWS = decltype(rows_c)::value;
     static_for<0, P>([&](auto jc) {
       constexpr int J = decltype(jc)::value;
       const int q = q0 + J;
       group_step();
-      s_prefetch<T, NT, ZS, EDGE, J>(st, a, L, gi0, q, LR);
+      s_prefetch<T, NT, ZS, EDGE, ROWS, J>(st, a, L, gi0, q, LR);
       // staged row q (level -1: red element J & 1, level 0: black element)
       constexpr int er = J & 1;
       if constexpr (N1 > 0) {
         st.U[0][md<6>(J)] = st.RU[md<kNru>(J)][er];
         st.U[1][md<6>(J)] = st.RU[md<kNru>(J)][1 - er];
-        s_sweeps<T, NT, EQ, PW2, EDGE, J, 1, S1>(st, a, L, gi0, q);
+        s_sweeps<T, NT, EQ, PW2, EDGE, ROWS, J, 1, S1>(st, a, L, gi0, q);
       }
       // prolongation of row pr = q - S1, whose level S1 is now complete
       {
         constexpr int JP = md<P>(J - S1);  // phase of row pr (P is a multiple of 6)
         constexpr int kk = JP >> 1;          // coarse row of row pr, mod 3
         const int pr = q - S1, gi = gi0 + pr;
-        if constexpr ((JP & 1) == 0) load_coarse(pr / 2 + 2, C[md<3>(kk + 2)], CR[md<3>(kk + 2)]);
+        if constexpr ((JP & 1) == 0)
+          load_coarse(pr / 2 + 2, C[md<3>(kk + 2)], CR[md<3>(kk + 2)], std::integral_constant<bool, ROWS>{});
         const T c00 = C[md<3>(kk)];
         const T c01r = from_right(c00);
         const T c01 = last ? CR[md<3>(kk)] : c01r;
@@ -553,14 +594,18 @@ __device__ __forceinline__ void sup_body(const LevelView<T> &a, const STile &tl,
         }
         bool a0 = true, a1 = true;
         if constexpr (EDGE) {
-          const bool rin = gi >= 1 && gi <= a.nx - 2;
-          a0 = rin && L.in0;
-          a1 = rin && L.in1;
+          a0 = L.in0;
+          a1 = L.in1;
+          if constexpr (ROWS) {
+            const bool rin = gi >= 1 && gi <= a.nx - 2;
+            a0 = a0 && rin;
+            a1 = a1 && rin;
+          }
         }
         if constexpr (N1 == 0) {
           T v0 = st.RU[md<kNru>(J)][0], v1 = st.RU[md<kNru>(J)][1];
-          if (a0) v0 = v0 + p0;
-          if (a1) v1 = v1 + p1;
+          v0 = a0 ? v0 + p0 : v0;
+          v1 = a1 ? v1 + p1 : v1;
           st.U[0][md<6>(J)] = er == 0 ? v0 : v1;
           st.U[1][md<6>(J)] = er == 0 ? v1 : v0;
           if constexpr (EPI == 2) {
@@ -574,9 +619,9 @@ __device__ __forceinline__ void sup_body(const LevelView<T> &a, const STile &tl,
           constexpr int eb = md<2>(JP + 1);
           T &v = st.U[S1 + 1][md<6>(JP)];
           if constexpr (eb == 0) {
-            if (a0) v = v + p0;
+            v = a0 ? v + p0 : v;
           } else {
-            if (a1) v = v + p1;
+            v = a1 ? v + p1 : v;
           }
         }
       }
@@ -584,25 +629,31 @@ __device__ __forceinline__ void sup_body(const LevelView<T> &a, const STile &tl,
         // prefetch FP64 rows: iterate row q - ST + 2, right-hand side row q - ST + 1
         const int r = q - ST + 2, gr = gi0 + r;
         Pair<double> x{0.0, 0.0}, f{0.0, 0.0};
-        bool go = r >= 0 && r < LR;
-        if constexpr (EDGE) go = go && gr >= 0 && gr <= a.nx - 1 && L.ld;
-        if (go) x = ld_pair(row_of(mx.u64in, gr, a.pitch), boff8);
         const int rf = q - ST + 1, gf = gi0 + rf;
-        bool gof = rf >= 0 && rf < LR;
-        if constexpr (EDGE) gof = gof && gf >= 0 && gf <= a.nx - 1 && L.ld;
-        if (gof) f = ld_pair(row_of(mx.fn64, gf, a.pitch), boff8);
+        if constexpr (ROWS) {
+          bool go = r >= 0 && r < LR;
+          if constexpr (EDGE) go = go && gr >= 0 && gr <= a.nx - 1 && L.ld;
+          if (go) x = ld_pair(row_of(mx.u64in, gr, a.pitch), boff8);
+          bool gof = rf >= 0 && rf < LR;
+          if constexpr (EDGE) gof = gof && gf >= 0 && gf <= a.nx - 1 && L.ld;
+          if (gof) f = ld_pair(row_of(mx.fn64, gf, a.pitch), boff8);
+        } else {
+          // both rows are domain rows (values of rows before the tile are unused)
+          x = ld_pair(row_of(mx.u64in, gr, a.pitch), boff8);
+          f = ld_pair(row_of(mx.fn64, gf, a.pitch), boff8);
+        }
         X64[md<3>(J - ST + 2)][0] = x.x0;
         X64[md<3>(J - ST + 2)][1] = x.x1;
         FN[md<3>(J - ST + 1)][0] = f.x0;
         FN[md<3>(J - ST + 1)][1] = f.x1;
       }
-      s_sweeps<T, NT, EQ, PW2, EDGE, J, S1 + 1, ST>(st, a, L, gi0, q);
+      s_sweeps<T, NT, EQ, PW2, EDGE, ROWS, J, S1 + 1, ST>(st, a, L, gi0, q);
       // store row q - ST (+ update metric)
       {
         const int p = q - ST, gi = gi0 + p;
         const T x0 = st.template full<ST, J, ST, 0>(), x1 = st.template full<ST, J, ST, 1>();
         bool orow = p >= H && p < H + R;
-        if constexpr (EDGE) orow = orow && gi >= 1 && gi <= a.nx - 2;
+        if constexpr (ROWS) orow = orow && gi >= 1 && gi <= a.nx - 2;
         if constexpr (EPI != 3) {
           if (orow) s_store<T, EDGE>(a.uout, a.pitch, gi, L, x0, x1);
         } else {
@@ -610,9 +661,14 @@ __device__ __forceinline__ void sup_body(const LevelView<T> &a, const STile &tl,
           constexpr int sx = md<3>(J - ST);
           double r0 = X64[sx][0], r1 = X64[sx][1];
           if constexpr (EDGE) {
-            const bool rin = gi >= 1 && gi <= a.nx - 2;
-            if (rin && L.in0) r0 = r0 + (double)x0;
-            if (rin && L.in1) r1 = r1 + (double)x1;
+            bool a0 = L.in0, a1 = L.in1;
+            if constexpr (ROWS) {
+              const bool rin = gi >= 1 && gi <= a.nx - 2;
+              a0 = a0 && rin;
+              a1 = a1 && rin;
+            }
+            r0 = a0 ? r0 + (double)x0 : r0;
+            r1 = a1 ? r1 + (double)x1 : r1;
           } else {
             r0 = r0 + (double)x0;
             r1 = r1 + (double)x1;
@@ -632,7 +688,7 @@ __device__ __forceinline__ void sup_body(const LevelView<T> &a, const STile &tl,
         // the next cycle's pre-smoothing of the rows just stored, then the
         // residual of row q - SE - 1 and the restriction at even row
         // q - SE - 2 (as k_sdown, MultiGridScheme.cs:134-169)
-        s_sweeps<T, NT, EQ, PW2, EDGE, J, ST + 1, SE>(st, a, L, gi0, q);
+        s_sweeps<T, NT, EQ, PW2, EDGE, ROWS, J, ST + 1, SE>(st, a, L, gi0, q);
         {
           constexpr int d = SE + 1;
           constexpr int pr = J - d;
@@ -654,7 +710,7 @@ __device__ __forceinline__ void sup_body(const LevelView<T> &a, const STile &tl,
             const T rim = RR[md<3>(px - 1)][0], rip = RR[md<3>(px + 1)][0];
             const int I = (gi0 + p) / 2;
             bool go = p >= H && p < H + R;
-            if constexpr (EDGE) go = go && I >= 1 && I <= ncx - 2;
+            if constexpr (ROWS) go = go && I >= 1 && I <= ncx - 2;
             if (go && oc_c) a.rc[I * pc + Jc] = T(0.125) * (r0 * T(4) + (rip + rim + rjp + rjm));
           }
         }
@@ -671,7 +727,7 @@ __device__ __forceinline__ void sup_body(const LevelView<T> &a, const STile &tl,
         const double xl = from_left(x1), xr = from_right(x0);
         const T e0 = st.template full<ST, J, d, 0>(), e1 = st.template full<ST, J, d, 1>();
         bool orow = p >= H && p < H + R;
-        if constexpr (EDGE) orow = orow && gi >= 1 && gi <= a.nx - 2;
+        if constexpr (ROWS) orow = orow && gi >= 1 && gi <= a.nx - 2;
         if (orow && L.oc) {
           const double u20 = x0 * 2.0, u21 = x1 * 2.0;
           double res0, res1;
@@ -715,7 +771,7 @@ __device__ __forceinline__ void sup_body(const LevelView<T> &a, const STile &tl,
         const T a1 = st.template full<ST, J, d + 1, 1>(), b1 = st.template full<ST, J, d - 1, 1>();
         const T xl = from_left(x1), xr = from_right(x0);
         bool orow = p >= H && p < H + R;
-        if constexpr (EDGE) orow = orow && gi >= 1 && gi <= a.nx - 2;
+        if constexpr (ROWS) orow = orow && gi >= 1 && gi <= a.nx - 2;
         if (orow && L.oc) {
           const double r0 = (double)s_res<T, PW2>(x0, a0, b0, xl, x1, st.F[md<P>(pr)][0], a);
           const double r1 = (double)s_res<T, PW2>(x1, a1, b1, x0, xr, st.F[md<P>(pr)][1], a);
@@ -724,6 +780,9 @@ __device__ __forceinline__ void sup_body(const LevelView<T> &a, const STile &tl,
         }
       }
     });
+    };
+    if (rows_in) body(std::false_type{});
+    else body(std::true_type{});
   }
   if constexpr (EPI == 1 || EPI == 3) {
     acc = wave_sum(acc);
@@ -754,18 +813,11 @@ __global__ void __launch_bounds__(64 * kWaves) k_sup(LevelView<T> a, const T *__
     s_idle(a.rows + 2 * H);
     return;
   }
-  // Interior tiles run a body without boundary predicates only in the plain
-  // ascent (EPI 0, the levels below level 0), where the split is faster.  With
-  // an epilogue a kernel holding the predicated body alone is faster
-  // (measured: its schedule overlaps the loads better, and some instances
-  // need fewer registers), so every tile runs it.
+  // Every tile runs the column-predicated body; its loop bodies drop the row
+  // checks wherever all their rows are interior rows (measured faster than a
+  // separate body for interior tiles, which also needs more registers).
   if (a.pw2) {
-    if constexpr (EPI == 0) {
-      if (tl.interior) sup_body<T, NS, N1, EPI, EQ, true, false, Z, NB>(a, tl, uc, ncx, ncy, pc, part, mx);
-      else sup_body<T, NS, N1, EPI, EQ, true, true, Z, NB>(a, tl, uc, ncx, ncy, pc, part, mx);
-    } else {
-      sup_body<T, NS, N1, EPI, EQ, true, true, Z, NB>(a, tl, uc, ncx, ncy, pc, part, mx);
-    }
+    sup_body<T, NS, N1, EPI, EQ, true, true, Z, NB>(a, tl, uc, ncx, ncy, pc, part, mx);
   } else {
     sup_body<T, NS, N1, EPI, EQ, false, true, Z, NB>(a, tl, uc, ncx, ncy, pc, part, mx);
   }

@@ -3,6 +3,8 @@
     python -m vlp2d_amd.rocprof_summary trace <kernel_trace.csv> [<kernel_stats.csv>]
     python -m vlp2d_amd.rocprof_summary pmc <fetch counter_collection.csv> \
         <write counter_collection.csv> --copy-bytes B
+    python -m vlp2d_amd.rocprof_summary solve <fetch counter_collection.csv> \
+        <write counter_collection.csv> --copy-bytes B --label LABEL
 
 ``trace`` groups kernel dispatches by kernel and grid size (launches on
 different multigrid levels differ in grid size) and prints launches, average
@@ -18,6 +20,15 @@ times the nodes of the next level) it adds a line
     TRAFFIC <class>@level0/<f32|f64> <bytes>
 
 where <class> is the launch class the library reports in its kernel timings.
+
+``solve`` sums the calibrated traffic of every dispatch of the last solve in
+the run (``bench.py --only <precision> --steps 1 --warmup 1``: the dispatches
+after the last device copy, which restores the iterate before each solve, up
+to the end) and prints the per-kernel breakdown and a line
+
+    SOLVE_TRAFFIC <label> <bytes> <dispatches>
+
+with the HBM bytes one whole solve moves.
 """
 import argparse
 import collections
@@ -162,6 +173,55 @@ def pmc_report(fetch_csv, write_csv, copy_bytes):
     return "\n".join(lines)
 
 
+def dispatch_values(path, counter):
+    """[(dispatch id, short kernel name, grid, value)] in dispatch order
+    (values of one dispatch summed over the rows that carry them)."""
+    acc = collections.OrderedDict()
+    for r in csv.DictReader(open(path)):
+        if r["Counter_Name"] != counter:
+            continue
+        did = int(r.get("Dispatch_Id") or r.get("Correlation_Id") or len(acc))
+        if did not in acc:
+            acc[did] = [short_name(r["Kernel_Name"]), grid_of(r), 0.0]
+        acc[did][2] += float(r["Counter_Value"])
+    return [(d, v[0], v[1], v[2]) for d, v in sorted(acc.items())]
+
+
+def last_solve(rows):
+    """Dispatches after the last device copy (the restore before the solve)."""
+    last = max((k for k, r in enumerate(rows) if COPY_PATTERN.search(r[1])), default=None)
+    if last is None:
+        raise SystemExit("no device copy in the run: cannot find the solve")
+    return rows[last + 1:]
+
+
+def solve_report(fetch_csv, write_csv, copy_bytes, label):
+    fetch = dispatch_values(fetch_csv, "FETCH_SIZE")
+    write = dispatch_values(write_csv, "WRITE_SIZE")
+    cf_copy = [r[3] for r in fetch if COPY_PATTERN.search(r[1])]
+    cw_copy = [r[3] for r in write if COPY_PATTERN.search(r[1])]
+    cf = copy_bytes / (max(cf_copy) * 1024.0)
+    cw = copy_bytes / (max(cw_copy) * 1024.0)
+    fs, ws = last_solve(fetch), last_solve(write)
+    if [(r[1], r[2]) for r in fs] != [(r[1], r[2]) for r in ws]:
+        raise SystemExit("the FETCH_SIZE and WRITE_SIZE runs dispatched different solves")
+    per = collections.defaultdict(lambda: [0, 0.0])
+    total = 0.0
+    for f, w in zip(fs, ws):
+        b = f[3] * 1024.0 * cf + w[3] * 1024.0 * cw
+        per[(f[1], f[2])][0] += 1
+        per[(f[1], f[2])][1] += b
+        total += b
+    lines = [f"Calibration: the run's device copy of {copy_bytes:.0f} B reads FETCH_SIZE {max(cf_copy):.0f} KiB and "
+             f"WRITE_SIZE {max(cw_copy):.0f} KiB, so FETCH_SIZE is scaled by {cf:.3f} and WRITE_SIZE by {cw:.3f}.",
+             "", f"One solve ({label}): {len(fs)} dispatches, {total:.4g} calibrated HBM bytes.", "",
+             "| kernel | grid (threads) | dispatches | calibrated HBM bytes | share |", "|---|---|---|---|---|"]
+    for (name, grid), (n, b) in sorted(per.items(), key=lambda kv: -kv[1][1]):
+        lines.append(f"| `{name}` | {grid} | {n} | {b:.4g} | {100 * b / total:.1f}% |")
+    lines += ["", "```", f"SOLVE_TRAFFIC {label.replace(' ', '')} {total:.0f} {len(fs)}", "```"]
+    return "\n".join(lines)
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     sub = ap.add_subparsers(dest="mode", required=True)
@@ -172,14 +232,21 @@ def main(argv=None):
     p.add_argument("fetch_csv")
     p.add_argument("write_csv")
     p.add_argument("--copy-bytes", type=float, required=True)
+    q = sub.add_parser("solve")
+    q.add_argument("fetch_csv")
+    q.add_argument("write_csv")
+    q.add_argument("--copy-bytes", type=float, required=True)
+    q.add_argument("--label", required=True)
     a = ap.parse_args(argv)
     if a.mode == "trace":
         print(trace_table(a.kernel_trace))
         if a.kernel_stats:
             print()
             print(stats_table(a.kernel_stats))
-    else:
+    elif a.mode == "pmc":
         print(pmc_report(a.fetch_csv, a.write_csv, a.copy_bytes))
+    else:
+        print(solve_report(a.fetch_csv, a.write_csv, a.copy_bytes, a.label))
 
 
 if __name__ == "__main__":
