@@ -36,8 +36,8 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
 PMC_SUMMARY = os.path.join(ROOT, "profiles", "poisson8193_level0_pmc.md")
 SOLVE_PMC_SUMMARY = os.path.join(ROOT, "profiles", "poisson8193_solve_pmc.md")
 # launch classes that run on the FP32 hierarchy in a mixed-precision solve
-FP32_CLASSES_IN_MIXED = {"down_fused", "up_fused", "up_fused_refine", "up_lds", "coarse_vcycle", "rb_half", "residual", "restrict",
-                         "prolong", "fill"}
+FP32_CLASSES_IN_MIXED = {"down_fused", "down_fused_resnorm64", "up_fused", "up_fused_refine", "up_down_fused", "up_lds",
+                         "coarse_vcycle", "rb_half", "residual", "restrict", "prolong", "fill"}
 
 
 def log(msg):

@@ -41,6 +41,7 @@ enum KClass {
   KC_UP_LDS,
   KC_UP_REFINE,
   KC_UP_DOWN,
+  KC_DOWN_R64,
   KC_COUNT
 };
 
@@ -196,5 +197,8 @@ template <typename T, int NS>
 int launch_up_ns(Ctx *c, Lvl<T> &f, const Lvl<T> &g, int epi, bool rz, bool zi, const MixView &mx);
 // Level 0 between the two FP32 cycles of a mixed step (k_sup epilogue 4).
 template <int NS> void launch_up_down0_ns(Ctx *c, Lvl<float> &f, Lvl<float> &g);
+// First FP32 descent of a mixed solve (one pre-sweep), its right-hand side the
+// FP64 residual of mx.u64in (k_sdown_r64).  Returns the partials written.
+int launch_down_r64(Ctx *c, Lvl<float> &f, Lvl<float> &g, const MixView &mx);
 
 }  // namespace vlp2d

@@ -46,7 +46,7 @@ thread_local std::string g_create_error;
 const char *kKClassName[KC_COUNT] = {"rb_half",   "residual",     "restrict", "fill",    "prolong",
                                      "copy_interior", "absdiff_max", "resnorm64", "reduce", "update_mixed",
                                      "adjoint",   "down_fused",   "up_fused", "coarse_vcycle", "setup",
-                                     "up_lds",    "up_fused_refine", "up_down_fused"};
+                                     "up_lds",    "up_fused_refine", "up_down_fused", "down_fused_resnorm64"};
 
 #define HIPCHK(ctx, call)                                                \
   do {                                                                   \
@@ -408,13 +408,26 @@ template <typename T> void coarse(Ctx *c, Hier<T> &H, bool zero_first, double *s
 // returns the number of partials it writes (0 when lc == 0).  from / to:
 // first descent / last ascent level (1: level 0's kernel is fused with the
 // neighbouring cycle's, up_down0).
+void down_r64(Ctx *c, Lvl<float> &f, Lvl<float> &g, const MixView &mx);
+
+// r64 (FP32 hierarchies, zero0, from == 0): the level-0 descent takes the FP64
+// residual of r64->u64in as its right-hand side (down_r64).
 template <typename T>
-int vcycle_fused(Ctx *c, Hier<T> &H, bool zero0, int epi, const MixView &mx, double *snap, int from = 0, int to = 0) {
+int vcycle_fused(Ctx *c, Hier<T> &H, bool zero0, int epi, const MixView &mx, double *snap, int from = 0, int to = 0,
+                 const MixView *r64 = nullptr) {
   const int lc = H.lc;
   auto zi = [&](int l) { return l > 0 || zero0; };
   auto ep = [&](int l) { return l == 0 ? epi : (int)EPI_NONE; };
   auto rz = [&](int l) { return recompute_presmooth(c, zi(l), ep(l)) && !lds_ascent(c, H, l); };
-  for (int l = from; l < lc; ++l) down(c, H.lv[l], H.lv[l + 1], l, zi(l), !rz(l));
+  for (int l = from; l < lc; ++l) {
+    if constexpr (sizeof(T) == 4) {
+      if (l == 0 && r64) {
+        down_r64(c, H.lv[0], H.lv[1], *r64);
+        continue;
+      }
+    }
+    down(c, H.lv[l], H.lv[l + 1], l, zi(l), !rz(l));
+  }
   coarse(c, H, lc > 0 || zero0, lc == 0 ? snap : nullptr);
   int np = 0;
   for (int l = lc - 1; l >= to; --l) {
@@ -464,6 +477,26 @@ void reduce_part(Ctx *c, const double *part, int np, int slot, bool mx) {
     else
       hipLaunchKernelGGL(k_reduce<false>, dim3(1), dim3(RED_THREADS), 0, c->st, part, np, c->dscal + slot);
   });
+}
+
+// First FP32 descent of a mixed solve with the FP64 residual computed inside
+// (u64 and fn read, r32 written: 20 B per fine node, plus the coarse
+// right-hand side); ||r0||^2 -> dscal[2].
+void down_r64(Ctx *c, Lvl<float> &f, Lvl<float> &g, const MixView &mx) {
+  int np = 0;
+  launch(c, KC_DOWN_R64, 0, 20.0 * nodes(f) + 4.0 * nodes(g), [&] { np = launch_down_r64(c, f, g, mx); });
+  reduce_part(c, c->part, np, 2, false);
+}
+
+// Whether the first refinement step of a mixed solve computes the initial
+// FP64 residual inside its first descent (one pre-sweep on fused levels, the
+// FP32 and FP64 level 0 sharing pitch and power-of-two steps) instead of a
+// separate residual pass before it.
+bool fused_first_residual(const Ctx *c, const Lvl<double> &P) {
+  const Hier<float> &Z = c->h32;
+  if (c->simple || Z.lv.empty() || Z.lc == 0) return false;
+  const Lvl<float> &z = Z.lv[0];
+  return z.pw2 == P.pw2 && z.pitch == P.pitch && recompute_presmooth(c, true, EPI_NONE);
 }
 
 // max |u - snap| over interior nodes -> dscal[0]; pub: published for the
@@ -517,7 +550,10 @@ void up_down0(Ctx *c, Lvl<float> &f, Lvl<float> &g) {
 // h32 level 0 are swapped by the caller afterwards.
 // m >= 2: the correction equation gets m FP32 V-cycles, the first from
 // e = 0, the refinement update follows the last.
-void mixed_step(Ctx *c, Lvl<double> &P, int m = 1, bool pub = false) {
+// first: the first step of a solve; when fused_first_residual holds, its first
+// descent also computes the initial FP64 residual (r32 into z.rhs, ||r0||^2
+// into dscal[2]), which the caller then does not compute beforehand.
+void mixed_step(Ctx *c, Lvl<double> &P, int m = 1, bool pub = false, bool first = false) {
   Hier<float> &Z = c->h32;
   Lvl<float> &z = Z.lv[0];
   MixView none{};
@@ -532,13 +568,16 @@ void mixed_step(Ctx *c, Lvl<double> &P, int m = 1, bool pub = false) {
   mx.ihy2 = P.ihy2;
   mx.pw2 = P.pw2 ? 1 : 0;
   mx.part_max = c->part2;
+  MixView rv = mx;
+  rv.r32out = z.rhs;
+  const MixView *r64 = first && fused_first_residual(c, P) ? &rv : nullptr;
   const bool fused_refine = !c->simple && Z.lc > 0 && !(c->o.flags & VLP2D_FLAG_SEPARATE_REFINE) &&
                             z.pw2 == P.pw2 && z.pitch == P.pitch;
   if (m == 2 && fused_refine && c->o.nu1 == 1 && c->o.nu2 <= 3 &&
       !(c->o.flags & (VLP2D_FLAG_STORE_PRESMOOTH | VLP2D_FLAG_NO_CYCLE_FUSION))) {
     // first cycle down to level 1, level 0 of both cycles in between in one
     // launch, second cycle from level 1 with the refinement ascent
-    vcycle_fused(c, Z, true, EPI_NONE, none, nullptr, 0, 1);
+    vcycle_fused(c, Z, true, EPI_NONE, none, nullptr, 0, 1, r64);
     up_down0(c, z, Z.lv[1]);
     const int np = vcycle_fused(c, Z, false, EPI_MIXED, mx, nullptr, 1, 0);
     finish(c, c->part2, np, c->part, np, pub);
@@ -547,7 +586,7 @@ void mixed_step(Ctx *c, Lvl<double> &P, int m = 1, bool pub = false) {
   // all cycles but the last (correction e -> z.u)
   for (int k = 0; k + 1 < m; ++k) {
     if (c->simple) vcycle_simple(c, Z, k == 0, nullptr);
-    else vcycle_fused(c, Z, k == 0, EPI_NONE, none, nullptr);
+    else vcycle_fused(c, Z, k == 0, EPI_NONE, none, nullptr, 0, 0, k == 0 ? r64 : nullptr);
   }
   const bool zero_last = m == 1;
   if (c->simple || Z.lc == 0) {
@@ -566,11 +605,11 @@ void mixed_step(Ctx *c, Lvl<double> &P, int m = 1, bool pub = false) {
     // refinement update inside the level-0 ascent of the FP32 cycle (the
     // kernel evaluates both precisions' divisions the same way, as
     // multiplications when the steps are powers of two)
-    const int np = vcycle_fused(c, Z, zero_last, EPI_MIXED, mx, nullptr);
+    const int np = vcycle_fused(c, Z, zero_last, EPI_MIXED, mx, nullptr, 0, 0, m == 1 ? r64 : nullptr);
     finish(c, c->part2, np, c->part, np, pub);
     return;
   }
-  vcycle_fused(c, Z, zero_last, EPI_NONE, mx, nullptr);  // correction e -> z.u
+  vcycle_fused(c, Z, zero_last, EPI_NONE, mx, nullptr, 0, 0, m == 1 ? r64 : nullptr);  // correction e -> z.u
   const int rows = refine_row_blocks(P.nx);
   const int np = refine_strips(P.ny) * rows;
   const int chunk = (np + 7) / 8;
@@ -621,6 +660,7 @@ template <typename F> int run_program(Ctx *c, int key, F &&body) {
 
 // graph keys: program id * 4 + swap parities of the buffers it captured
 enum Prog { PG_CYCLE_METRIC = 0, PG_CYCLE_NORM = 1, PG_MIXED = 2, PG_ADJ_CYCLE_NORM = 3, PG_ADJ_MIXED = 4 };
+constexpr int kProgFirst = 8;  // or-ed into a mixed program: the first step of a solve
 int gkey(int prog, int par, int rpar, int m = 1) { return ((prog * 16 + m) * 2 + par) * 2 + rpar; }
 
 int sync_read(Ctx *c, int nslots) {
@@ -853,7 +893,10 @@ bool speculate_next(double rel, double rel_prev, int m_prev, bool have_rate, int
 int solve_mixed(Ctx *c, Lvl<double> &P, int prog_mixed, double rtol, int max_cycles, int &k, int &vcycles,
                 double &r0, double &rel) {
   const int M = c->o.max_inner;
-  resnorm(c, P, P.u, c->h32.lv[0].rhs, false, 2);  // ||r0||^2 -> dscal[2], published with step 0
+  // ||r0||^2 -> dscal[2], published with step 0: computed by step 0's first
+  // descent, or by a residual pass before it
+  const bool fuse_r0 = fused_first_residual(c, P);
+  if (!fuse_r0) resnorm(c, P, P.u, c->h32.lv[0].rhs, false, 2);
   HIPCHK(c, hipGetLastError());
   struct Step {
     int m;
@@ -861,8 +904,10 @@ int solve_mixed(Ctx *c, Lvl<double> &P, int prog_mixed, double rtol, int max_cyc
   };
   Step fl[2] = {};
   int nfl = 0;
-  auto launch_step = [&](int m) -> int {
-    RCHK(run_program(c, gkey(prog_mixed, P.par, c->h32.lv[0].rpar, m) + kPubKey, [&] { mixed_step(c, P, m, true); }));
+  auto launch_step = [&](int m, bool first) -> int {
+    first = first && fuse_r0;
+    RCHK(run_program(c, gkey(first ? prog_mixed | kProgFirst : prog_mixed, P.par, c->h32.lv[0].rpar, m) + kPubKey,
+                     [&] { mixed_step(c, P, m, true, first); }));
     swap_mixed(c, P);
     fl[nfl++] = Step{m, ++c->pub_seq};
     return VLP2D_OK;
@@ -883,10 +928,10 @@ int solve_mixed(Ctx *c, Lvl<double> &P, int prog_mixed, double rtol, int max_cyc
     }
     if (nfl == 0) {
       if (k >= max_cycles) return fail(c, VLP2D_E_NOTCONV, "max_cycles reached before rtol");
-      RCHK(launch_step(k == 0 ? M : inner_cycles(rel, rel_prev, m_prev, M, rtol)));
+      RCHK(launch_step(k == 0 ? M : inner_cycles(rel, rel_prev, m_prev, M, rtol), k == 0));
     }
     if (nfl == 1 && k + 1 < max_cycles && speculate_next(rel, rel_prev, m_prev, k > 0, fl[0].m, M, rtol))
-      RCHK(launch_step(M));
+      RCHK(launch_step(M, false));
     RCHK(wait_result(c, fl[0].seq));
     const Step s0 = fl[0];
     fl[0] = fl[1];
@@ -943,12 +988,18 @@ int solve_impl(Ctx *c, Hier<double> &H, int prog_fp64, int prog_mixed, double rt
     c->r32_valid = &H == &c->h64;
     rc = solve_mixed(c, P, prog_mixed, rtol, max_cycles, k, vcycles, r0, rel);
   } else {
-    resnorm(c, P, P.u, mixed ? c->h32.lv[0].rhs : nullptr, true);
-    HIPCHK(c, hipGetLastError());
-    RCHK(publish_wait(c));
-    r0 = std::sqrt(c->hscal[1]);
+    // a mixed solve's first step may compute ||r0|| itself (as solve_mixed)
+    const bool fuse_r0 = mixed && max_cycles >= 1 && fused_first_residual(c, P);
     c->r32_valid = mixed && (&H == &c->h64);
-    if (!std::isfinite(r0)) rc = fail(c, VLP2D_E_NONFINITE, "initial residual is not finite");
+    if (fuse_r0) {
+      r0 = 1.0;  // set from the first step's result
+    } else {
+      resnorm(c, P, P.u, mixed ? c->h32.lv[0].rhs : nullptr, true);
+      HIPCHK(c, hipGetLastError());
+      RCHK(publish_wait(c));
+      r0 = std::sqrt(c->hscal[1]);
+      if (!std::isfinite(r0)) rc = fail(c, VLP2D_E_NONFINITE, "initial residual is not finite");
+    }
     rel = r0 > 0 ? 1.0 : 0.0;
     double rel_prev = 1.0;
     int m_prev = 1;
@@ -961,10 +1012,11 @@ int solve_impl(Ctx *c, Hier<double> &H, int prog_fp64, int prog_mixed, double rt
         rc = fail(c, VLP2D_E_CANCELLED, "cancelled");
         break;
       }
+      const bool first = fuse_r0 && k == 0;
       if (mixed) {
         const int m = k == 0 ? c->o.max_inner : inner_cycles(rel, rel_prev, m_prev, c->o.max_inner, rtol);
-        RCHK(run_program(c, gkey(prog_mixed, P.par, c->h32.lv[0].rpar, m) + kPubKey,
-                         [&] { mixed_step(c, P, m, true); }));
+        RCHK(run_program(c, gkey(first ? prog_mixed | kProgFirst : prog_mixed, P.par, c->h32.lv[0].rpar, m) + kPubKey,
+                         [&] { mixed_step(c, P, m, true, first); }));
         swap_mixed(c, P);
         vcycles += m;
         m_prev = m;
@@ -973,6 +1025,19 @@ int solve_impl(Ctx *c, Hier<double> &H, int prog_fp64, int prog_mixed, double rt
         vcycles += c->o.cycle == VLP2D_CYCLE_FMG ? 2 : 1;
       }
       RCHK(publish_wait(c));
+      if (first) {
+        r0 = std::sqrt(c->hscal[2]);
+        if (!std::isfinite(r0)) {
+          rc = fail(c, VLP2D_E_NONFINITE, "initial residual is not finite");
+          break;
+        }
+        if (!(r0 > 0)) {  // already the discrete solution: the step is undone
+          swap_mixed(c, P);
+          vcycles = 0;
+          rel = 0.0;
+          break;
+        }
+      }
       ++k;
       rel_prev = rel;
       rel = std::sqrt(c->hscal[1]) / r0;

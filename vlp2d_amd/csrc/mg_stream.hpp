@@ -374,11 +374,89 @@ __device__ __forceinline__ void s_idle(int LR) {
 }
 
 // ---------------------------------------------------------------------------
+// FP64 residual as the right-hand side of a zero-initialised FP32 descent (the
+// first descent of a mixed-precision solve, R64): staged row m gets
+// f = (float)(fn - Lap u64), evaluated in FP64 from the iterate rows m-1..m+1
+// and the right-hand-side row m in the order of UtilsOpLap.cs:15-20 (the
+// formula of the refinement epilogue and of k_refine_stream), with the rows
+// loaded two steps ahead into rings.  The tile's output nodes also get r32
+// written and r^2 summed (the initial residual norm of the solve).
+// ---------------------------------------------------------------------------
+struct R64State {
+  double X[6][2];  // iterate rows, slot m % 6 (row m is read from step m - kPf - 3 to m - kPf + 1)
+  double F[3][2];  // right-hand-side rows, slot m % 3
+  double acc;      // sum of r^2 over the tile's output nodes
+};
+
+// Step J (row q): loads iterate row q + kPf + 3 and right-hand-side row
+// q + kPf + 2, then (J >= -kPf) evaluates the residual of staged row
+// m = q + kPf into st.F.  The pre-loop runs J = -kPf - 4 .. -1 with q = J.
+template <typename T, int NS, bool EDGE, bool ROWS, bool PW2, int J>
+__device__ __forceinline__ void r64_stage(SweepState<T, NS> &st, R64State &rs, const LevelView<T> &a,
+                                          const MixView &mx, const Lane &L, unsigned boff8, int gi0, int q, int R,
+                                          int H) {
+  constexpr int P = SweepState<T, NS>::P;
+  {
+    const int mu = q + kPf + 3, gu = gi0 + mu;
+    const int mf = q + kPf + 2, gf = gi0 + mf;
+    Pair<double> x{0.0, 0.0}, f{0.0, 0.0};
+    if constexpr (ROWS) {
+      bool gx = gu >= 0 && gu <= a.nx - 1, gff = mf >= 0 && gf >= 0 && gf <= a.nx - 1;
+      if constexpr (EDGE) {
+        gx = gx && L.ld;
+        gff = gff && L.ld;
+      }
+      if (gx) x = ld_pair(row_of(mx.u64in, gu, a.pitch), boff8);
+      if (gff) f = ld_pair(row_of(mx.fn64, gf, a.pitch), boff8);
+    } else {
+      x = ld_pair(row_of(mx.u64in, gu, a.pitch), boff8);
+      f = ld_pair(row_of(mx.fn64, gf, a.pitch), boff8);
+    }
+    rs.X[md<6>(J + kPf + 3)][0] = x.x0;
+    rs.X[md<6>(J + kPf + 3)][1] = x.x1;
+    rs.F[md<3>(J + kPf + 2)][0] = f.x0;
+    rs.F[md<3>(J + kPf + 2)][1] = f.x1;
+  }
+  if constexpr (J + kPf >= 0) {
+    constexpr int jm = J + kPf;
+    const double u0 = rs.X[md<6>(jm)][0], u1 = rs.X[md<6>(jm)][1];
+    const double a0 = rs.X[md<6>(jm - 1)][0], a1 = rs.X[md<6>(jm - 1)][1];
+    const double c0 = rs.X[md<6>(jm + 1)][0], c1 = rs.X[md<6>(jm + 1)][1];
+    const double l = from_left(u1), rt = from_right(u0);
+    const double u20 = u0 * 2.0, u21 = u1 * 2.0;
+    double res0, res1;
+    if constexpr (PW2) {
+      res0 = rs.F[md<3>(jm)][0] - ((a0 + c0 - u20) * mx.ihx2 + (l + u1 - u20) * mx.ihy2);
+      res1 = rs.F[md<3>(jm)][1] - ((a1 + c1 - u21) * mx.ihx2 + (u0 + rt - u21) * mx.ihy2);
+    } else {
+      res0 = rs.F[md<3>(jm)][0] - ((a0 + c0 - u20) / mx.hx2 + (l + u1 - u20) / mx.hy2);
+      res1 = rs.F[md<3>(jm)][1] - ((a1 + c1 - u21) / mx.hx2 + (u0 + rt - u21) / mx.hy2);
+    }
+    const float r0f = (float)res0, r1f = (float)res1;
+    st.F[md<P>(jm)][0] = r0f;
+    st.F[md<P>(jm)][1] = r1f;
+    const int m = q + kPf, gm = gi0 + m;
+    bool orow = m >= H && m < H + R;
+    if constexpr (ROWS) orow = orow && gm >= 1 && gm <= a.nx - 2;
+    if (orow && L.oc) {
+      const bool o0 = !EDGE || L.in0, o1 = !EDGE || L.in1;
+      float *rrow = row_of(mx.r32out, gm, a.pitch);
+      if (o0 && o1) st_pair(rrow, L.boff, r0f, r1f);
+      else if (o0) rrow[L.gj] = r0f;
+      else if (o1) rrow[L.gj + 1] = r1f;
+      if (o0) rs.acc += res0 * res0;
+      if (o1) rs.acc += res1 * res1;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Descent
 // ---------------------------------------------------------------------------
-template <typename T, int NS, bool ZI, bool STORE, bool EQ, bool PW2, bool EDGE>
+template <typename T, int NS, bool ZI, bool STORE, bool EQ, bool PW2, bool EDGE, bool R64 = false>
 __device__ __forceinline__ void sdown_body(const LevelView<T> &a, const STile &tl, T *__restrict__ rhsc, int ncx,
-                                           int ncy, int pc) {
+                                           int ncy, int pc, const MixView &mx, double *__restrict__ part) {
+  static_assert(!R64 || (ZI && !STORE && sizeof(T) == 4), "the FP64-residual descent starts the FP32 correction");
   constexpr int S = 2 * NS, H = s_down_halo(NS), P = s_period(NS), TJ = SCOLS - 2 * H;
   const int R = a.rows, LR = R + 2 * H;  // LR is a multiple of P
   const int gi0 = tl.gi0, cb = tl.cb;
@@ -388,11 +466,26 @@ __device__ __forceinline__ void sdown_body(const LevelView<T> &a, const STile &t
   T RR[3][2];  // residual of rows (slot p % 3); element 1 only on even rows
 #pragma unroll
   for (int k = 0; k < 3; ++k) RR[k][0] = RR[k][1] = T(0);
-  // rows 0 .. kPf - 1 (the loop prefetches q + kPf)
-  static_for<-kPf, 0>([&](auto jc) {
-    constexpr int J = decltype(jc)::value;
-    s_prefetch<T, NS, ZI, EDGE, true, J>(st, a, L, gi0, J, LR);
-  });
+  R64State rs;
+  const unsigned boff8 = L.ld ? (unsigned)L.gj * 8u : 0u;
+  if constexpr (R64) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) rs.X[k][0] = rs.X[k][1] = 0.0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) rs.F[k][0] = rs.F[k][1] = 0.0;
+    rs.acc = 0.0;
+    // iterate rows -1 .. kPf + 2, right-hand-side rows 0 .. kPf + 1, F of rows 0 .. kPf - 1
+    static_for<-kPf - 4, 0>([&](auto jc) {
+      constexpr int J = decltype(jc)::value;
+      r64_stage<T, NS, EDGE, true, PW2, J>(st, rs, a, mx, L, boff8, gi0, J, R, H);
+    });
+  } else {
+    // rows 0 .. kPf - 1 (the loop prefetches q + kPf)
+    static_for<-kPf, 0>([&](auto jc) {
+      constexpr int J = decltype(jc)::value;
+      s_prefetch<T, NS, ZI, EDGE, true, J>(st, a, L, gi0, J, LR);
+    });
+  }
   const int Jc = L.m + cb / 2;  // coarse column of element 0 (fine column 2 Jc)
   const bool oc_c = EDGE ? L.oc && Jc >= 1 && Jc <= ncy - 2 : L.oc;
   T *const rc = rhsc + Jc;
@@ -407,7 +500,8 @@ __device__ __forceinline__ void sdown_body(const LevelView<T> &a, const STile &t
       constexpr int J = decltype(jc)::value;
       const int q = q0 + J;
       group_step();
-      s_prefetch<T, NS, ZI, EDGE, ROWS, J>(st, a, L, gi0, q, LR);
+      if constexpr (R64) r64_stage<T, NS, EDGE, ROWS, PW2, J>(st, rs, a, mx, L, boff8, gi0, q, R, H);
+      else s_prefetch<T, NS, ZI, EDGE, ROWS, J>(st, a, L, gi0, q, LR);
       // level 0 of row q: red element (J & 1), black element
       constexpr int er = J & 1;
       st.U[0][md<6>(J)] = st.RU[md<kNru>(J)][er];
@@ -452,6 +546,10 @@ __device__ __forceinline__ void sdown_body(const LevelView<T> &a, const STile &t
     if (rows_in) body(std::false_type{});
     else body(std::true_type{});
   }
+  if constexpr (R64) {
+    const double v = wave_sum(rs.acc);
+    if (L.m == 0) part[tl.t] = v;
+  }
 }
 
 // STORE = false: the smoothed iterate is not written (the ascent recomputes
@@ -464,10 +562,28 @@ __global__ void __launch_bounds__(64 * kWaves) k_sdown(LevelView<T> a, T *__rest
     s_idle(a.rows + 2 * H);
     return;
   }
-  // every tile runs the predicated body (measured faster than a separate
-  // interior body for these kernels)
-  if (a.pw2) sdown_body<T, NS, ZI, STORE, EQ, true, true>(a, tl, rhsc, ncx, ncy, pc);
-  else sdown_body<T, NS, ZI, STORE, EQ, false, true>(a, tl, rhsc, ncx, ncy, pc);
+  // every tile runs the column-predicated body
+  if (a.pw2) sdown_body<T, NS, ZI, STORE, EQ, true, true>(a, tl, rhsc, ncx, ncy, pc, MixView{}, nullptr);
+  else sdown_body<T, NS, ZI, STORE, EQ, false, true>(a, tl, rhsc, ncx, ncy, pc, MixView{}, nullptr);
+}
+
+// The first FP32 descent of a mixed-precision solve on level 0 (zero
+// correction, pre-smoothed iterate not stored), its right-hand side the FP64
+// residual of mx.u64in computed as the rows stream in (R64): writes r32 =
+// (float)(fn - Lap u64) into mx.r32out on interior nodes, the restricted
+// residual into rhsc, and one partial sum of r^2 per tile into part.  The
+// FP64 steps of mx are powers of two exactly when the FP32 steps of a are.
+template <int NS, bool EQ>
+__global__ void __launch_bounds__(64 * kWaves) k_sdown_r64(LevelView<float> a, float *__restrict__ rhsc, int ncx,
+                                                           int ncy, int pc, MixView mx, double *__restrict__ part) {
+  constexpr int H = s_down_halo(NS);
+  STile tl;
+  if (!s_tile<H>(a.rows, a.tiles_i, a.tiles_j, a.chunk, a.nx, a.ny, tl)) {
+    s_idle(a.rows + 2 * H);
+    return;
+  }
+  if (a.pw2) sdown_body<float, NS, true, false, EQ, true, true, true>(a, tl, rhsc, ncx, ncy, pc, mx, part);
+  else sdown_body<float, NS, true, false, EQ, false, true, true>(a, tl, rhsc, ncx, ncy, pc, mx, part);
 }
 
 // ---------------------------------------------------------------------------
@@ -627,20 +743,24 @@ __device__ __forceinline__ void sup_body(const LevelView<T> &a, const STile &tl,
       }
       if constexpr (EPI == 3) {
         // prefetch FP64 rows: iterate row q - ST + 2, right-hand side row q - ST + 1
+        // only the rows the FP64 residual of the output rows [H, H + R) reads:
+        // iterate rows H - 1 .. H + R, right-hand-side rows H .. H + R - 1
+        // (the slots of the other rows hold zeros no output uses)
         const int r = q - ST + 2, gr = gi0 + r;
         Pair<double> x{0.0, 0.0}, f{0.0, 0.0};
         const int rf = q - ST + 1, gf = gi0 + rf;
+        const bool need_x = r >= H - 1 && r <= H + R, need_f = rf >= H && rf < H + R;
         if constexpr (ROWS) {
-          bool go = r >= 0 && r < LR;
+          bool go = need_x;
           if constexpr (EDGE) go = go && gr >= 0 && gr <= a.nx - 1 && L.ld;
           if (go) x = ld_pair(row_of(mx.u64in, gr, a.pitch), boff8);
-          bool gof = rf >= 0 && rf < LR;
+          bool gof = need_f;
           if constexpr (EDGE) gof = gof && gf >= 0 && gf <= a.nx - 1 && L.ld;
           if (gof) f = ld_pair(row_of(mx.fn64, gf, a.pitch), boff8);
         } else {
-          // both rows are domain rows (values of rows before the tile are unused)
-          x = ld_pair(row_of(mx.u64in, gr, a.pitch), boff8);
-          f = ld_pair(row_of(mx.fn64, gf, a.pitch), boff8);
+          // both rows are domain rows
+          if (need_x) x = ld_pair(row_of(mx.u64in, gr, a.pitch), boff8);
+          if (need_f) f = ld_pair(row_of(mx.fn64, gf, a.pitch), boff8);
         }
         X64[md<3>(J - ST + 2)][0] = x.x0;
         X64[md<3>(J - ST + 2)][1] = x.x1;
@@ -797,13 +917,28 @@ __device__ __forceinline__ void sup_body(const LevelView<T> &a, const STile &tl,
   }
 }
 
+// experiment: wavefronts per SIMD the compiler keeps registers for
+template <typename T, int EPI> constexpr int s_up_waves() {
+#ifdef VLP2D_WPE_REFINE
+  if (EPI == 3) return VLP2D_WPE_REFINE;
+#endif
+#ifdef VLP2D_WPE_BND
+  if (EPI == 4) return VLP2D_WPE_BND;
+#endif
+#ifdef VLP2D_WPE_UP0
+  if (EPI == 0) return VLP2D_WPE_UP0;
+#endif
+  return 1;
+}
+
 // EPI 3 requires the FP64 steps of mx to be powers of two exactly when the
 // FP32 steps of a are (the host checks).
 // N1 > 0 pairs with k_sdown<T, N1, ZS, EQ, false> on the same level: ZS, the
 // descent started from zero; otherwise from a.uin, which the ascent then
 // stages (it may not write a.uin: only EPI 3, which writes no iterate).
 template <typename T, int NS, int EPI, bool EQ, int N1 = 0, bool ZS = true, int NB = 0>
-__global__ void __launch_bounds__(64 * kWaves) k_sup(LevelView<T> a, const T *__restrict__ uc, int ncx, int ncy, int pc,
+__global__ void __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(s_up_waves<T, EPI>())))
+k_sup(LevelView<T> a, const T *__restrict__ uc, int ncx, int ncy, int pc,
                                             double *__restrict__ part, MixView mx) {
   constexpr int H = s_up_halo(N1 + NS + NB, EPI);
   constexpr bool Z = ZS && N1 > 0;

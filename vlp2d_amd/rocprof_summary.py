@@ -39,6 +39,7 @@ import sys
 # kernel function name -> launch class reported by vlp2d_mg_kernel_times
 KERNEL_CLASS = {
     "k_sdown": "down_fused",
+    "k_sdown_r64": "down_fused_resnorm64",
     "k_sup": "up_fused",
     "k_lt_up": "up_lds",
     "k_coarse": "coarse_vcycle",
@@ -59,8 +60,8 @@ KERNEL_CLASS = {
 }
 
 # classes whose largest grid is the level-0 (finest grid) launch
-LEVEL0_CLASSES = {"down_fused", "up_fused", "up_fused_refine", "up_down_fused", "update_mixed", "resnorm64", "rb_half", "residual", "restrict",
-                  "prolong", "copy_interior", "absdiff_max"}
+LEVEL0_CLASSES = {"down_fused", "down_fused_resnorm64", "up_fused", "up_fused_refine", "up_down_fused", "update_mixed",
+                  "resnorm64", "rb_half", "residual", "restrict", "prolong", "copy_interior", "absdiff_max"}
 
 COPY_PATTERN = re.compile(r"copyBuffer", re.I)
 
@@ -77,7 +78,7 @@ def short_name(full):
 def base_and_precision(short):
     base = short.split("<", 1)[0]
     args = short[len(base):]
-    prec = "f32" if "float" in args.split(",", 1)[0] else "f64"
+    prec = "f32" if "float" in args.split(",", 1)[0] or base == "k_sdown_r64" else "f64"
     return base, prec
 
 
