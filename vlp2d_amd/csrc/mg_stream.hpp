@@ -32,11 +32,14 @@
 // edge hold partial results after half-sweep s; the halo H covers them.
 //
 // Every tile keeps Dirichlet nodes fixed with per-lane column predicates
-// (one select per half-sweep).  Row checks (Dirichlet rows, rows outside the
-// domain, predicated loads) are needed only in the loop bodies of tiles at
-// the top and bottom of the domain: each loop body of P rows runs in one of
-// two compiled forms, with row checks (ROWS) or, when every row it touches is
-// an interior row, without them and with unconditional loads.
+// (EDGE: one select per half-sweep; a separate body without them for tiles
+// whose columns are all interior measured no faster).  Row checks (Dirichlet
+// rows, rows outside the domain, predicated loads) are needed only in the
+// loop bodies of tiles at the top and bottom of the domain: each loop body of
+// P rows runs in one of two compiled forms, with row checks (ROWS) or, when
+// every row it touches is an interior row, without them and with
+// unconditional loads.  Row checks depend on ROWS alone and column checks on
+// EDGE alone.
 //
 // Kernels:
 //   k_sdown : [zero or staged iterate] -> S = 2*NS half-sweeps -> residual
@@ -296,8 +299,9 @@ __device__ __forceinline__ void s_sweeps(SweepState<T, NS> &st, const LevelView<
       ujp = from_right(mid);
     }
     const T g = s_gs<T, EQ, PW2>(above, below, ujm, ujp, st.F[md<P>(p)][e], a);
-    if constexpr (EDGE) {
-      bool ok = e == 0 ? L.in0 : L.in1;
+    if constexpr (EDGE || ROWS) {
+      bool ok = true;
+      if constexpr (EDGE) ok = e == 0 ? L.in0 : L.in1;
       if constexpr (ROWS) {
         const int gi = gi0 + q - s;
         ok = ok && gi >= 1 && gi <= a.nx - 2;
@@ -321,8 +325,8 @@ __device__ __forceinline__ void s_prefetch(SweepState<T, NS> &st, const LevelVie
   const int qq = q + kPf, gi = gi0 + qq;
   Pair<T> u{T(0), T(0)}, f{T(0), T(0)};
   if constexpr (ROWS) {
-    bool go = qq < LR;
-    if constexpr (EDGE) go = go && gi >= 0 && gi <= a.nx - 1 && L.ld;
+    bool go = qq < LR && gi >= 0 && gi <= a.nx - 1;
+    if constexpr (EDGE) go = go && L.ld;
     if (go) {
       if constexpr (!ZI) u = ld_pair(row_of(a.uin, gi, a.pitch), L.boff);
       if (!a.lap) f = ld_pair(row_of(a.f, gi, a.pitch), L.boff);
@@ -342,7 +346,6 @@ __device__ __forceinline__ void s_prefetch(SweepState<T, NS> &st, const LevelVie
 // An interior tile stages only interior nodes of the domain.
 struct STile {
   int t, gi0, cb;
-  bool interior;
 };
 template <int H> __device__ __forceinline__ bool s_tile(int R, int tiles_i, int tiles_j, int chunk, int nx, int ny,
                                                         STile &tl) {
@@ -363,7 +366,6 @@ template <int H> __device__ __forceinline__ bool s_tile(int R, int tiles_i, int 
   tile_coords(tl.t, tiles_i, tiles_j, ti, tj);
   tl.gi0 = ti * R - H;
   tl.cb = tj * TJ - H;
-  tl.interior = tl.gi0 >= 1 && tl.gi0 + LR - 1 <= nx - 2 && tl.cb >= 1 && tl.cb + SCOLS - 1 <= ny - 2;
   return true;
 }
 
@@ -382,23 +384,29 @@ __device__ __forceinline__ void s_idle(int LR) {
 // loaded two steps ahead into rings.  The tile's output nodes also get r32
 // written and r^2 summed (the initial residual norm of the solve).
 // ---------------------------------------------------------------------------
+#ifndef VLP2D_R64D
+#define VLP2D_R64D 2
+#endif
+constexpr int kR64D = VLP2D_R64D;  // steps between the FP64 loads of a row and its residual
+static_assert(kR64D >= 1 && kR64D <= 3, "FP64 prefetch distance of the residual descent");
 struct R64State {
-  double X[6][2];  // iterate rows, slot m % 6 (row m is read from step m - kPf - 3 to m - kPf + 1)
-  double F[3][2];  // right-hand-side rows, slot m % 3
+  double X[6][2];  // iterate rows, slot m % 6 (row m lives from step m - kPf - 1 - kR64D to m - kPf + 1)
+  double F[6][2];  // right-hand-side rows, slot m % 6 (step m - kPf - kR64D to m - kPf)
   double acc;      // sum of r^2 over the tile's output nodes
 };
 
-// Step J (row q): loads iterate row q + kPf + 3 and right-hand-side row
-// q + kPf + 2, then (J >= -kPf) evaluates the residual of staged row
-// m = q + kPf into st.F.  The pre-loop runs J = -kPf - 4 .. -1 with q = J.
+// Step J (row q): loads iterate row q + kPf + 1 + kR64D and right-hand-side
+// row q + kPf + kR64D, then (J >= -kPf) evaluates the residual of staged row
+// m = q + kPf into st.F.  The pre-loop runs J = -kPf - 2 - kR64D .. -1 with
+// q = J.
 template <typename T, int NS, bool EDGE, bool ROWS, bool PW2, int J>
 __device__ __forceinline__ void r64_stage(SweepState<T, NS> &st, R64State &rs, const LevelView<T> &a,
                                           const MixView &mx, const Lane &L, unsigned boff8, int gi0, int q, int R,
                                           int H) {
   constexpr int P = SweepState<T, NS>::P;
   {
-    const int mu = q + kPf + 3, gu = gi0 + mu;
-    const int mf = q + kPf + 2, gf = gi0 + mf;
+    const int mu = q + kPf + 1 + kR64D, gu = gi0 + mu;
+    const int mf = q + kPf + kR64D, gf = gi0 + mf;
     Pair<double> x{0.0, 0.0}, f{0.0, 0.0};
     if constexpr (ROWS) {
       bool gx = gu >= 0 && gu <= a.nx - 1, gff = mf >= 0 && gf >= 0 && gf <= a.nx - 1;
@@ -412,10 +420,10 @@ __device__ __forceinline__ void r64_stage(SweepState<T, NS> &st, R64State &rs, c
       x = ld_pair(row_of(mx.u64in, gu, a.pitch), boff8);
       f = ld_pair(row_of(mx.fn64, gf, a.pitch), boff8);
     }
-    rs.X[md<6>(J + kPf + 3)][0] = x.x0;
-    rs.X[md<6>(J + kPf + 3)][1] = x.x1;
-    rs.F[md<3>(J + kPf + 2)][0] = f.x0;
-    rs.F[md<3>(J + kPf + 2)][1] = f.x1;
+    rs.X[md<6>(J + kPf + 1 + kR64D)][0] = x.x0;
+    rs.X[md<6>(J + kPf + 1 + kR64D)][1] = x.x1;
+    rs.F[md<6>(J + kPf + kR64D)][0] = f.x0;
+    rs.F[md<6>(J + kPf + kR64D)][1] = f.x1;
   }
   if constexpr (J + kPf >= 0) {
     constexpr int jm = J + kPf;
@@ -426,11 +434,11 @@ __device__ __forceinline__ void r64_stage(SweepState<T, NS> &st, R64State &rs, c
     const double u20 = u0 * 2.0, u21 = u1 * 2.0;
     double res0, res1;
     if constexpr (PW2) {
-      res0 = rs.F[md<3>(jm)][0] - ((a0 + c0 - u20) * mx.ihx2 + (l + u1 - u20) * mx.ihy2);
-      res1 = rs.F[md<3>(jm)][1] - ((a1 + c1 - u21) * mx.ihx2 + (u0 + rt - u21) * mx.ihy2);
+      res0 = rs.F[md<6>(jm)][0] - ((a0 + c0 - u20) * mx.ihx2 + (l + u1 - u20) * mx.ihy2);
+      res1 = rs.F[md<6>(jm)][1] - ((a1 + c1 - u21) * mx.ihx2 + (u0 + rt - u21) * mx.ihy2);
     } else {
-      res0 = rs.F[md<3>(jm)][0] - ((a0 + c0 - u20) / mx.hx2 + (l + u1 - u20) / mx.hy2);
-      res1 = rs.F[md<3>(jm)][1] - ((a1 + c1 - u21) / mx.hx2 + (u0 + rt - u21) / mx.hy2);
+      res0 = rs.F[md<6>(jm)][0] - ((a0 + c0 - u20) / mx.hx2 + (l + u1 - u20) / mx.hy2);
+      res1 = rs.F[md<6>(jm)][1] - ((a1 + c1 - u21) / mx.hx2 + (u0 + rt - u21) / mx.hy2);
     }
     const float r0f = (float)res0, r1f = (float)res1;
     st.F[md<P>(jm)][0] = r0f;
@@ -472,10 +480,11 @@ __device__ __forceinline__ void sdown_body(const LevelView<T> &a, const STile &t
 #pragma unroll
     for (int k = 0; k < 6; ++k) rs.X[k][0] = rs.X[k][1] = 0.0;
 #pragma unroll
-    for (int k = 0; k < 3; ++k) rs.F[k][0] = rs.F[k][1] = 0.0;
+    for (int k = 0; k < 6; ++k) rs.F[k][0] = rs.F[k][1] = 0.0;
     rs.acc = 0.0;
-    // iterate rows -1 .. kPf + 2, right-hand-side rows 0 .. kPf + 1, F of rows 0 .. kPf - 1
-    static_for<-kPf - 4, 0>([&](auto jc) {
+    // iterate rows -1 .. kPf + kR64D, right-hand-side rows 0 .. kPf + kR64D - 1,
+    // F of rows 0 .. kPf - 1
+    static_for<-kPf - 2 - kR64D, 0>([&](auto jc) {
       constexpr int J = decltype(jc)::value;
       r64_stage<T, NS, EDGE, true, PW2, J>(st, rs, a, mx, L, boff8, gi0, J, R, H);
     });
@@ -712,11 +721,11 @@ __device__ __forceinline__ void sup_body(const LevelView<T> &a, const STile &tl,
         if constexpr (EDGE) {
           a0 = L.in0;
           a1 = L.in1;
-          if constexpr (ROWS) {
-            const bool rin = gi >= 1 && gi <= a.nx - 2;
-            a0 = a0 && rin;
-            a1 = a1 && rin;
-          }
+        }
+        if constexpr (ROWS) {
+          const bool rin = gi >= 1 && gi <= a.nx - 2;
+          a0 = a0 && rin;
+          a1 = a1 && rin;
         }
         if constexpr (N1 == 0) {
           T v0 = st.RU[md<kNru>(J)][0], v1 = st.RU[md<kNru>(J)][1];
@@ -751,11 +760,11 @@ __device__ __forceinline__ void sup_body(const LevelView<T> &a, const STile &tl,
         const int rf = q - ST + 1, gf = gi0 + rf;
         const bool need_x = r >= H - 1 && r <= H + R, need_f = rf >= H && rf < H + R;
         if constexpr (ROWS) {
-          bool go = need_x;
-          if constexpr (EDGE) go = go && gr >= 0 && gr <= a.nx - 1 && L.ld;
+          bool go = need_x && gr >= 0 && gr <= a.nx - 1;
+          if constexpr (EDGE) go = go && L.ld;
           if (go) x = ld_pair(row_of(mx.u64in, gr, a.pitch), boff8);
-          bool gof = need_f;
-          if constexpr (EDGE) gof = gof && gf >= 0 && gf <= a.nx - 1 && L.ld;
+          bool gof = need_f && gf >= 0 && gf <= a.nx - 1;
+          if constexpr (EDGE) gof = gof && L.ld;
           if (gof) f = ld_pair(row_of(mx.fn64, gf, a.pitch), boff8);
         } else {
           // both rows are domain rows
@@ -780,8 +789,12 @@ __device__ __forceinline__ void sup_body(const LevelView<T> &a, const STile &tl,
           // refined FP64 row q - ST: u64 + e on interior nodes
           constexpr int sx = md<3>(J - ST);
           double r0 = X64[sx][0], r1 = X64[sx][1];
-          if constexpr (EDGE) {
-            bool a0 = L.in0, a1 = L.in1;
+          if constexpr (EDGE || ROWS) {
+            bool a0 = true, a1 = true;
+            if constexpr (EDGE) {
+              a0 = L.in0;
+              a1 = L.in1;
+            }
             if constexpr (ROWS) {
               const bool rin = gi >= 1 && gi <= a.nx - 2;
               a0 = a0 && rin;
@@ -917,28 +930,13 @@ __device__ __forceinline__ void sup_body(const LevelView<T> &a, const STile &tl,
   }
 }
 
-// experiment: wavefronts per SIMD the compiler keeps registers for
-template <typename T, int EPI> constexpr int s_up_waves() {
-#ifdef VLP2D_WPE_REFINE
-  if (EPI == 3) return VLP2D_WPE_REFINE;
-#endif
-#ifdef VLP2D_WPE_BND
-  if (EPI == 4) return VLP2D_WPE_BND;
-#endif
-#ifdef VLP2D_WPE_UP0
-  if (EPI == 0) return VLP2D_WPE_UP0;
-#endif
-  return 1;
-}
-
 // EPI 3 requires the FP64 steps of mx to be powers of two exactly when the
 // FP32 steps of a are (the host checks).
 // N1 > 0 pairs with k_sdown<T, N1, ZS, EQ, false> on the same level: ZS, the
 // descent started from zero; otherwise from a.uin, which the ascent then
 // stages (it may not write a.uin: only EPI 3, which writes no iterate).
 template <typename T, int NS, int EPI, bool EQ, int N1 = 0, bool ZS = true, int NB = 0>
-__global__ void __launch_bounds__(64 * kWaves) __attribute__((amdgpu_waves_per_eu(s_up_waves<T, EPI>())))
-k_sup(LevelView<T> a, const T *__restrict__ uc, int ncx, int ncy, int pc,
+__global__ void __launch_bounds__(64 * kWaves) k_sup(LevelView<T> a, const T *__restrict__ uc, int ncx, int ncy, int pc,
                                             double *__restrict__ part, MixView mx) {
   constexpr int H = s_up_halo(N1 + NS + NB, EPI);
   constexpr bool Z = ZS && N1 > 0;
