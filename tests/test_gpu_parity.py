@@ -367,6 +367,37 @@ def test_performance_mode_matches_oracle(cfg, seg, mixed, inner, nu):
     s.cleanup()
 
 
+@pytest.mark.parametrize("mixed,inner,nu", [(False, 1, (2, 2)), (True, 2, (2, 2)), (False, 1, (1, 3))])
+@pytest.mark.parametrize("cfg,seg", [(2, None), (1, 256)])
+def test_full_size_configs_to_rtol_match_oracle(cfg, seg, mixed, inner, nu):
+    """Config 2 at its full size (4097 x 2049: [0,2] x [0,1], sine-mode f,
+    cubic side boundaries) and config 1 at its full size (257^2 Laplace,
+    g = x^2 - y^2) solved to relative residual 1e-10 from a zero interior with
+    the configs' V(2,2) cycle (and the benchmark's V(1,3)): the same number of
+    cycles or refinement steps as the oracle (MultiGridScheme.cs:63-71 looped
+    to the residual stop) and the same iterate, bit for bit."""
+    p = problems.config(cfg, seg)
+    _say(f"full-size cfg{cfg} {p.seg_x + 1}x{p.seg_y + 1} mixed={mixed} nu={nu}")
+    u0 = p.initial("zero")
+    if mixed:
+        ur, kr, hr = mixed_solve(p.seg_x, p.seg_y, p.hx, p.hy, p.f, u0, nu[0], nu[1], 3, 1e-10, 60, inner=inner)
+    else:
+        ur, kr, hr = plain_solve(p.seg_x, p.seg_y, p.hx, p.hy, p.f, u0, nu[0], nu[1], 3, 1e-10, 60)
+    s = MultiGridSchemeHIP(p.seg_x, p.seg_y, p.hx, p.hy, p.f,
+                           options=Options.performance(mixed, nu1=nu[0], nu2=nu[1], max_inner=inner))
+    s.un0[:] = u0
+    s.init_after_boundaries_and_initial_iteration_inited()
+    st = s.solve(1e-10, 60)
+    ug = s.get_array().copy()
+    s.cleanup()
+    _say(f"  {st['cycles']} steps ({kr} oracle), rel {st['rel_residual']:.3e}")
+    assert st["cycles"] == kr
+    assert st["rel_residual"] <= 1e-10
+    assert np.array_equal(ug, ur), np.abs(ug - ur).max()
+    if cfg == 1:  # the discrete solution of the harmonic quadratic is the quadratic itself
+        assert np.abs(ug - p.exact).max() <= 1e-9
+
+
 def test_config2_full_size_parity():
     """Config 2 at its full size (4097 x 2049, coarsest 5 x 3): FP64 GPU vs
     the oracle's double multigrid at equal cycle counts (V(3,3), Mean init).
@@ -519,6 +550,18 @@ def test_divergent_solve_reports_nonfinite(mixed):
     with pytest.raises(VLP2DError) as e:
         s.solve(1e-10, 1000)
     assert e.value.code == _lib.E_NONFINITE
+    s.cleanup()
+    # the reference's own loop (one doIteration per call, update-norm metric)
+    # on the same diverging problem ends the same way once the update
+    # overflows to Inf or NaN
+    s = MultiGridSchemeHIP(256, 128, 1 / 256, 2 / 128, f, options=Options.performance(mixed))
+    s.un0[:] = g
+    s.init_after_boundaries_and_initial_iteration_inited()
+    with pytest.raises(VLP2DError) as e:
+        for it in range(3000):
+            s.do_iteration(it)
+    assert e.value.code == _lib.E_NONFINITE
+    _say(f"divergent cycles mixed={mixed}: non-finite update after {it + 1} cycles")
     s.cleanup()
 
 
