@@ -463,21 +463,49 @@ def test_forward_cycles_bitwise_full_sizes(seg, K):
 
 
 # ------------------------------------------------------------------ adjoint
-@pytest.mark.parametrize("seg_x,seg_y,hx,hy,K", [(16, 16, 1 / 16, 1 / 16, 2), (64, 32, 0.01, 0.02, 3),
-                                                (1024, 1024, 1 / 1024, 1 / 1024, 2)])
-def test_adjoint_cycles_match_oracle(seg_x, seg_y, hx, hy, K):
-    _say(f"adjoint cycles {seg_x}x{seg_y} K={K}")
+@pytest.mark.parametrize("seg_x,seg_y,hx,hy,K,nu", [(16, 16, 1 / 16, 1 / 16, 2, (2, 2)),
+                                                   (64, 32, 0.01, 0.02, 3, (2, 2)),
+                                                   (1024, 1024, 1 / 1024, 1 / 1024, 2, (2, 2)),
+                                                   (256, 512, 1 / 256, 1 / 512, 2, (1, 3)),
+                                                   (512, 128, 0.002, 0.007, 3, (3, 1)),
+                                                   (128, 128, 1 / 128, 1 / 128, 2, (1, 1)),
+                                                   (256, 256, 1 / 256, 1 / 256, 1, (3, 3))])
+def test_adjoint_cycles_match_oracle(seg_x, seg_y, hx, hy, K, nu):
+    """The transposed sweep (the streaming k_tdown/k_tup pair on every level
+    above the coarsest, one operator per launch there) against the oracle's
+    transposed cycles, bit for bit, for every sweep count the streaming
+    kernels take."""
+    _say(f"adjoint cycles {seg_x}x{seg_y} K={K} nu={nu}")
     f, g = _random(seg_x, seg_y, 99)
     rng = np.random.default_rng(5)
     ubar = rng.standard_normal(g.shape)
-    s = MultiGridSchemeHIP(seg_x, seg_y, hx, hy, f, options=Options.performance())
+    s = MultiGridSchemeHIP(seg_x, seg_y, hx, hy, f, options=Options.performance(nu1=nu[0], nu2=nu[1]))
     s.un0[:] = g
     s.init_after_boundaries_and_initial_iteration_inited()
     s.forward_cycles(K)
     fb, gb, u0b = s.adjoint_cycles(ubar)
-    rf, rg, ru = adjoint_cycles(seg_x, seg_y, hx, hy, ubar, K, 2, 2, 3)
+    rf, rg, ru = adjoint_cycles(seg_x, seg_y, hx, hy, ubar, K, nu[0], nu[1], 3)
     assert np.array_equal(fb, rf) and np.array_equal(gb, rg) and np.array_equal(u0b, ru)
     s.cleanup()
+
+
+def test_adjoint_cycles_streaming_matches_one_operator_path():
+    """The streaming transposed cycle and the one-operator path
+    (VLP2D_FLAG_SIMPLE) give the same bits at 2049 x 1025."""
+    seg_x, seg_y, hx, hy, K = 2048, 1024, 1 / 2048, 1 / 1024, 2
+    _say(f"adjoint cycles streaming vs one-operator {seg_x}x{seg_y}")
+    f, g = _random(seg_x, seg_y, 7)
+    ubar = np.random.default_rng(6).standard_normal(g.shape)
+    out = []
+    for flags in (0, _lib.FLAG_SIMPLE):
+        s = MultiGridSchemeHIP(seg_x, seg_y, hx, hy, f, options=Options.performance(flags=flags))
+        s.un0[:] = g
+        s.init_after_boundaries_and_initial_iteration_inited()
+        s.forward_cycles(K)
+        out.append(s.adjoint_cycles(ubar))
+        s.cleanup()
+    for a, b in zip(*out):
+        assert np.array_equal(a, b)
 
 
 @pytest.mark.parametrize("kw", [{}, dict(smoother=_lib.SMOOTH_WJACOBI), dict(restriction=_lib.RESTRICT_FULL),

@@ -42,6 +42,8 @@ enum KClass {
   KC_UP_REFINE,
   KC_UP_DOWN,
   KC_DOWN_R64,
+  KC_ADJ_DOWN,
+  KC_ADJ_UP,
   KC_COUNT
 };
 
@@ -102,6 +104,7 @@ struct vlp2d_mg_ctx {
   bool r32_valid = false;   // h32 level-0 rhs holds (float)(fn - Lap u64)
   // transposed-cycle buffers
   std::vector<double *> aub, arb, aresb;
+  std::vector<double *> aub2;  // second ubar buffers of the streaming transposed levels
   int K = 0;  // forward cycles recorded for the transposed sweep
   std::map<int, std::pair<hipGraph_t, hipGraphExec_t>> graphs;
   int num_cu = 0;                    // compute units of the device
@@ -200,5 +203,14 @@ template <int NS> void launch_up_down0_ns(Ctx *c, Lvl<float> &f, Lvl<float> &g);
 // First FP32 descent of a mixed solve (one pre-sweep), its right-hand side the
 // FP64 residual of mx.u64in (k_sdown_r64).  Returns the partials written.
 int launch_down_r64(Ctx *c, Lvl<float> &f, Lvl<float> &g, const MixView &mx);
+// Transposed cycle on FP64 level f above the coarsest (mg_k_adj.hip): A, the
+// transposed post-smoothing (ns sweeps) and prolongation into the coarse
+// level (ubc = P^T ubar, rbc = 0); B, the transposed restriction and residual
+// from the coarse rbc and the transposed pre-smoothing (ns sweeps).  ubar
+// out of place (ubin -> ubout), rbar in place (rb).
+void launch_tdown(Ctx *c, int ns, const Lvl<double> &f, const double *ubin, double *ubout, double *rb,
+                  const Lvl<double> &g, double *ubc, double *rbc);
+void launch_tup(Ctx *c, int ns, const Lvl<double> &f, const double *ubin, double *ubout, double *rb,
+                const Lvl<double> &g, double *rbc);
 
 }  // namespace vlp2d

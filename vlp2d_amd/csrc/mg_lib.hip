@@ -46,7 +46,8 @@ thread_local std::string g_create_error;
 const char *kKClassName[KC_COUNT] = {"rb_half",   "residual",     "restrict", "fill",    "prolong",
                                      "copy_interior", "absdiff_max", "resnorm64", "reduce", "update_mixed",
                                      "adjoint",   "down_fused",   "up_fused", "coarse_vcycle", "setup",
-                                     "up_lds",    "up_fused_refine", "up_down_fused", "down_fused_resnorm64"};
+                                     "up_lds",    "up_fused_refine", "up_down_fused", "down_fused_resnorm64",
+                                     "adjoint_down_fused", "adjoint_up_fused"};
 
 #define HIPCHK(ctx, call)                                                \
   do {                                                                   \
@@ -1063,18 +1064,29 @@ int solve_impl(Ctx *c, Hier<double> &H, int prog_fp64, int prog_mixed, double rt
 }
 
 // ---- transposed cycle -------------------------------------------------------
+// Levels above the coarsest run the transposed cycle as two streaming
+// kernels (k_tdown, k_tup), ubar alternating between aub and aub2, rbar in
+// place in arb; VLP2D_FLAG_SIMPLE and sweep counts outside 1..3 keep one
+// operator per launch in place, with resbar in aresb.
+bool t_streamed(const Ctx *c, int l) { return !c->simple && l < c->L - 1; }
+
 int ensure_adj_buffers(Ctx *c) {
   if (!c->aub.empty()) return VLP2D_OK;
   c->aub.assign(c->L, nullptr);
   c->arb.assign(c->L, nullptr);
   c->aresb.assign(c->L, nullptr);
+  c->aub2.assign(c->L, nullptr);
   for (int l = 0; l < c->L; ++l) {
     size_t n = c->h64.lv[l].elems();
     RCHK(dalloc(c, &c->aub[l], n));
     RCHK(dalloc(c, &c->arb[l], n));
     fill(c, c->aub[l], n, 0.0);
     fill(c, c->arb[l], n, 0.0);
-    if (l < c->L - 1) {
+    if (t_streamed(c, l)) {
+      RCHK(dalloc(c, &c->aub2[l], n));
+      fill(c, c->aub2[l], n, 0.0);
+    } else if (l < c->L - 1 || l == 0) {
+      // resbar of the one-operator levels; level 0's also stages the outputs
       RCHK(dalloc(c, &c->aresb[l], n));
       fill(c, c->aresb[l], n, 0.0);
     }
@@ -1104,6 +1116,18 @@ void cycle_T(Ctx *c, int l) {
   }
   Lvl<double> &f = c->h64.lv[l];
   Lvl<double> &g = c->h64.lv[l + 1];
+  if (t_streamed(c, l)) {
+    // aub -> aub2, coarse ubar = P^T ubar, coarse rbar = 0
+    launch(c, KC_ADJ_DOWN, l, 32.0 * nodes(f) + 16.0 * nodes(g), [&] {
+      launch_tdown(c, c->o.nu2, f, c->aub[l], c->aub2[l], c->arb[l], g, c->aub[l + 1], c->arb[l + 1]);
+    });
+    cycle_T(c, l + 1);
+    // aub2 + R^T coarse rbar -> aub
+    launch(c, KC_ADJ_UP, l, 32.0 * nodes(f) + 8.0 * nodes(g), [&] {
+      launch_tup(c, c->o.nu1, f, c->aub2[l], c->aub[l], c->arb[l], g, c->arb[l + 1]);
+    });
+    return;
+  }
   smooth_T(c, l, c->o.nu2);
   launch(c, KC_ADJ, l, 8.0 * (nodes(f) + nodes(g)), [&] {
     hipLaunchKernelGGL(k_adj_prolongT, grid2(g.ny, g.nx), kBlock, 0, c->st, c->aub[l + 1], c->aub[l], g.nx, g.ny, g.pitch, f.pitch);
@@ -1599,20 +1623,19 @@ int vlp2d_mg_adjoint_cycles(vlp2d_mg_handle c, const double *dL_du, double *dL_d
     cycle_T(c, 0);
     HIPCHK(c, hipGetLastError());
   }
+  // outputs formed on the device, one packed nx x ny array at a time in a
+  // staging buffer, then copied to the caller
+  double *stage = c->aub2[0] ? c->aub2[0] : c->aresb[0];
+  double *outs[3] = {dL_df, dL_dg, dL_du0};
+  for (int k = 0; k < 3; ++k) {
+    if (!outs[k]) continue;
+    hipLaunchKernelGGL(k_adj_output, grid2(P.ny, P.nx), kBlock, 0, c->st, stage, c->aub[0], c->arb[0], P.nx, P.ny,
+                       P.pitch, k);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(outs[k], stage, (size_t)P.nx * P.ny * sizeof(double), hipMemcpyDeviceToHost, c->st));
+  }
   HIPCHK(c, hipStreamSynchronize(c->st));
   if (c->profile) collect_profile(c);
-  const int nx = c->nx, ny = c->ny;
-  std::vector<double> ub((size_t)nx * ny), rb((size_t)nx * ny);
-  RCHK(download_grid(c, ub.data(), c->aub[0], P.pitch));
-  RCHK(download_grid(c, rb.data(), c->arb[0], P.pitch));
-  for (int i = 0; i < nx; ++i)
-    for (int j = 0; j < ny; ++j) {
-      size_t q = (size_t)i * ny + j;
-      bool inner = i > 0 && i < nx - 1 && j > 0 && j < ny - 1;
-      if (dL_df) dL_df[q] = inner ? -rb[q] : 0.0;
-      if (dL_dg) dL_dg[q] = inner ? 0.0 : ub[q];
-      if (dL_du0) dL_du0[q] = inner ? ub[q] : 0.0;
-    }
   return VLP2D_OK;
 }
 
@@ -1701,6 +1724,7 @@ void vlp2d_mg_destroy(vlp2d_mg_handle c) {
     dfree(c, c->aub[l], n);
     dfree(c, c->arb[l], n);
     dfree(c, c->aresb[l], n);
+    dfree(c, c->aub2[l], n);
   }
   dfree(c, c->part, (size_t)c->part_cap);
   dfree(c, c->part2, (size_t)c->part_cap);

@@ -147,6 +147,24 @@ __global__ void __launch_bounds__(BX *BY) k_adj_bT(double *__restrict__ gb, cons
   gb[c] = v;
 }
 
+// Outputs of the transposed sweep, packed nx x ny (row pitch ny): kind 0 the
+// right-hand-side gradient -rbar (interior), 1 the Dirichlet-data gradient
+// ubar (boundary), 2 the initial-iterate gradient ubar (interior); 0 elsewhere.
+__global__ void __launch_bounds__(BX *BY) k_adj_output(double *__restrict__ out, const double *__restrict__ ub,
+                                                       const double *__restrict__ rb, int nx, int ny, int pitch,
+                                                       int kind) {
+  const int j = blockIdx.x * BX + threadIdx.x;
+  const int i = blockIdx.y * BY + threadIdx.y;
+  if (i > nx - 1 || j > ny - 1) return;
+  const int c = i * pitch + j;
+  const bool inner = i > 0 && i < nx - 1 && j > 0 && j < ny - 1;
+  double v = 0.0;
+  if (kind == 0) v = inner ? -rb[c] : 0.0;
+  else if (kind == 1) v = inner ? 0.0 : ub[c];
+  else v = inner ? ub[c] : 0.0;
+  out[(size_t)i * ny + j] = v;
+}
+
 }  // namespace vlp2d
 
 namespace vlp2d {

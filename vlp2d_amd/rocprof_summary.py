@@ -57,6 +57,14 @@ KERNEL_CLASS = {
     "k_fill": "fill",
     "k_copy_interior": "copy_interior",
     "k_absdiff_max": "absdiff_max",
+    "k_tdown": "adjoint_down_fused",
+    "k_tup": "adjoint_up_fused",
+    "k_adj_gather": "adjoint",
+    "k_adj_color": "adjoint",
+    "k_adj_prolongT": "adjoint",
+    "k_adj_restrictT": "adjoint",
+    "k_adj_residualT": "adjoint",
+    "k_adj_output": "adjoint_output",
 }
 
 # classes whose largest grid is the level-0 (finest grid) launch

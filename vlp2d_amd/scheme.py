@@ -306,7 +306,7 @@ class MultiGridSchemeHIP:
     def adjoint_cycles(self, dL_du):
         shape = (self.dim_x, self.dim_y)
         ub = np.ascontiguousarray(dL_du, dtype=np.float64).reshape(shape)
-        df, dg, du0 = np.zeros(shape), np.zeros(shape), np.zeros(shape)
+        df, dg, du0 = np.empty(shape), np.empty(shape), np.empty(shape)  # every node is written
         check(self.lib, self.h, self.lib.vlp2d_mg_adjoint_cycles(self.h, ptr(ub), ptr(df), ptr(dg), ptr(du0)))
         return df, dg, du0
 
