@@ -150,7 +150,7 @@ template <typename T> LevelView<T> view(const Lvl<T> &v, const T *uin, T *uout, 
   a.hy2 = v.hy2;
   a.coef = v.coef;
   a.eq = v.eq ? 1 : 0;
-  a.lap = v.lap ? 1 : 0;
+  a.fpitch = v.lap ? 0 : v.pitch;
   a.ihx2 = v.ihx2;
   a.ihy2 = v.ihy2;
   a.pw2 = v.pw2 ? 1 : 0;

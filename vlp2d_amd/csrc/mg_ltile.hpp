@@ -79,8 +79,7 @@ __device__ __forceinline__ void lt_up_body(const LevelView<T> &a, const T *__res
     const bool rok = ld && gi >= 0 && gi <= a.nx - 1;
     const int gic = std::min(std::max(gi, 0), a.nx - 1);
     Pair<T> u = ld_pair(row_of(a.uin, gic, a.pitch), boff);
-    Pair<T> f{T(0), T(0)};
-    if (!a.lap) f = ld_pair(row_of(a.f, gic, a.pitch), boff);
+    const Pair<T> f = ld_pair(row_of(a.f, gic, a.fpitch), boff);
     const int I0 = std::min(gic >> 1, ncx - 1), I1 = std::min(I0 + 1, ncx - 1);
     const T c00 = uc[I0 * pc + J0], c01 = uc[I0 * pc + J1];
     T q0, q1;

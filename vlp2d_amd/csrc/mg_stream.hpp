@@ -318,7 +318,15 @@ __device__ __forceinline__ void s_sweeps(SweepState<T, NS> &st, const LevelView<
 // ROWS = false: every row this loop body touches is an interior row of the
 // domain, so the load needs no predicate: lanes outside the row allocation
 // read its columns 0..1 (L.boff = 0), values that reach no interior node.
-template <typename T, int NS, bool ZI, bool EDGE, bool ROWS, int J>
+// FBR: the right-hand-side load sits behind a uniform branch (skipped on
+// Laplace levels).  The branch makes the compiler wait for the previous
+// step's loads and stores before the step goes on; without it more memory
+// operations stay in flight.  Measured on MI355X at 8193^2, the bandwidth-
+// bound descents and level-0 ascents with a residual epilogue are faster with
+// the branch (refinement ascent 509 against 541 us, FP64 ascent 360 against
+// 371 us), the latency-bound ascents without it (the cycle-boundary kernel
+// 177 against 218 us, the FP64 level-1 ascent 70 against 76 us).
+template <typename T, int NS, bool ZI, bool EDGE, bool ROWS, int J, bool FBR>
 __device__ __forceinline__ void s_prefetch(SweepState<T, NS> &st, const LevelView<T> &a, const Lane &L, int gi0, int q,
                                            int LR) {
   constexpr int P = SweepState<T, NS>::P;
@@ -329,11 +337,19 @@ __device__ __forceinline__ void s_prefetch(SweepState<T, NS> &st, const LevelVie
     if constexpr (EDGE) go = go && L.ld;
     if (go) {
       if constexpr (!ZI) u = ld_pair(row_of(a.uin, gi, a.pitch), L.boff);
-      if (!a.lap) f = ld_pair(row_of(a.f, gi, a.pitch), L.boff);
+      if constexpr (FBR) {
+        if (a.fpitch) f = ld_pair(row_of(a.f, gi, a.fpitch), L.boff);
+      } else {
+        f = ld_pair(row_of(a.f, gi, a.fpitch), L.boff);
+      }
     }
   } else {
     if constexpr (!ZI) u = ld_pair(row_of(a.uin, gi, a.pitch), L.boff);
-    if (!a.lap) f = ld_pair(row_of(a.f, gi, a.pitch), L.boff);
+    if constexpr (FBR) {
+      if (a.fpitch) f = ld_pair(row_of(a.f, gi, a.fpitch), L.boff);
+    } else {
+      f = ld_pair(row_of(a.f, gi, a.fpitch), L.boff);
+    }
   }
   st.RU[md<kNru>(J + kPf)][0] = u.x0;
   st.RU[md<kNru>(J + kPf)][1] = u.x1;
@@ -492,7 +508,7 @@ __device__ __forceinline__ void sdown_body(const LevelView<T> &a, const STile &t
     // rows 0 .. kPf - 1 (the loop prefetches q + kPf)
     static_for<-kPf, 0>([&](auto jc) {
       constexpr int J = decltype(jc)::value;
-      s_prefetch<T, NS, ZI, EDGE, true, J>(st, a, L, gi0, J, LR);
+      s_prefetch<T, NS, ZI, EDGE, true, J, true>(st, a, L, gi0, J, LR);
     });
   }
   const int Jc = L.m + cb / 2;  // coarse column of element 0 (fine column 2 Jc)
@@ -510,7 +526,7 @@ __device__ __forceinline__ void sdown_body(const LevelView<T> &a, const STile &t
       const int q = q0 + J;
       group_step();
       if constexpr (R64) r64_stage<T, NS, EDGE, ROWS, PW2, J>(st, rs, a, mx, L, boff8, gi0, q, R, H);
-      else s_prefetch<T, NS, ZI, EDGE, ROWS, J>(st, a, L, gi0, q, LR);
+      else s_prefetch<T, NS, ZI, EDGE, ROWS, J, true>(st, a, L, gi0, q, LR);
       // level 0 of row q: red element (J & 1), black element
       constexpr int er = J & 1;
       st.U[0][md<6>(J)] = st.RU[md<kNru>(J)][er];
@@ -612,6 +628,7 @@ __device__ __forceinline__ void sup_body(const LevelView<T> &a, const STile &tl,
   constexpr int H = s_up_halo(NT, EPI), P = s_period(NT), TJ = SCOLS - 2 * H;
   static_assert(N1 == 0 || EPI != 2, "the update metric needs the staged iterate");
   static_assert((NB > 0) == (EPI == 4), "pre-sweeps of the next cycle belong to epilogue 4");
+  constexpr bool FBR = EPI == 1 || EPI == 3;  // see s_prefetch
   const int R = a.rows, LR = R + 2 * H;  // LR is a multiple of P
   const int gi0 = tl.gi0, cb = tl.cb;
   const Lane L = make_lane<T>(cb, H, TJ, a.ny, a.pitch);
@@ -674,7 +691,7 @@ __device__ __forceinline__ void sup_body(const LevelView<T> &a, const STile &tl,
   const bool oc_c = EDGE ? L.oc && Jc >= 1 && Jc <= ncy - 2 : L.oc;
   static_for<-kPf, 0>([&](auto jc) {
     constexpr int J = decltype(jc)::value;
-    s_prefetch<T, NT, ZS, EDGE, true, J>(st, a, L, gi0, J, LR);
+    s_prefetch<T, NT, ZS, EDGE, true, J, FBR>(st, a, L, gi0, J, LR);
   });
 #pragma unroll 1
   for (int q0 = 0; q0 < LR; q0 += P) {
@@ -688,7 +705,7 @@ __device__ __forceinline__ void sup_body(const LevelView<T> &a, const STile &tl,
       constexpr int J = decltype(jc)::value;
       const int q = q0 + J;
       group_step();
-      s_prefetch<T, NT, ZS, EDGE, ROWS, J>(st, a, L, gi0, q, LR);
+      s_prefetch<T, NT, ZS, EDGE, ROWS, J, FBR>(st, a, L, gi0, q, LR);
       // staged row q (level -1: red element J & 1, level 0: black element)
       constexpr int er = J & 1;
       if constexpr (N1 > 0) {

@@ -13,7 +13,11 @@ template <typename T> struct LevelView {
   int nx, ny, pitch;
   T hx2, hy2, coef;
   T ihx2, ihy2;  // 1/hx2, 1/hy2: exact when pw2 (both squares are powers of two)
-  int eq, lap, pw2;
+  int eq, pw2;
+  // row stride of f: pitch, or 0 on Laplace levels, whose right-hand side
+  // array is all zeros (every row then reads row 0, which stays in L2), so
+  // the right-hand-side loads need no branch
+  int fpitch;
   int tiles_i, tiles_j, chunk;  // tile grid of this launch and per-XCD chunk
   int rows;                     // output rows per tile
 };
