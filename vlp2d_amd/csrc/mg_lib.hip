@@ -412,14 +412,18 @@ template <typename T> void coarse(Ctx *c, Hier<T> &H, bool zero_first, double *s
 void down_r64(Ctx *c, Lvl<float> &f, Lvl<float> &g, const MixView &mx);
 
 // r64 (FP32 hierarchies, zero0, from == 0): the level-0 descent takes the FP64
-// residual of r64->u64in as its right-hand side (down_r64).
+// residual of r64->u64in as its right-hand side (down_r64).  stored0 (from ==
+// 1): level 0's pre-smoothed iterate is in its u2 (written by up_down0), and
+// the level-0 ascent stages it.
 template <typename T>
 int vcycle_fused(Ctx *c, Hier<T> &H, bool zero0, int epi, const MixView &mx, double *snap, int from = 0, int to = 0,
-                 const MixView *r64 = nullptr) {
+                 const MixView *r64 = nullptr, bool stored0 = false) {
   const int lc = H.lc;
   auto zi = [&](int l) { return l > 0 || zero0; };
   auto ep = [&](int l) { return l == 0 ? epi : (int)EPI_NONE; };
-  auto rz = [&](int l) { return recompute_presmooth(c, zi(l), ep(l)) && !lds_ascent(c, H, l); };
+  auto rz = [&](int l) {
+    return !(l == 0 && stored0) && recompute_presmooth(c, zi(l), ep(l)) && !lds_ascent(c, H, l);
+  };
   for (int l = from; l < lc; ++l) {
     if constexpr (sizeof(T) == 4) {
       if (l == 0 && r64) {
@@ -577,10 +581,11 @@ void mixed_step(Ctx *c, Lvl<double> &P, int m = 1, bool pub = false, bool first 
   if (m == 2 && fused_refine && c->o.nu1 == 1 && c->o.nu2 <= 3 &&
       !(c->o.flags & (VLP2D_FLAG_STORE_PRESMOOTH | VLP2D_FLAG_NO_CYCLE_FUSION))) {
     // first cycle down to level 1, level 0 of both cycles in between in one
-    // launch, second cycle from level 1 with the refinement ascent
+    // launch (it stores the second cycle's pre-smoothed iterate in z.u2),
+    // second cycle from level 1 with the refinement ascent staging it
     vcycle_fused(c, Z, true, EPI_NONE, none, nullptr, 0, 1, r64);
     up_down0(c, z, Z.lv[1]);
-    const int np = vcycle_fused(c, Z, false, EPI_MIXED, mx, nullptr, 1, 0);
+    const int np = vcycle_fused(c, Z, false, EPI_MIXED, mx, nullptr, 1, 0, nullptr, true);
     finish(c, c->part2, np, c->part, np, pub);
     return;
   }

@@ -54,10 +54,11 @@
 //             FP64 refinement u64 += e, r32 = (float)(fn - Lap u64), partial
 //             sums of r^2 and partial max |e|; e itself is not written;
 //             H = S + 2) or 4 (the level-0 correction e of the first FP32
-//             cycle of a two-cycle mixed step: write e, then continue into
-//             the second cycle's descent on the same rows, NB pre-sweeps on
-//             e, residual and half-weighting restriction into the coarse
-//             right-hand side a.rc; H = S + 2NB + 2).
+//             cycle of a two-cycle mixed step, continued into the second
+//             cycle's descent on the same rows: NB pre-sweeps on e, residual
+//             and half-weighting restriction into the coarse right-hand side
+//             a.rc; the pre-smoothed iterate, not e, is written, and the
+//             second cycle's ascent stages it; H = S + 2NB + 2).
 // On a zero-initialised level the pre-smoothed iterate is a local function
 // of the right-hand side (radius 2 NS1 for NS1 pre-sweeps).  There the
 // descent may skip writing it (k_sdown STORE = false) and the ascent
@@ -800,7 +801,7 @@ __device__ __forceinline__ void sup_body(const LevelView<T> &a, const STile &tl,
         const T x0 = st.template full<ST, J, ST, 0>(), x1 = st.template full<ST, J, ST, 1>();
         bool orow = p >= H && p < H + R;
         if constexpr (ROWS) orow = orow && gi >= 1 && gi <= a.nx - 2;
-        if constexpr (EPI != 3) {
+        if constexpr (EPI != 3 && EPI != 4) {
           if (orow) s_store<T, EDGE>(a.uout, a.pitch, gi, L, x0, x1);
         } else {
           // refined FP64 row q - ST: u64 + e on interior nodes
@@ -839,6 +840,16 @@ __device__ __forceinline__ void sup_body(const LevelView<T> &a, const STile &tl,
         // residual of row q - SE - 1 and the restriction at even row
         // q - SE - 2 (as k_sdown, MultiGridScheme.cs:134-169)
         s_sweeps<T, NT, EQ, PW2, EDGE, ROWS, J, ST + 1, SE>(st, a, L, gi0, q);
+        {
+          // the next cycle's pre-smoothed iterate of row q - SE, which its
+          // ascent stages (instead of recomputing it from this cycle's e)
+          const int p = q - SE, gi = gi0 + p;
+          bool orow = p >= H && p < H + R;
+          if constexpr (ROWS) orow = orow && gi >= 1 && gi <= a.nx - 2;
+          if (orow)
+            s_store<T, EDGE>(a.uout, a.pitch, gi, L, st.template full<SE, J, SE, 0>(),
+                             st.template full<SE, J, SE, 1>());
+        }
         {
           constexpr int d = SE + 1;
           constexpr int pr = J - d;
