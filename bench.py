@@ -132,6 +132,50 @@ def host_cpus():
     return {"nproc": nproc, "affinity": aff, "cgroup": quota, "model": model, "usable": usable}
 
 
+def stream_ceiling(n):
+    """Bandwidth the GPU's HBM delivers to plain streaming kernels (torch's
+    elementwise copy and add, and a sum) over FP64 arrays of n elements each:
+    the practical ceiling for a kernel with the same read:write mix, below the
+    published 8 TB/s.  Measured in a child process (torch's HIP runtime and
+    the solver library's are separate libraries, kept out of one process)."""
+    import subprocess
+    out = subprocess.run([sys.executable, os.path.abspath(__file__), "--ceiling-only", str(n)], check=True,
+                         capture_output=True, text=True).stdout
+    return json.loads(out.strip().splitlines()[-1])
+
+
+def _stream_ceiling_child(n, reps=20):
+    """Best of reps launches of each streaming kernel, timed with events on
+    torch's stream (where these kernels run)."""
+    import torch
+    dev = torch.device("cuda", 0)
+    a = torch.rand(n, dtype=torch.float64, device=dev)
+    b = torch.rand(n, dtype=torch.float64, device=dev)
+    c = torch.empty_like(a)
+    nb = 8.0 * n
+    cases = {"copy_1r1w": (lambda: c.copy_(a), 2 * nb), "add_2r1w": (lambda: torch.add(a, b, out=c), 3 * nb),
+             "sum_1r": (lambda: a.sum(), nb)}
+    out = {}
+    for key, (fn, nbytes) in cases.items():
+        for _ in range(3):
+            fn()
+        best = None
+        for _ in range(reps):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            fn()
+            e1.record()
+            e1.synchronize()
+            ms = e0.elapsed_time(e1)
+            best = ms if best is None else min(best, ms)
+        out[key] = round(nbytes / (best * 1e-3) / 1e9, 1)
+    del a, b, c
+    torch.cuda.empty_cache()
+    out["unit"] = "GB/s"
+    out["arrays"] = f"{n} FP64 elements each ({8 * n / 1e6:.0f} MB)"
+    return out
+
+
 def dd128_check(nu, seg=512, rtol=1e-10, inner=1):
     """Errors of the GPU mixed-precision and FP64 solutions (to rtol) of the
     513^2 replica against the oracle's double-double multigrid converged to
@@ -178,7 +222,12 @@ def main():
     ap.add_argument("--only", choices=("mixed", "fp64"),
                     help="time one precision only (the runs rocprofv3 --pmc profiles)")
     ap.add_argument("--no-v22", action="store_true", help="skip the V(2,2) sub-record")
+    ap.add_argument("--no-ceiling", action="store_true", help="skip the plain-stream bandwidth ceiling")
+    ap.add_argument("--ceiling-only", type=int, help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.ceiling_only:
+        print(json.dumps(_stream_ceiling_child(args.ceiling_only)), flush=True)
+        return
     nu = tuple(int(x) for x in args.cycle.split(","))
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -230,10 +279,22 @@ def main():
         achieved = per_launch_bytes / avg_s / 1e9
         key = f"{name}@level{lvl}"
         prec = "f32" if name in FP32_CLASSES_IN_MIXED else "f64"
+        traffic = measured_traffic(f"{key}/{prec}")
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(f"{key}/{prec}"),
+                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                     "kernel": f"{key}/{prec}", "avg_launch_us": round(avg_s * 1e6, 3),
                     "alg_bytes_per_launch": per_launch_bytes, "share_of_solve": round(ms / tot, 3)}
+        if traffic:
+            roofline["traffic_over_alg"] = round(traffic / per_launch_bytes, 4)
+            roofline["measured_GBps"] = round(traffic / avg_s / 1e9, 1)
+        if not args.no_ceiling:
+            # the dominant kernel reads about twice what it writes: compare
+            # with a plain stream of the same mix on this GPU
+            ceil = stream_ceiling(p.shape[0] * p.shape[1])
+            roofline["stream_ceiling"] = ceil
+            roofline["frac_of_2r1w_ceiling"] = round(achieved / ceil["add_2r1w"], 4)
+            if traffic:
+                roofline["measured_frac_of_2r1w_ceiling"] = round(traffic / avg_s / 1e9 / ceil["add_2r1w"], 4)
         kernels = sorted(([e[0], e[1], e[2], round(e[3], 4), e[4]] for e in kt), key=lambda e: -e[3])[:12]
     # whole solve: the HBM bytes one solve of each precision moves (rocprofv3
     # FETCH_SIZE + WRITE_SIZE over all its dispatches, committed summary of a

@@ -682,7 +682,13 @@ __device__ __forceinline__ void sup_body(const LevelView<T> &a, const STile &tl,
   double X64[N3][2], XR[N3][2], FN[N3][2];
 #pragma unroll
   for (int k = 0; k < N3; ++k) X64[k][0] = X64[k][1] = XR[k][0] = XR[k][1] = FN[k][0] = FN[k][1] = 0.0;
-  const unsigned boff8 = L.ld ? (unsigned)L.gj * 8u : 0u;
+  // EPI 3's FP64 rows: only lanes H/2 - 1 .. H/2 + TJ/2 (the output columns
+  // and the neighbour column on each side, which the FP64 residual reads)
+  // need their values; the others load a needed lane's pair instead of their
+  // own halo columns (addresses already requested: no extra traffic)
+  const int m64 = L.m < H / 2 - 1 ? H / 2 - 1 : (L.m > H / 2 + TJ / 2 ? H / 2 + TJ / 2 : L.m);
+  const int gj64 = cb + 2 * m64;
+  const unsigned boff8 = gj64 >= 0 && gj64 <= a.pitch - 2 ? (unsigned)gj64 * 8u : 0u;
   // EPI 4: residual of rows (slot p % 3; element 1 only on even rows) and
   // the coarse output column of element 0 (fine column 2 Jc)
   constexpr int N4 = EPI == 4 ? 3 : 1;
