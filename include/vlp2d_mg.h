@@ -50,13 +50,14 @@ extern "C" {
 #define VLP2D_SMOOTH_RBGS 0    /* red-black Gauss-Seidel, red first (the reference's) */
 #define VLP2D_SMOOTH_WJACOBI 1 /* weighted Jacobi u += omega (g - u), g the same
                                   point formula; no counterpart in the reference;
-                                  runs one operator per launch                  */
+                                  fused streaming kernels like Gauss-Seidel's
+                                  (VLP2D_FLAG_SIMPLE: one operator per launch) */
 
 #define VLP2D_RESTRICT_HALF 0 /* half weighting 1/8 [0 1 0; 1 4 1; 0 1 0] (the
                                  reference's, MultiGridScheme.cs:145-169)      */
 #define VLP2D_RESTRICT_FULL 1 /* full weighting 1/16 [1 2 1; 2 4 2; 1 2 1]; no
-                                 counterpart in the reference; runs one
-                                 operator per launch                           */
+                                 counterpart in the reference; in the fused
+                                 descent kernels' restriction epilogue       */
 
 #define VLP2D_CYCLE_V 0   /* V-cycle per iteration (the reference's doIteration,
                              MultiGridScheme.cs:65)                          */

@@ -200,21 +200,28 @@ def test_speculative_step_dropped_when_already_solved():
     s.cleanup()
 
 
+# the fused streaming kernels and one operator per launch
+SMOOTHER_PATHS = {"fused": 0, "one_op_per_launch": _lib.FLAG_SIMPLE}
+
+
+@pytest.mark.parametrize("path", list(SMOOTHER_PATHS))
 @pytest.mark.parametrize("omega", [0.8, 2 / 3])
 @pytest.mark.parametrize("seg_x,seg_y,hx,hy,laplace", [(64, 32, 1 / 64, 1 / 64, False),
                                                        (128, 256, np.e / 128, np.pi / 256, False),
                                                        (128, 128, 1 / 128, 1 / 128, True)])
-def test_weighted_jacobi_bitwise(seg_x, seg_y, hx, hy, laplace, omega):
+def test_weighted_jacobi_bitwise(seg_x, seg_y, hx, hy, laplace, omega, path):
     """Weighted-Jacobi smoother (an addition; the reference smooths with
     red-black Gauss-Seidel only): parity-mode cycles with the update norm,
     and FP64 and mixed V(2,2) solves to 1e-10, bit for bit the oracle's."""
-    _say(f"jacobi {seg_x}x{seg_y} omega={omega:.3f}")
+    _say(f"jacobi {seg_x}x{seg_y} omega={omega:.3f} {path}")
+    fl = SMOOTHER_PATHS[path]
     f, g = _random(seg_x, seg_y, 11 + seg_x, laplace)
     u0 = _mean_init(g)
     o = Oracle(seg_x, seg_y, hx, hy, f)
     o.set_smoother(1, omega)
     o.set_u(u0)
-    s = MultiGridSchemeHIP(seg_x, seg_y, hx, hy, f, eps=1e-13, options=Options(smoother=_lib.SMOOTH_WJACOBI, omega=omega))
+    s = MultiGridSchemeHIP(seg_x, seg_y, hx, hy, f, eps=1e-13,
+                           options=Options(smoother=_lib.SMOOTH_WJACOBI, omega=omega, flags=fl))
     s.un0[:] = u0
     s.init_after_boundaries_and_initial_iteration_inited()
     for k in range(4):
@@ -229,7 +236,7 @@ def test_weighted_jacobi_bitwise(seg_x, seg_y, hx, hy, laplace, omega):
         # (point Jacobi smooths the anisotropic grid, (hx/hy)^2 = 3, slowly: 109 cycles at omega = 2/3)
         ur, kr, _ = solve(seg_x, seg_y, hx, hy, z, g, 2, 2, 3, 1e-10, 150, smoother=1, omega=omega)
         s = MultiGridSchemeHIP(seg_x, seg_y, hx, hy, z,
-                               options=Options.performance(mixed, smoother=_lib.SMOOTH_WJACOBI, omega=omega))
+                               options=Options.performance(mixed, smoother=_lib.SMOOTH_WJACOBI, omega=omega, flags=fl))
         s.un0[:] = g
         s.init_after_boundaries_and_initial_iteration_inited()
         st = s.solve(1e-10, 150)
@@ -238,15 +245,17 @@ def test_weighted_jacobi_bitwise(seg_x, seg_y, hx, hy, laplace, omega):
         s.cleanup()
 
 
+@pytest.mark.parametrize("path", list(SMOOTHER_PATHS))
 @pytest.mark.parametrize("smoother", [_lib.SMOOTH_RBGS, _lib.SMOOTH_WJACOBI])
 @pytest.mark.parametrize("seg_x,seg_y,hx,hy,laplace", [(64, 32, 1 / 64, 1 / 64, False),
                                                        (256, 128, np.e / 256, np.pi / 128, False),
                                                        (128, 128, 1 / 128, 1 / 128, True)])
-def test_full_weighting_bitwise(seg_x, seg_y, hx, hy, laplace, smoother):
+def test_full_weighting_bitwise(seg_x, seg_y, hx, hy, laplace, smoother, path):
     """Full-weighting restriction (an addition; the reference restricts by
     half weighting): parity-mode cycles with the update norm, and FP64 and
     mixed V(2,2) solves to 1e-10, bit for bit the oracle's."""
-    _say(f"full weighting {seg_x}x{seg_y} smoother={smoother}")
+    _say(f"full weighting {seg_x}x{seg_y} smoother={smoother} {path}")
+    fl = SMOOTHER_PATHS[path]
     f, g = _random(seg_x, seg_y, 23 + seg_x, laplace)
     u0 = _mean_init(g)
     o = Oracle(seg_x, seg_y, hx, hy, f)
@@ -254,7 +263,7 @@ def test_full_weighting_bitwise(seg_x, seg_y, hx, hy, laplace, smoother):
     o.set_smoother(smoother)
     o.set_u(u0)
     s = MultiGridSchemeHIP(seg_x, seg_y, hx, hy, f, eps=1e-13,
-                           options=Options(restriction=_lib.RESTRICT_FULL, smoother=smoother))
+                           options=Options(restriction=_lib.RESTRICT_FULL, smoother=smoother, flags=fl))
     s.un0[:] = u0
     s.init_after_boundaries_and_initial_iteration_inited()
     for k in range(4):
@@ -268,12 +277,42 @@ def test_full_weighting_bitwise(seg_x, seg_y, hx, hy, laplace, smoother):
         solve = mixed_solve if mixed else plain_solve
         ur, kr, _ = solve(seg_x, seg_y, hx, hy, z, g, 2, 2, 3, 1e-10, 150, smoother=smoother, restriction=1)
         s = MultiGridSchemeHIP(seg_x, seg_y, hx, hy, z,
-                               options=Options.performance(mixed, smoother=smoother, restriction=_lib.RESTRICT_FULL))
+                               options=Options.performance(mixed, smoother=smoother, restriction=_lib.RESTRICT_FULL,
+                                                           flags=fl))
         s.un0[:] = g
         s.init_after_boundaries_and_initial_iteration_inited()
         st = s.solve(1e-10, 150)
         assert st["cycles"] == kr, (mixed, st["cycles"], kr)
         assert np.array_equal(s.get_array(), ur), (mixed, np.abs(s.get_array() - ur).max())
+        s.cleanup()
+
+
+@pytest.mark.parametrize("smoother,restriction", [(_lib.SMOOTH_WJACOBI, _lib.RESTRICT_HALF),
+                                                  (_lib.SMOOTH_RBGS, _lib.RESTRICT_FULL),
+                                                  (_lib.SMOOTH_WJACOBI, _lib.RESTRICT_FULL)])
+@pytest.mark.parametrize("nu", [(1, 3), (2, 2), (3, 1)])
+def test_fused_smoother_restriction_large(smoother, restriction, nu):
+    """Weighted Jacobi and full weighting in the fused streaming kernels on a
+    grid of many tiles per level (2048 x 1024 segments, equal steps):
+    FP64 and mixed (one and two FP32 cycles per refinement step) solves to
+    1e-10, cycle counts and iterates bit for bit the oracle's."""
+    seg_x, seg_y = 2048, 1024
+    hx = hy = 1 / 2048
+    _say(f"fused {seg_x}x{seg_y} smoother={smoother} restriction={restriction} V{nu}")
+    f, g = _random(seg_x, seg_y, 5 + 3 * smoother + restriction, False)
+    for mixed, inner in ((False, 1), (True, 1), (True, 2)):
+        solve = mixed_solve if mixed else plain_solve
+        kw = {"inner": inner} if mixed else {}
+        ur, kr, _ = solve(seg_x, seg_y, hx, hy, f, g, nu[0], nu[1], 3, 1e-10, 200, smoother=smoother,
+                          restriction=restriction, **kw)
+        s = MultiGridSchemeHIP(seg_x, seg_y, hx, hy, f,
+                               options=Options.performance(mixed, nu1=nu[0], nu2=nu[1], smoother=smoother,
+                                                           restriction=restriction, max_inner=inner))
+        s.un0[:] = g
+        s.init_after_boundaries_and_initial_iteration_inited()
+        st = s.solve(1e-10, 200)
+        assert st["cycles"] == kr, (mixed, inner, st["cycles"], kr)
+        assert np.array_equal(s.get_array(), ur), (mixed, inner, np.abs(s.get_array() - ur).max())
         s.cleanup()
 
 

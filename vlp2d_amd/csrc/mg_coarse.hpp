@@ -23,6 +23,7 @@ template <typename T> struct CoarseLevel {
   T hx2, hy2, coef;
   T ihx2, ihy2;
   int eq, lap, pw2;
+  T *u2;  // second iterate buffer (k_coarse's weighted-Jacobi sweeps), or null
 };
 template <typename T> struct CoarseArgs {
   CoarseLevel<T> lv[kMaxLevels];
@@ -30,7 +31,32 @@ template <typename T> struct CoarseArgs {
   int nu1, nu2, nuc;
   int zero_first;
   double *snap;  // level-0 snapshot before post-smoothing (l0 == 0), or null
+  int jac;       // weighted-Jacobi smoother (weight w) instead of red-black Gauss-Seidel
+  T w;
+  int fw;  // full-weighting restriction instead of half weighting
 };
+
+// Gauss-Seidel point formula of node c (MultiGridSlidingIteration.cs:53-57),
+// row pitch P; the weighted-Jacobi update uses the same g.
+template <typename T> __device__ inline T c_point(const T *u, const T *f, int c, int P, const CoarseLevel<T> &g) {
+  if (g.eq) {
+    if (g.lap) return (u[c - P] + u[c + P] + u[c - 1] + u[c + 1]) * T(0.25);
+    return (u[c - P] + u[c + P] + u[c - 1] + u[c + 1] - f[c] * g.hx2) * T(0.25);
+  }
+  const T sx = divh(u[c - P] + u[c + P], g.hx2, g.ihx2, g.pw2);
+  const T sy = divh(u[c - 1] + u[c + 1], g.hy2, g.ihy2, g.pw2);
+  if (g.lap) return g.coef * (sx + sy);
+  return g.coef * (sx + sy - f[c]);
+}
+
+// Restriction of the residual r(c) around fine node k (row pitch P):
+// half weighting (MultiGridScheme.cs:158-167) or full weighting (the oracle's
+// restrict_fw), terms summed in the oracle's order.
+template <typename T, typename R> __device__ inline T c_restrict(R r, int k, int P, int fw) {
+  const T r0 = r(k), rip = r(k + P), rim = r(k - P), rjp = r(k + 1), rjm = r(k - 1);
+  if (!fw) return T(0.125) * (r0 * T(4) + (rip + rim + rjp + rjm));
+  return T(0.0625) * (r0 * T(4) + (rip + rim + rjp + rjm) * T(2) + (r(k + P + 1) + r(k + P - 1) + r(k - P + 1) + r(k - P - 1)));
+}
 
 template <typename T>
 __device__ inline void cg_zero(const CoarseLevel<T> &v) {
@@ -43,7 +69,26 @@ __device__ inline void cg_zero(const CoarseLevel<T> &v) {
 }
 
 template <typename T>
-__device__ inline void cg_smooth(const CoarseLevel<T> &v, int n) {
+__device__ inline void cg_smooth(const CoarseLevel<T> &v, int n, const CoarseArgs<T> &a) {
+  if (a.jac) {
+    // weighted Jacobi: new values into u2, then back into u
+    const int ni = (v.nx - 2) * (v.ny - 2);
+    for (int s = 0; s < n; ++s) {
+      for (int p = threadIdx.x; p < ni; p += CNT) {
+        const int i = 1 + p / (v.ny - 2), j = 1 + p - (p / (v.ny - 2)) * (v.ny - 2);
+        const int c = i * v.pitch + j;
+        const T o = v.u[c];
+        v.u2[c] = o + a.w * (c_point(v.u, v.f, c, v.pitch, v) - o);
+      }
+      __syncthreads();
+      for (int p = threadIdx.x; p < ni; p += CNT) {
+        const int i = 1 + p / (v.ny - 2), j = 1 + p - (p / (v.ny - 2)) * (v.ny - 2);
+        v.u[i * v.pitch + j] = v.u2[i * v.pitch + j];
+      }
+      __syncthreads();
+    }
+    return;
+  }
   const int wc = (v.ny - 1) / 2;  // items per row and colour (ny odd)
   const int items = (v.nx - 2) * wc;
   for (int s = 0; s < n; ++s)
@@ -84,21 +129,19 @@ __global__ void __launch_bounds__(CNT) k_coarse(CoarseArgs<T> a) {
     const CoarseLevel<T> &f = a.lv[l];
     const CoarseLevel<T> &g = a.lv[l + 1];
     if (l > a.l0 || a.zero_first) cg_zero(f);
-    cg_smooth(f, a.nu1);
+    cg_smooth(f, a.nu1, a);
     const int n = (g.nx - 2) * (g.ny - 2);
     for (int p = threadIdx.x; p < n; p += CNT) {
       const int I = 1 + p / (g.ny - 2), J = 1 + p - (p / (g.ny - 2)) * (g.ny - 2);
       const int c = 2 * I * f.pitch + 2 * J;
-      const T r0 = cg_resid(f, c), rip = cg_resid(f, c + f.pitch), rim = cg_resid(f, c - f.pitch);
-      const T rjp = cg_resid(f, c + 1), rjm = cg_resid(f, c - 1);
-      ((T *)g.f)[I * g.pitch + J] = T(0.125) * (r0 * T(4) + (rip + rim + rjp + rjm));
+      ((T *)g.f)[I * g.pitch + J] = c_restrict<T>([&](int k) { return cg_resid(f, k); }, c, f.pitch, a.fw);
     }
     __syncthreads();
   }
   {
     const CoarseLevel<T> &v = a.lv[a.L - 1];
     if (a.L - 1 > a.l0 || a.zero_first) cg_zero(v);
-    cg_smooth(v, a.nuc);
+    cg_smooth(v, a.nuc, a);
   }
   for (int l = a.L - 2; l >= a.l0; --l) {
     const CoarseLevel<T> &f = a.lv[l];
@@ -112,7 +155,7 @@ __global__ void __launch_bounds__(CNT) k_coarse(CoarseArgs<T> a) {
       if (l == 0 && a.snap) a.snap[c] = (double)v;
     }
     __syncthreads();
-    cg_smooth(f, a.nu2);
+    cg_smooth(f, a.nu2, a);
   }
 }
 
@@ -133,10 +176,12 @@ template <typename T> struct LdsLevel {
   int ly;  // ny = 2^ly + 1 (every level of a power-of-two grid)
 };
 
-// LDS bytes k_coarse_lds needs for levels [l0, L) of the given sizes.
-inline size_t coarse_lds_bytes(const int *nx, const int *ny, int l0, int L, size_t elem) {
+// LDS bytes k_coarse_lds needs for levels [l0, L) of the given sizes (jac:
+// plus one level-l0 array for the new values of a Jacobi sweep).
+inline size_t coarse_lds_bytes(const int *nx, const int *ny, int l0, int L, size_t elem, bool jac = false) {
   size_t n = 0;
   for (int l = l0; l < L; ++l) n += 2 * (size_t)nx[l] * (size_t)ny[l];
+  if (jac) n += (size_t)nx[l0] * (size_t)ny[l0];
   return n * elem;
 }
 
@@ -154,7 +199,29 @@ template <typename T> __device__ inline LdsLevel<T> cl_level(const CoarseArgs<T>
 // Work items are mapped to nodes with shifts and masks only (row lengths are
 // powers of two plus one): one item per node of one colour in a half-sweep,
 // one per interior node (column 0 of each row idle) elsewhere.
-template <typename T> __device__ inline void cl_smooth(const LdsLevel<T> &v, const CoarseLevel<T> &g, int n) {
+template <typename T>
+__device__ inline void cl_smooth(const LdsLevel<T> &v, const CoarseLevel<T> &g, int n, const CoarseArgs<T> &a, T *tmp) {
+  if (a.jac) {
+    // weighted Jacobi: new values into tmp (LDS after the levels), then back
+    const int items = (v.nx - 2) << v.ly;
+    for (int s = 0; s < n; ++s) {
+      for (int p = threadIdx.x; p < items; p += CNT) {
+        const int i = 1 + (p >> v.ly), j = p & ((1 << v.ly) - 1);
+        if (j == 0) continue;
+        const int c = i * v.ny + j;
+        const T o = v.u[c];
+        tmp[c] = o + a.w * (c_point(v.u, v.f, c, v.ny, g) - o);
+      }
+      __syncthreads();
+      for (int p = threadIdx.x; p < items; p += CNT) {
+        const int i = 1 + (p >> v.ly), j = p & ((1 << v.ly) - 1);
+        if (j == 0) continue;
+        v.u[i * v.ny + j] = tmp[i * v.ny + j];
+      }
+      __syncthreads();
+    }
+    return;
+  }
   const int lw = v.ly - 1;  // 2^lw items per row and colour
   const int items = (v.nx - 2) << lw;
   const int P = v.ny;
@@ -194,6 +261,8 @@ template <typename T>
 __global__ void __launch_bounds__(CNT) k_coarse_lds(CoarseArgs<T> a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T *sm = reinterpret_cast<T *>(smem_raw);
+  // Jacobi's new values: after the last level
+  T *const tmp = sm + (cl_level(a, sm, a.L - 1).f - sm) + a.lv[a.L - 1].nx * a.lv[a.L - 1].ny;
   {
     // level l0: the whole iterate (its boundary feeds the stencils; interior
     // zeroed when zero_first) and the right-hand side (0 on Laplace levels)
@@ -212,21 +281,19 @@ __global__ void __launch_bounds__(CNT) k_coarse_lds(CoarseArgs<T> a) {
   for (int l = a.l0; l <= a.L - 2; ++l) {
     const LdsLevel<T> f = cl_level(a, sm, l), g = cl_level(a, sm, l + 1);
     const CoarseLevel<T> &fg = a.lv[l];
-    cl_smooth(f, fg, a.nu1);
+    cl_smooth(f, fg, a.nu1, a, tmp);
     for (int p = threadIdx.x; p < (g.nx - 2) << g.ly; p += CNT) {
       const int I = 1 + (p >> g.ly), J = p & ((1 << g.ly) - 1);
       if (J == 0) continue;
       const int c = 2 * I * f.ny + 2 * J;
-      const T r0 = cl_resid(f, fg, c), rip = cl_resid(f, fg, c + f.ny), rim = cl_resid(f, fg, c - f.ny);
-      const T rjp = cl_resid(f, fg, c + 1), rjm = cl_resid(f, fg, c - 1);
-      g.f[I * g.ny + J] = T(0.125) * (r0 * T(4) + (rip + rim + rjp + rjm));
+      g.f[I * g.ny + J] = c_restrict<T>([&](int k) { return cl_resid(f, fg, k); }, c, f.ny, a.fw);
     }
     // the coarse iterate, boundary included, starts at 0 (the coarse
     // right-hand side's boundary is never read)
     for (int p = threadIdx.x; p < g.nx * g.ny; p += CNT) g.u[p] = T(0);
     __syncthreads();
   }
-  cl_smooth(cl_level(a, sm, a.L - 1), a.lv[a.L - 1], a.nuc);
+  cl_smooth(cl_level(a, sm, a.L - 1), a.lv[a.L - 1], a.nuc, a, tmp);
   for (int l = a.L - 2; l >= a.l0; --l) {
     const LdsLevel<T> f = cl_level(a, sm, l), g = cl_level(a, sm, l + 1);
     for (int p = threadIdx.x; p < (f.nx - 2) << f.ly; p += CNT) {
@@ -238,7 +305,7 @@ __global__ void __launch_bounds__(CNT) k_coarse_lds(CoarseArgs<T> a) {
       if (l == 0 && a.snap) a.snap[i * a.lv[0].pitch + j] = (double)v;
     }
     __syncthreads();
-    cl_smooth(f, a.lv[l], a.nu2);
+    cl_smooth(f, a.lv[l], a.nu2, a, tmp);
   }
   {
     // level l0's interior iterate back to global memory; the coarser levels

@@ -155,6 +155,7 @@ template <typename T> LevelView<T> view(const Lvl<T> &v, const T *uin, T *uout, 
   a.ihy2 = v.ihy2;
   a.pw2 = v.pw2 ? 1 : 0;
   a.rc = nullptr;
+  a.w = T(0);
   a.tiles_i = a.tiles_j = a.chunk = a.rows = 0;
   return a;
 }
@@ -179,6 +180,7 @@ inline int resident_waves(Ctx *c, const void *fn) {
 template <typename T>
 LevelView<T> sview(Ctx *c, const void *fn, const Lvl<T> &v, const T *uin, T *uout, int H, int P) {
   LevelView<T> a = view(v, uin, uout, H);
+  a.w = (T)c->o.omega;
   const int tj = (v.ny - 2) / (SCOLS - 2 * H) + 1;
   const int ti = std::max(1, resident_waves(c, fn) / tj);
   const int R = s_rows_at_least((v.nx - 2) / ti + 1, H, P);

@@ -1,21 +1,8 @@
 // Descent launchers (k_sdown instances) for float levels, 3 pre-sweeps.
 // One translation unit per precision and sweep count, compiled in parallel.
-#include "mg_ctx.hpp"
+#include "mg_launch.hpp"
 
 namespace vlp2d {
-
-template <typename T, int NS>
-void launch_down_ns(Ctx *c, Lvl<T> &f, Lvl<T> &g, bool zi, bool store) {
-  constexpr int H = s_down_halo(NS), P = s_period(NS);
-  using K = void (*)(LevelView<T>, T *, int, int, int);
-  K k;
-  if (!zi && store) k = f.eq ? (K)k_sdown<T, NS, false, true> : (K)k_sdown<T, NS, false, false>;
-  else if (!zi) k = f.eq ? (K)k_sdown<T, NS, false, true, false> : (K)k_sdown<T, NS, false, false, false>;
-  else if (store) k = f.eq ? (K)k_sdown<T, NS, true, true> : (K)k_sdown<T, NS, true, false>;
-  else k = f.eq ? (K)k_sdown<T, NS, true, true, false> : (K)k_sdown<T, NS, true, false, false>;
-  const LevelView<T> a = sview(c, (const void *)k, f, f.u, f.u2, H, P);
-  hipLaunchKernelGGL(k, dim3(8 * a.chunk), dim3(64 * kWaves), 0, c->st, a, g.rhs, g.nx, g.ny, g.pitch);
-}
 
 template void launch_down_ns<float, 3>(Ctx *, Lvl<float> &, Lvl<float> &, bool, bool);
 

@@ -1,0 +1,92 @@
+// Launchers of the streaming kernels (k_sdown, k_sup), instantiated per
+// precision and sweep count in mg_k_down_*.hip / mg_k_up_*.hip, which are
+// compiled in parallel.  The smoother and the restriction of the handle's
+// options select the kernel instance.
+#pragma once
+#include "mg_ctx.hpp"
+
+namespace vlp2d {
+
+// Descent of level f into g's right-hand side (zi: f starts from zero;
+// store: the pre-smoothed iterate is written to f.u2).
+template <typename T, int NS, int SM, bool FW>
+void launch_down_k(Ctx *c, Lvl<T> &f, Lvl<T> &g, bool zi, bool store) {
+  constexpr int H = s_down_halo(NS, SM), P = s_period(NS, SM);
+  using K = void (*)(LevelView<T>, T *, int, int, int);
+  K k;
+  if constexpr (SM == SM_JAC) {
+    // a Jacobi-smoothed level is always stored (the host never asks otherwise)
+    (void)store;
+    if (!zi) k = f.eq ? (K)k_sdown<T, NS, false, true, true, SM, FW> : (K)k_sdown<T, NS, false, false, true, SM, FW>;
+    else k = f.eq ? (K)k_sdown<T, NS, true, true, true, SM, FW> : (K)k_sdown<T, NS, true, false, true, SM, FW>;
+  } else {
+    if (!zi && store)
+      k = f.eq ? (K)k_sdown<T, NS, false, true, true, SM, FW> : (K)k_sdown<T, NS, false, false, true, SM, FW>;
+    else if (!zi)
+      k = f.eq ? (K)k_sdown<T, NS, false, true, false, SM, FW> : (K)k_sdown<T, NS, false, false, false, SM, FW>;
+    else if (store)
+      k = f.eq ? (K)k_sdown<T, NS, true, true, true, SM, FW> : (K)k_sdown<T, NS, true, false, true, SM, FW>;
+    else k = f.eq ? (K)k_sdown<T, NS, true, true, false, SM, FW> : (K)k_sdown<T, NS, true, false, false, SM, FW>;
+  }
+  const LevelView<T> a = sview(c, (const void *)k, f, f.u, f.u2, H, P);
+  hipLaunchKernelGGL(k, dim3(8 * a.chunk), dim3(64 * kWaves), 0, c->st, a, g.rhs, g.nx, g.ny, g.pitch);
+}
+
+template <typename T, int NS>
+void launch_down_ns(Ctx *c, Lvl<T> &f, Lvl<T> &g, bool zi, bool store) {
+  const bool fw = c->o.restriction == VLP2D_RESTRICT_FULL;
+  if (c->o.smoother == VLP2D_SMOOTH_WJACOBI) {
+    if (fw) launch_down_k<T, NS, SM_JAC, true>(c, f, g, zi, store);
+    else launch_down_k<T, NS, SM_JAC, false>(c, f, g, zi, store);
+  } else {
+    if (fw) launch_down_k<T, NS, SM_GS, true>(c, f, g, zi, store);
+    else launch_down_k<T, NS, SM_GS, false>(c, f, g, zi, store);
+  }
+}
+
+template <typename T, int NS, int EPI, int N1, bool ZS = true, int SM = SM_GS>
+int launch_sup(Ctx *c, Lvl<T> &f, const Lvl<T> &g, const MixView &mx) {
+  constexpr int H = s_up_halo(N1 + NS, EPI, SM), P = s_period(N1 + NS, SM);
+  using K = void (*)(LevelView<T>, const T *, int, int, int, double *, MixView);
+  const K k = f.eq ? (K)k_sup<T, NS, EPI, true, N1, ZS, 0, SM> : (K)k_sup<T, NS, EPI, false, N1, ZS, 0, SM>;
+  // a recomputed non-zero start is staged from the descent's input f.u
+  const LevelView<T> a = sview(c, (const void *)k, f, ZS ? f.u2 : f.u, f.u, H, P);
+  hipLaunchKernelGGL(k, dim3(8 * a.chunk), dim3(64 * kWaves), 0, c->st, a, g.u, g.nx, g.ny, g.pitch, c->part, mx);
+  return a.tiles_i * a.tiles_j;
+}
+
+// FP64 levels run epilogues none / residual norm / update norm; FP32 levels
+// (the correction cycle of a mixed step) none / fused refinement.  rz: the
+// level's pre-smoothed iterate is recomputed (one Gauss-Seidel pre-sweep; a
+// Jacobi-smoothed level is always stored, rz false).
+template <typename T, int NS>
+int launch_up_ns(Ctx *c, Lvl<T> &f, const Lvl<T> &g, int epi, bool rz, bool zi, const MixView &mx) {
+  if (c->o.smoother == VLP2D_SMOOTH_WJACOBI) {
+    if constexpr (sizeof(T) == 8) {
+      switch (epi) {
+        case EPI_RESNORM: return launch_sup<T, NS, EPI_RESNORM, 0, true, SM_JAC>(c, f, g, mx);
+        case EPI_METRIC: return launch_sup<T, NS, EPI_METRIC, 0, true, SM_JAC>(c, f, g, mx);
+        default: return launch_sup<T, NS, EPI_NONE, 0, true, SM_JAC>(c, f, g, mx);
+      }
+    } else {
+      if (epi == EPI_MIXED) return launch_sup<T, NS, EPI_MIXED, 0, true, SM_JAC>(c, f, g, mx);
+      return launch_sup<T, NS, EPI_NONE, 0, true, SM_JAC>(c, f, g, mx);
+    }
+  }
+  if constexpr (sizeof(T) == 8) {
+    switch (epi) {
+      case EPI_RESNORM:
+        return rz ? launch_sup<T, NS, EPI_RESNORM, 1>(c, f, g, mx) : launch_sup<T, NS, EPI_RESNORM, 0>(c, f, g, mx);
+      case EPI_METRIC: return launch_sup<T, NS, EPI_METRIC, 0>(c, f, g, mx);
+      default: return rz ? launch_sup<T, NS, EPI_NONE, 1>(c, f, g, mx) : launch_sup<T, NS, EPI_NONE, 0>(c, f, g, mx);
+    }
+  } else {
+    if (epi == EPI_MIXED) {
+      if (rz && !zi) return launch_sup<T, NS, EPI_MIXED, 1, false>(c, f, g, mx);
+      return rz ? launch_sup<T, NS, EPI_MIXED, 1>(c, f, g, mx) : launch_sup<T, NS, EPI_MIXED, 0>(c, f, g, mx);
+    }
+    return rz ? launch_sup<T, NS, EPI_NONE, 1>(c, f, g, mx) : launch_sup<T, NS, EPI_NONE, 0>(c, f, g, mx);
+  }
+}
+
+}  // namespace vlp2d

@@ -294,7 +294,8 @@ bool is_pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
 bool recompute_presmooth(const Ctx *c, bool zi, int epi) {
   // a non-zero start is recomputed only by the refinement ascent, which
   // writes no iterate and so may stage the descent's input in place
-  return (zi || epi == EPI_MIXED) && c->o.nu1 == 1 && epi != EPI_METRIC &&
+  // (Gauss-Seidel only: a Jacobi-smoothed level is stored)
+  return (zi || epi == EPI_MIXED) && c->o.nu1 == 1 && epi != EPI_METRIC && c->o.smoother == VLP2D_SMOOTH_RBGS &&
          !(c->o.flags & VLP2D_FLAG_STORE_PRESMOOTH);
 }
 
@@ -309,7 +310,7 @@ constexpr long long kLtNodes = VLP2D_LT_NODES;
 template <typename T> bool lds_ascent(const Ctx *c, const Hier<T> &H, int l) {
   const Lvl<T> &v = H.lv[l];
   return l >= 1 && l < H.lc && (long long)v.nx * v.ny <= kLtNodes && c->o.nu2 >= 1 && c->o.nu2 <= 3 &&
-         !(c->o.flags & VLP2D_FLAG_STREAM_SMALL);
+         c->o.smoother == VLP2D_SMOOTH_RBGS && !(c->o.flags & VLP2D_FLAG_STREAM_SMALL);
 }
 
 
@@ -372,8 +373,8 @@ template <typename T> void coarse(Ctx *c, Hier<T> &H, bool zero_first, double *s
   double bytes = 0;
   for (int l = H.lc; l < L; ++l) {
     const Lvl<T> &v = H.lv[l];
-    a.lv[l] = CoarseLevel<T>{v.u,    v.rhs,  v.nx,           v.ny,           v.pitch,       v.hx2, v.hy2,
-                             v.coef, v.ihx2, v.ihy2, v.eq ? 1 : 0, v.lap ? 1 : 0, v.pw2 ? 1 : 0};
+    a.lv[l] = CoarseLevel<T>{v.u,    v.rhs,  v.nx,         v.ny,          v.pitch,       v.hx2, v.hy2,
+                             v.coef, v.ihx2, v.ihy2, v.eq ? 1 : 0, v.lap ? 1 : 0, v.pw2 ? 1 : 0, v.u2};
     bytes += 3.0 * sizeof(T) * nodes(v);
   }
   a.l0 = H.lc;
@@ -383,6 +384,9 @@ template <typename T> void coarse(Ctx *c, Hier<T> &H, bool zero_first, double *s
   a.nuc = c->o.nu_coarse;
   a.zero_first = zero_first ? 1 : 0;
   a.snap = snap;
+  a.jac = c->o.smoother == VLP2D_SMOOTH_WJACOBI ? 1 : 0;
+  a.w = (T)c->o.omega;
+  a.fw = c->o.restriction == VLP2D_RESTRICT_FULL ? 1 : 0;
   // every level in LDS when it fits in one workgroup's share, else in global
   // memory (the working set then sits in L2)
   // (the LDS kernel maps work items with shifts: rows of 2^k + 1 nodes)
@@ -392,7 +396,7 @@ template <typename T> void coarse(Ctx *c, Hier<T> &H, bool zero_first, double *s
     nx[l] = H.lv[l].nx, ny[l] = H.lv[l].ny;
     pow2_rows = pow2_rows && ny[l] >= 3 && is_pow2(ny[l] - 1);
   }
-  const size_t lds = coarse_lds_bytes(nx, ny, H.lc, L, sizeof(T));
+  const size_t lds = coarse_lds_bytes(nx, ny, H.lc, L, sizeof(T), a.jac != 0);
   if (!(c->o.flags & VLP2D_FLAG_COARSE_L2) && pow2_rows && lds <= (size_t)c->max_lds) {
     if (lds > 65536) (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_coarse_lds<T>),
                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -501,7 +505,8 @@ bool fused_first_residual(const Ctx *c, const Lvl<double> &P) {
   const Hier<float> &Z = c->h32;
   if (c->simple || Z.lv.empty() || Z.lc == 0) return false;
   const Lvl<float> &z = Z.lv[0];
-  return z.pw2 == P.pw2 && z.pitch == P.pitch && recompute_presmooth(c, true, EPI_NONE);
+  return z.pw2 == P.pw2 && z.pitch == P.pitch && recompute_presmooth(c, true, EPI_NONE) &&
+         c->o.restriction == VLP2D_RESTRICT_HALF;
 }
 
 // max |u - snap| over interior nodes -> dscal[0]; pub: published for the
@@ -578,8 +583,8 @@ void mixed_step(Ctx *c, Lvl<double> &P, int m = 1, bool pub = false, bool first 
   const MixView *r64 = first && fused_first_residual(c, P) ? &rv : nullptr;
   const bool fused_refine = !c->simple && Z.lc > 0 && !(c->o.flags & VLP2D_FLAG_SEPARATE_REFINE) &&
                             z.pw2 == P.pw2 && z.pitch == P.pitch;
-  if (m == 2 && fused_refine && c->o.nu1 == 1 && c->o.nu2 <= 3 &&
-      !(c->o.flags & (VLP2D_FLAG_STORE_PRESMOOTH | VLP2D_FLAG_NO_CYCLE_FUSION))) {
+  if (m == 2 && fused_refine && c->o.nu1 == 1 && c->o.nu2 <= 3 && c->o.smoother == VLP2D_SMOOTH_RBGS &&
+      c->o.restriction == VLP2D_RESTRICT_HALF && !(c->o.flags & (VLP2D_FLAG_STORE_PRESMOOTH | VLP2D_FLAG_NO_CYCLE_FUSION))) {
     // first cycle down to level 1, level 0 of both cycles in between in one
     // launch (it stores the second cycle's pre-smoothed iterate in z.u2),
     // second cycle from level 1 with the refinement ascent staging it
@@ -787,7 +792,8 @@ template <typename T> int complete_hier(Ctx *c, Hier<T> &H) {
     Lvl<T> &v = H.lv[l];
     RCHK(need(v.u, v.elems()));
     RCHK(need(v.rhs, v.elems()));
-    if (l == 0 || l < H.lc) RCHK(need(v.u2, v.elems(), l == 0));
+    // (weighted Jacobi on the coarse levels: k_coarse's second iterate buffer)
+    if (l == 0 || l < H.lc || c->o.smoother == VLP2D_SMOOTH_WJACOBI) RCHK(need(v.u2, v.elems(), l == 0));
     if (c->simple && (l < c->L - 1 || c->o.smoother == VLP2D_SMOOTH_WJACOBI)) RCHK(need(v.res, v.elems()));
   }
   for (const New &m : made) {
@@ -1329,7 +1335,7 @@ static bool checked_opts(const vlp2d_mg_opts *opts, vlp2d_mg_opts &o) {
 // Gauss-Seidel and half weighting; everything else runs one operator per launch.
 static bool one_operator_path(const vlp2d_mg_opts &o) {
   return (o.flags & VLP2D_FLAG_SIMPLE) != 0 || o.nu1 < 1 || o.nu2 < 1 || o.nu1 > 3 || o.nu2 > 3 ||
-         o.smoother == VLP2D_SMOOTH_WJACOBI || o.restriction == VLP2D_RESTRICT_FULL || o.cycle == VLP2D_CYCLE_FMG;
+         o.cycle == VLP2D_CYCLE_FMG;
 }
 
 // nLevels = min(log2 segX, log2 segY) (MultiGridScheme.cs:23-61)

@@ -98,12 +98,22 @@ __device__ __forceinline__ void group_step() {
 constexpr int kPf = VLP2D_PF;
 constexpr int kNru = kPf <= 2 ? 3 : 6;  // prefetch ring slots (a divisor of every period)
 static_assert(kPf >= 1 && kPf <= 5, "prefetch distance");
-__host__ __device__ constexpr int s_period(int ns) {
-  return 2 * ns + kPf + 2 <= 6 ? 6 : 2 * ns + kPf + 2 <= 12 ? 12 : 18;
+// Smoothers of the streaming kernels: SM_GS, red-black Gauss-Seidel (the
+// reference's), 2 ns half-sweep levels; SM_JAC, weighted Jacobi (an
+// addition), ns sweep levels.  A level shrinks the exact region of a strip
+// by one row and column on each side, so the halo is the number of levels
+// (+2 for a residual and restriction, +1 or 2 for a residual), rounded up
+// to an even number (the tile origin stays on an even row and column).
+constexpr int SM_GS = 0, SM_JAC = 1;
+__host__ __device__ constexpr int sweep_levels(int ns, int sm) { return sm == SM_JAC ? ns : 2 * ns; }
+__host__ __device__ constexpr int even_up(int h) { return (h + 1) / 2 * 2; }
+__host__ __device__ constexpr int s_period(int ns, int sm = SM_GS) {
+  return sweep_levels(ns, sm) + kPf + 2 <= 6 ? 6 : sweep_levels(ns, sm) + kPf + 2 <= 12 ? 12 : 18;
 }
-__host__ __device__ constexpr int s_down_halo(int ns) { return 2 * ns + 2; }
-__host__ __device__ constexpr int s_up_halo(int ns, int epi) {
-  return (epi == 1 || epi == 3 || epi == 4) ? 2 * ns + 2 : (2 * ns < 2 ? 2 : 2 * ns);
+__host__ __device__ constexpr int s_down_halo(int ns, int sm = SM_GS) { return even_up(sweep_levels(ns, sm) + 2); }
+__host__ __device__ constexpr int s_up_halo(int ns, int epi, int sm = SM_GS) {
+  return (epi == 1 || epi == 3 || epi == 4) ? even_up(sweep_levels(ns, sm) + 1)
+                                            : (sweep_levels(ns, sm) < 2 ? 2 : even_up(sweep_levels(ns, sm)));
 }
 // Output rows per tile R (a launch parameter): the staged rows R + 2H must
 // be a whole number of loop bodies; the smallest such R >= want.
@@ -250,6 +260,7 @@ __device__ __forceinline__ void s_store(T *uout, int pitch, int gi, const Lane &
 // the halo.
 // ---------------------------------------------------------------------------
 template <typename T, int NS> struct SweepState {
+  static constexpr bool kJacobi = false;
   static constexpr int S = 2 * NS;
   static constexpr int P = s_period(NS);
   static_assert(S + kPf + 2 <= P, "right-hand-side ring too short for the pipeline");
@@ -275,7 +286,42 @@ template <typename T, int NS> struct SweepState {
     if constexpr (e == es) return U[s + 1][md<6>(p)];
     else return U[s][md<6>(p)];
   }
+  // staged pair of row q (phase J): level -1 the red element, 0 the black
+  template <int J> __device__ __forceinline__ void stage(T x0, T x1) {
+    U[0][md<6>(J)] = (J & 1) == 0 ? x0 : x1;
+    U[1][md<6>(J)] = (J & 1) == 0 ? x1 : x0;
+  }
 };
+
+// Weighted-Jacobi sweep state: level s (0 staged, s after sweep s) keeps
+// both elements of each row, since a Jacobi update reads the previous
+// iterate at all four neighbours; the same ring slots as SweepState.
+template <typename T, int NS> struct JacState {
+  static constexpr bool kJacobi = true;
+  static constexpr int S = NS;
+  static constexpr int P = s_period(NS, SM_JAC);
+  static_assert(S + kPf + 2 <= P, "right-hand-side ring too short for the pipeline");
+  T U[S + 1][6][2];
+  T F[P][2];
+  T RU[kNru][2];
+  __device__ void zero() {
+#pragma unroll
+    for (int s = 0; s < S + 1; ++s)
+#pragma unroll
+      for (int k = 0; k < 6; ++k) U[s][k][0] = U[s][k][1] = T(0);
+#pragma unroll
+    for (int k = 0; k < P; ++k) F[k][0] = F[k][1] = T(0);
+#pragma unroll
+    for (int k = 0; k < kNru; ++k) RU[k][0] = RU[k][1] = T(0);
+  }
+  template <int s, int J, int d, int e> __device__ __forceinline__ T full() { return U[s][md<6>(J - d)][e]; }
+  template <int J> __device__ __forceinline__ void stage(T x0, T x1) {
+    U[0][md<6>(J)][0] = x0;
+    U[0][md<6>(J)][1] = x1;
+  }
+};
+template <typename T, int NS, int SM>
+using SState = typename std::conditional<SM == SM_JAC, JacState<T, NS>, SweepState<T, NS>>::type;
 
 // Apply half-sweeps SA..SB (default 1..S) at step J (row q = q0 + J):
 // half-sweep s updates row q - s, using level s-1 rows q-s-1, q-s, q-s+1.
@@ -314,6 +360,51 @@ __device__ __forceinline__ void s_sweeps(SweepState<T, NS> &st, const LevelView<
   });
 }
 
+// Weighted-Jacobi sweeps SA..SB at step J: sweep s updates both elements of
+// row q - s from level s-1 rows q-s-1, q-s, q-s+1, u + w (g - u) with g the
+// Gauss-Seidel point formula (the oracle's jacobi_sweep).
+template <typename T, int NS, bool EQ, bool PW2, bool EDGE, bool ROWS, int J, int SA, int SB>
+__device__ __forceinline__ void j_sweeps(JacState<T, NS> &st, const LevelView<T> &a, const Lane &L, int gi0, int q) {
+  constexpr int P = JacState<T, NS>::P;
+  static_for<SA, SB + 1>([&](auto sc) {
+    constexpr int s = decltype(sc)::value;
+    constexpr int p = J - s;
+    const T o0 = st.U[s - 1][md<6>(p)][0], o1 = st.U[s - 1][md<6>(p)][1];
+    const T g0 = s_gs<T, EQ, PW2>(st.U[s - 1][md<6>(p - 1)][0], st.U[s - 1][md<6>(p + 1)][0], from_left(o1), o1,
+                                  st.F[md<P>(p)][0], a);
+    const T g1 = s_gs<T, EQ, PW2>(st.U[s - 1][md<6>(p - 1)][1], st.U[s - 1][md<6>(p + 1)][1], o0, from_right(o0),
+                                  st.F[md<P>(p)][1], a);
+    T n0 = o0 + a.w * (g0 - o0), n1 = o1 + a.w * (g1 - o1);
+    if constexpr (EDGE || ROWS) {
+      bool ok0 = true, ok1 = true;
+      if constexpr (EDGE) {
+        ok0 = L.in0;
+        ok1 = L.in1;
+      }
+      if constexpr (ROWS) {
+        const int gi = gi0 + q - s;
+        const bool rin = gi >= 1 && gi <= a.nx - 2;
+        ok0 = ok0 && rin;
+        ok1 = ok1 && rin;
+      }
+      n0 = ok0 ? n0 : o0;
+      n1 = ok1 ? n1 : o1;
+    }
+    st.U[s][md<6>(p)][0] = n0;
+    st.U[s][md<6>(p)][1] = n1;
+  });
+}
+
+// Sweep levels SA..SB of either smoother at step J.
+template <bool EQ, bool PW2, bool EDGE, bool ROWS, int J, int SA, int SB, typename T, int NS>
+__device__ __forceinline__ void sweeps(SweepState<T, NS> &st, const LevelView<T> &a, const Lane &L, int gi0, int q) {
+  s_sweeps<T, NS, EQ, PW2, EDGE, ROWS, J, SA, SB>(st, a, L, gi0, q);
+}
+template <bool EQ, bool PW2, bool EDGE, bool ROWS, int J, int SA, int SB, typename T, int NS>
+__device__ __forceinline__ void sweeps(JacState<T, NS> &st, const LevelView<T> &a, const Lane &L, int gi0, int q) {
+  j_sweeps<T, NS, EQ, PW2, EDGE, ROWS, J, SA, SB>(st, a, L, gi0, q);
+}
+
 // Load the raw pair of staged row q + kPf (u unless zero-initialised) and
 // its right-hand side.
 // ROWS = false: every row this loop body touches is an interior row of the
@@ -327,10 +418,9 @@ __device__ __forceinline__ void s_sweeps(SweepState<T, NS> &st, const LevelView<
 // the branch (refinement ascent 509 against 541 us, FP64 ascent 360 against
 // 371 us), the latency-bound ascents without it (the cycle-boundary kernel
 // 177 against 218 us, the FP64 level-1 ascent 70 against 76 us).
-template <typename T, int NS, bool ZI, bool EDGE, bool ROWS, int J, bool FBR>
-__device__ __forceinline__ void s_prefetch(SweepState<T, NS> &st, const LevelView<T> &a, const Lane &L, int gi0, int q,
-                                           int LR) {
-  constexpr int P = SweepState<T, NS>::P;
+template <bool ZI, bool EDGE, bool ROWS, int J, bool FBR, typename St, typename T>
+__device__ __forceinline__ void s_prefetch(St &st, const LevelView<T> &a, const Lane &L, int gi0, int q, int LR) {
+  constexpr int P = St::P;
   const int qq = q + kPf, gi = gi0 + qq;
   Pair<T> u{T(0), T(0)}, f{T(0), T(0)};
   if constexpr (ROWS) {
@@ -478,17 +568,24 @@ __device__ __forceinline__ void r64_stage(SweepState<T, NS> &st, R64State &rs, c
 // ---------------------------------------------------------------------------
 // Descent
 // ---------------------------------------------------------------------------
-template <typename T, int NS, bool ZI, bool STORE, bool EQ, bool PW2, bool EDGE, bool R64 = false>
+// SM: smoother (SM_GS, SM_JAC); FW: full-weighting restriction (an addition,
+// (1/16)[1 2 1; 2 4 2; 1 2 1], the oracle's restrict_fw) instead of half
+// weighting.
+template <typename T, int NS, bool ZI, bool STORE, bool EQ, bool PW2, bool EDGE, bool R64 = false, int SM = SM_GS,
+          bool FW = false>
 __device__ __forceinline__ void sdown_body(const LevelView<T> &a, const STile &tl, T *__restrict__ rhsc, int ncx,
                                            int ncy, int pc, const MixView &mx, double *__restrict__ part) {
-  static_assert(!R64 || (ZI && !STORE && sizeof(T) == 4), "the FP64-residual descent starts the FP32 correction");
-  constexpr int S = 2 * NS, H = s_down_halo(NS), P = s_period(NS), TJ = SCOLS - 2 * H;
+  static_assert(!R64 || (ZI && !STORE && sizeof(T) == 4 && SM == SM_GS && !FW),
+                "the FP64-residual descent starts the FP32 correction");
+  static_assert(SM == SM_GS || STORE, "a Jacobi-smoothed level is stored (the ascent recomputes Gauss-Seidel only)");
+  using St = SState<T, NS, SM>;
+  constexpr int S = St::S, H = s_down_halo(NS, SM), P = St::P, TJ = SCOLS - 2 * H;
   const int R = a.rows, LR = R + 2 * H;  // LR is a multiple of P
   const int gi0 = tl.gi0, cb = tl.cb;
   const Lane L = make_lane<T>(cb, H, TJ, a.ny, a.pitch);
-  SweepState<T, NS> st;
+  St st;
   st.zero();
-  T RR[3][2];  // residual of rows (slot p % 3); element 1 only on even rows
+  T RR[3][2];  // residual of rows (slot p % 3); element 1 only on even rows unless FW
 #pragma unroll
   for (int k = 0; k < 3; ++k) RR[k][0] = RR[k][1] = T(0);
   R64State rs;
@@ -509,7 +606,7 @@ __device__ __forceinline__ void sdown_body(const LevelView<T> &a, const STile &t
     // rows 0 .. kPf - 1 (the loop prefetches q + kPf)
     static_for<-kPf, 0>([&](auto jc) {
       constexpr int J = decltype(jc)::value;
-      s_prefetch<T, NS, ZI, EDGE, true, J, true>(st, a, L, gi0, J, LR);
+      s_prefetch<ZI, EDGE, true, J, true>(st, a, L, gi0, J, LR);
     });
   }
   const int Jc = L.m + cb / 2;  // coarse column of element 0 (fine column 2 Jc)
@@ -527,12 +624,9 @@ __device__ __forceinline__ void sdown_body(const LevelView<T> &a, const STile &t
       const int q = q0 + J;
       group_step();
       if constexpr (R64) r64_stage<T, NS, EDGE, ROWS, PW2, J>(st, rs, a, mx, L, boff8, gi0, q, R, H);
-      else s_prefetch<T, NS, ZI, EDGE, ROWS, J, true>(st, a, L, gi0, q, LR);
-      // level 0 of row q: red element (J & 1), black element
-      constexpr int er = J & 1;
-      st.U[0][md<6>(J)] = st.RU[md<kNru>(J)][er];
-      st.U[1][md<6>(J)] = st.RU[md<kNru>(J)][1 - er];
-      s_sweeps<T, NS, EQ, PW2, EDGE, ROWS, J>(st, a, L, gi0, q);
+      else s_prefetch<ZI, EDGE, ROWS, J, true>(st, a, L, gi0, q, LR);
+      st.template stage<J>(st.RU[md<kNru>(J)][0], st.RU[md<kNru>(J)][1]);
+      sweeps<EQ, PW2, EDGE, ROWS, J, 1, S>(st, a, L, gi0, q);
       // store row q - S (unless the ascent recomputes it)
       if constexpr (STORE) {
         const int p = q - S, gi = gi0 + p;
@@ -548,12 +642,13 @@ __device__ __forceinline__ void sdown_body(const LevelView<T> &a, const STile &t
         const T x0 = st.template full<S, J, d, 0>(), x1 = st.template full<S, J, d, 1>();
         const T a0 = st.template full<S, J, d + 1, 0>(), b0 = st.template full<S, J, d - 1, 0>();
         RR[md<3>(pr)][0] = s_res<T, PW2>(x0, a0, b0, from_left(x1), x1, f0, a);
-        if constexpr (md<2>(pr) == 0) {
+        if constexpr (FW || md<2>(pr) == 0) {
           const T a1 = st.template full<S, J, d + 1, 1>(), b1 = st.template full<S, J, d - 1, 1>();
           RR[md<3>(pr)][1] = s_res<T, PW2>(x1, a1, b1, x0, from_right(x0), f1, a);
         }
       }
-      // restriction at even row q - S - 2 (MultiGridScheme.cs:158-167)
+      // restriction at even row q - S - 2 (MultiGridScheme.cs:158-167; full
+      // weighting: the oracle's restrict_fw, corners summed in its order)
       {
         constexpr int px = J - S - 2;
         if constexpr (md<2>(px) == 0) {
@@ -564,7 +659,14 @@ __device__ __forceinline__ void sdown_body(const LevelView<T> &a, const STile &t
           const int I = (gi0 + p) / 2;
           bool go = p >= H && p < H + R;
           if constexpr (ROWS) go = go && I >= 1 && I <= ncx - 2;
-          if (go && oc_c) rc[I * pc] = T(0.125) * (r0 * T(4) + (rip + rim + rjp + rjm));
+          if constexpr (FW) {
+            const T cpp = RR[md<3>(px + 1)][1], cmp = RR[md<3>(px - 1)][1];
+            const T cpm = from_left(cpp), cmm = from_left(cmp);
+            if (go && oc_c)
+              rc[I * pc] = T(0.0625) * (r0 * T(4) + (rip + rim + rjp + rjm) * T(2) + (cpp + cpm + cmp + cmm));
+          } else {
+            if (go && oc_c) rc[I * pc] = T(0.125) * (r0 * T(4) + (rip + rim + rjp + rjm));
+          }
         }
       }
     });
@@ -579,18 +681,20 @@ __device__ __forceinline__ void sdown_body(const LevelView<T> &a, const STile &t
 }
 
 // STORE = false: the smoothed iterate is not written (the ascent recomputes
-// it, k_sup with N1 = NS).
-template <typename T, int NS, bool ZI, bool EQ, bool STORE = true>
+// it, k_sup with N1 = NS).  SM, FW as sdown_body.
+template <typename T, int NS, bool ZI, bool EQ, bool STORE = true, int SM = SM_GS, bool FW = false>
 __global__ void __launch_bounds__(64 * kWaves) k_sdown(LevelView<T> a, T *__restrict__ rhsc, int ncx, int ncy, int pc) {
-  constexpr int H = s_down_halo(NS);
+  constexpr int H = s_down_halo(NS, SM);
   STile tl;
   if (!s_tile<H>(a.rows, a.tiles_i, a.tiles_j, a.chunk, a.nx, a.ny, tl)) {
     s_idle(a.rows + 2 * H);
     return;
   }
   // every tile runs the column-predicated body
-  if (a.pw2) sdown_body<T, NS, ZI, STORE, EQ, true, true>(a, tl, rhsc, ncx, ncy, pc, MixView{}, nullptr);
-  else sdown_body<T, NS, ZI, STORE, EQ, false, true>(a, tl, rhsc, ncx, ncy, pc, MixView{}, nullptr);
+  if (a.pw2)
+    sdown_body<T, NS, ZI, STORE, EQ, true, true, false, SM, FW>(a, tl, rhsc, ncx, ncy, pc, MixView{}, nullptr);
+  else
+    sdown_body<T, NS, ZI, STORE, EQ, false, true, false, SM, FW>(a, tl, rhsc, ncx, ncy, pc, MixView{}, nullptr);
 }
 
 // The first FP32 descent of a mixed-precision solve on level 0 (zero
@@ -615,7 +719,7 @@ __global__ void __launch_bounds__(64 * kWaves) k_sdown_r64(LevelView<float> a, f
 // ---------------------------------------------------------------------------
 // Ascent
 // ---------------------------------------------------------------------------
-template <typename T, int NS, int N1, int EPI, bool EQ, bool PW2, bool EDGE, bool ZS, int NB>
+template <typename T, int NS, int N1, int EPI, bool EQ, bool PW2, bool EDGE, bool ZS, int NB, int SM = SM_GS>
 __device__ __forceinline__ void sup_body(const LevelView<T> &a, const STile &tl, const T *__restrict__ uc, int ncx,
                                          int ncy, int pc, double *__restrict__ part, const MixView &mx) {
   // N1 > 0: the level was pre-smoothed with N1 sweeps by the descent, which
@@ -625,15 +729,19 @@ __device__ __forceinline__ void sup_body(const LevelView<T> &a, const STile &tl,
   // bits) and adds the prolongation to level S1 of each row.
   // half-sweeps: 1..S1 recompute, S1+1..ST post-smoothing, ST+1..SE the
   // next cycle's pre-smoothing (EPI 4)
-  constexpr int NT = N1 + NS + NB, S1 = 2 * N1, ST = S1 + 2 * NS, SE = ST + 2 * NB;
-  constexpr int H = s_up_halo(NT, EPI), P = s_period(NT), TJ = SCOLS - 2 * H;
+  // (SM_JAC: NS Jacobi sweep levels, no recomputed or next-cycle sweeps)
+  constexpr int NT = N1 + NS + NB, S1 = 2 * N1, ST = S1 + sweep_levels(NS, SM), SE = ST + 2 * NB;
+  using St = SState<T, NT, SM>;
+  constexpr int H = s_up_halo(NT, EPI, SM), P = St::P, TJ = SCOLS - 2 * H;
+  static_assert(St::S == SE, "sweep levels of the state");
   static_assert(N1 == 0 || EPI != 2, "the update metric needs the staged iterate");
   static_assert((NB > 0) == (EPI == 4), "pre-sweeps of the next cycle belong to epilogue 4");
+  static_assert(SM == SM_GS || (N1 == 0 && NB == 0), "recomputed and next-cycle sweeps are Gauss-Seidel's");
   constexpr bool FBR = EPI == 1 || EPI == 3;  // see s_prefetch
   const int R = a.rows, LR = R + 2 * H;  // LR is a multiple of P
   const int gi0 = tl.gi0, cb = tl.cb;
   const Lane L = make_lane<T>(cb, H, TJ, a.ny, a.pitch);
-  SweepState<T, NT> st;
+  St st;
   st.zero();
   // coarse rows kk = 0, 1, ... of the strip: global row gi0/2 + kk, column
   // Jc = cb/2 + m (fine column gj = 2 Jc); ring slot kk % 3.  The right
@@ -698,7 +806,7 @@ __device__ __forceinline__ void sup_body(const LevelView<T> &a, const STile &tl,
   const bool oc_c = EDGE ? L.oc && Jc >= 1 && Jc <= ncy - 2 : L.oc;
   static_for<-kPf, 0>([&](auto jc) {
     constexpr int J = decltype(jc)::value;
-    s_prefetch<T, NT, ZS, EDGE, true, J, FBR>(st, a, L, gi0, J, LR);
+    s_prefetch<ZS, EDGE, true, J, FBR>(st, a, L, gi0, J, LR);
   });
 #pragma unroll 1
   for (int q0 = 0; q0 < LR; q0 += P) {
@@ -712,13 +820,11 @@ __device__ __forceinline__ void sup_body(const LevelView<T> &a, const STile &tl,
       constexpr int J = decltype(jc)::value;
       const int q = q0 + J;
       group_step();
-      s_prefetch<T, NT, ZS, EDGE, ROWS, J, FBR>(st, a, L, gi0, q, LR);
-      // staged row q (level -1: red element J & 1, level 0: black element)
-      constexpr int er = J & 1;
+      s_prefetch<ZS, EDGE, ROWS, J, FBR>(st, a, L, gi0, q, LR);
+      // staged row q (Gauss-Seidel: level -1 the red element, 0 the black)
       if constexpr (N1 > 0) {
-        st.U[0][md<6>(J)] = st.RU[md<kNru>(J)][er];
-        st.U[1][md<6>(J)] = st.RU[md<kNru>(J)][1 - er];
-        s_sweeps<T, NT, EQ, PW2, EDGE, ROWS, J, 1, S1>(st, a, L, gi0, q);
+        st.template stage<J>(st.RU[md<kNru>(J)][0], st.RU[md<kNru>(J)][1]);
+        sweeps<EQ, PW2, EDGE, ROWS, J, 1, S1>(st, a, L, gi0, q);
       }
       // prolongation of row pr = q - S1, whose level S1 is now complete
       {
@@ -755,8 +861,7 @@ __device__ __forceinline__ void sup_body(const LevelView<T> &a, const STile &tl,
           T v0 = st.RU[md<kNru>(J)][0], v1 = st.RU[md<kNru>(J)][1];
           v0 = a0 ? v0 + p0 : v0;
           v1 = a1 ? v1 + p1 : v1;
-          st.U[0][md<6>(J)] = er == 0 ? v0 : v1;
-          st.U[1][md<6>(J)] = er == 0 ? v1 : v0;
+          st.template stage<J>(v0, v1);
           if constexpr (EPI == 2) {
             V0[md<NV>(J)][0] = v0;
             V0[md<NV>(J)][1] = v1;
@@ -800,7 +905,7 @@ __device__ __forceinline__ void sup_body(const LevelView<T> &a, const STile &tl,
         FN[md<3>(J - ST + 1)][0] = f.x0;
         FN[md<3>(J - ST + 1)][1] = f.x1;
       }
-      s_sweeps<T, NT, EQ, PW2, EDGE, ROWS, J, S1 + 1, ST>(st, a, L, gi0, q);
+      sweeps<EQ, PW2, EDGE, ROWS, J, S1 + 1, ST>(st, a, L, gi0, q);
       // store row q - ST (+ update metric)
       {
         const int p = q - ST, gi = gi0 + p;
@@ -845,7 +950,7 @@ __device__ __forceinline__ void sup_body(const LevelView<T> &a, const STile &tl,
         // the next cycle's pre-smoothing of the rows just stored, then the
         // residual of row q - SE - 1 and the restriction at even row
         // q - SE - 2 (as k_sdown, MultiGridScheme.cs:134-169)
-        s_sweeps<T, NT, EQ, PW2, EDGE, ROWS, J, ST + 1, SE>(st, a, L, gi0, q);
+        sweeps<EQ, PW2, EDGE, ROWS, J, ST + 1, SE>(st, a, L, gi0, q);
         {
           // the next cycle's pre-smoothed iterate of row q - SE, which its
           // ascent stages (instead of recomputing it from this cycle's e)
@@ -969,10 +1074,11 @@ __device__ __forceinline__ void sup_body(const LevelView<T> &a, const STile &tl,
 // N1 > 0 pairs with k_sdown<T, N1, ZS, EQ, false> on the same level: ZS, the
 // descent started from zero; otherwise from a.uin, which the ascent then
 // stages (it may not write a.uin: only EPI 3, which writes no iterate).
-template <typename T, int NS, int EPI, bool EQ, int N1 = 0, bool ZS = true, int NB = 0>
+// SM: the smoother (SM_JAC with N1 = NB = 0 only).
+template <typename T, int NS, int EPI, bool EQ, int N1 = 0, bool ZS = true, int NB = 0, int SM = SM_GS>
 __global__ void __launch_bounds__(64 * kWaves) k_sup(LevelView<T> a, const T *__restrict__ uc, int ncx, int ncy, int pc,
                                             double *__restrict__ part, MixView mx) {
-  constexpr int H = s_up_halo(N1 + NS + NB, EPI);
+  constexpr int H = s_up_halo(N1 + NS + NB, EPI, SM);
   constexpr bool Z = ZS && N1 > 0;
   static_assert(ZS || (N1 > 0 && EPI == 3), "a recomputed non-zero start is read back in place: EPI 3 only");
   STile tl;
@@ -984,9 +1090,9 @@ __global__ void __launch_bounds__(64 * kWaves) k_sup(LevelView<T> a, const T *__
   // checks wherever all their rows are interior rows (measured faster than a
   // separate body for interior tiles, which also needs more registers).
   if (a.pw2) {
-    sup_body<T, NS, N1, EPI, EQ, true, true, Z, NB>(a, tl, uc, ncx, ncy, pc, part, mx);
+    sup_body<T, NS, N1, EPI, EQ, true, true, Z, NB, SM>(a, tl, uc, ncx, ncy, pc, part, mx);
   } else {
-    sup_body<T, NS, N1, EPI, EQ, false, true, Z, NB>(a, tl, uc, ncx, ncy, pc, part, mx);
+    sup_body<T, NS, N1, EPI, EQ, false, true, Z, NB, SM>(a, tl, uc, ncx, ncy, pc, part, mx);
   }
 }
 

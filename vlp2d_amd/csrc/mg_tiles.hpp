@@ -13,6 +13,7 @@ template <typename T> struct LevelView {
   int nx, ny, pitch;
   T hx2, hy2, coef;
   T ihx2, ihy2;  // 1/hx2, 1/hy2: exact when pw2 (both squares are powers of two)
+  T w;           // weighted-Jacobi weight (SM_JAC kernels)
   int eq, pw2;
   // row stride of f: pitch, or 0 on Laplace levels, whose right-hand side
   // array is all zeros (every row then reads row 0, which stays in L2), so
