@@ -28,7 +28,9 @@
 //
 // Within one colour the update is order independent, and every formula is
 // evaluated in the reference's operation order, so the result is bitwise the
-// reference's multigrid.  Rows and columns within distance s of the strip
+// reference's multigrid.  The weighted-Jacobi smoother (an addition, SM_JAC)
+// runs in the same pipeline with both elements of a row per sweep level
+// (JacState).  Rows and columns within distance s of the strip
 // edge hold partial results after half-sweep s; the halo H covers them.
 //
 // Every tile keeps Dirichlet nodes fixed with per-lane column predicates
@@ -260,7 +262,6 @@ __device__ __forceinline__ void s_store(T *uout, int pitch, int gi, const Lane &
 // the halo.
 // ---------------------------------------------------------------------------
 template <typename T, int NS> struct SweepState {
-  static constexpr bool kJacobi = false;
   static constexpr int S = 2 * NS;
   static constexpr int P = s_period(NS);
   static_assert(S + kPf + 2 <= P, "right-hand-side ring too short for the pipeline");
@@ -297,7 +298,6 @@ template <typename T, int NS> struct SweepState {
 // both elements of each row, since a Jacobi update reads the previous
 // iterate at all four neighbours; the same ring slots as SweepState.
 template <typename T, int NS> struct JacState {
-  static constexpr bool kJacobi = true;
   static constexpr int S = NS;
   static constexpr int P = s_period(NS, SM_JAC);
   static_assert(S + kPf + 2 <= P, "right-hand-side ring too short for the pipeline");
