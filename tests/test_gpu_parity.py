@@ -375,7 +375,7 @@ def test_config1_reference_stop_rule_cycle_count():
     s.cleanup()
 
 
-@pytest.mark.parametrize("nu", [(2, 2), (1, 3)])
+@pytest.mark.parametrize("nu", [(2, 2), (1, 3), (3, 1)])
 @pytest.mark.parametrize("mixed,inner", [(False, 1), (True, 1), (True, 2), (True, 3)])
 @pytest.mark.parametrize("cfg,seg", [(2, 512), (3, 256), (1, 128), (3, 1024)])
 def test_performance_mode_matches_oracle(cfg, seg, mixed, inner, nu):

@@ -200,11 +200,13 @@ template <typename T, int NS> void launch_down_ns(Ctx *c, Lvl<T> &f, Lvl<T> &g, 
 // recomputed; zi: f started from zero.  Returns the partials written.
 template <typename T, int NS>
 int launch_up_ns(Ctx *c, Lvl<T> &f, const Lvl<T> &g, int epi, bool rz, bool zi, const MixView &mx);
-// Level 0 between the two FP32 cycles of a mixed step (k_sup epilogue 4).
-template <int NS> void launch_up_down0_ns(Ctx *c, Lvl<float> &f, Lvl<float> &g);
-// First FP32 descent of a mixed solve (one pre-sweep), its right-hand side the
-// FP64 residual of mx.u64in (k_sdown_r64).  Returns the partials written.
-int launch_down_r64(Ctx *c, Lvl<float> &f, Lvl<float> &g, const MixView &mx);
+// Level 0 between the two FP32 cycles of a mixed step (k_sup epilogue 4;
+// nu1 pre-sweeps recomputed and run on the next cycle, 2 nu1 + NS <= 7).
+template <int NS> void launch_up_down0_ns(Ctx *c, Lvl<float> &f, Lvl<float> &g, int nu1);
+// First FP32 descent of a mixed solve (NS pre-sweeps, not stored), its
+// right-hand side the FP64 residual of mx.u64in (k_sdown_r64).  Returns the
+// partials written.
+template <int NS> int launch_down_r64_ns(Ctx *c, Lvl<float> &f, Lvl<float> &g, const MixView &mx);
 // Transposed cycle on FP64 level f above the coarsest (mg_k_adj.hip): A, the
 // transposed post-smoothing (ns sweeps) and prolongation into the coarse
 // level (ubc = P^T ubar, rbc = 0); B, the transposed restriction and residual

@@ -5,5 +5,6 @@
 namespace vlp2d {
 
 template void launch_down_ns<float, 3>(Ctx *, Lvl<float> &, Lvl<float> &, bool, bool);
+template int launch_down_r64_ns<3>(Ctx *, Lvl<float> &, Lvl<float> &, const MixView &);
 
 }  // namespace vlp2d

@@ -89,4 +89,38 @@ int launch_up_ns(Ctx *c, Lvl<T> &f, const Lvl<T> &g, int epi, bool rz, bool zi, 
   }
 }
 
+// Level 0 between the two FP32 cycles of a mixed step, one launch: the first
+// cycle's ascent (N1 pre-sweeps recomputed from zero, prolongation, NS
+// post-sweeps) continues into the second cycle's descent (N1 sweeps on e,
+// their result -> z.u2 for the second cycle's ascent, residual, restriction
+// into level 1's right-hand side).
+template <int N1, int NS> void launch_up_down0_k(Ctx *c, Lvl<float> &f, Lvl<float> &g) {
+  constexpr int NB = N1, H = s_up_halo(N1 + NS + NB, 4), P = s_period(N1 + NS + NB);
+  using K = void (*)(LevelView<float>, const float *, int, int, int, double *, MixView);
+  const K k = f.eq ? (K)k_sup<float, NS, 4, true, N1, true, NB> : (K)k_sup<float, NS, 4, false, N1, true, NB>;
+  LevelView<float> a = sview(c, (const void *)k, f, f.u, f.u2, H, P);  // zero start: f.u is not read
+  a.rc = g.rhs;
+  hipLaunchKernelGGL(k, dim3(8 * a.chunk), dim3(64 * kWaves), 0, c->st, a, g.u, g.nx, g.ny, g.pitch, c->part, MixView{});
+}
+
+// (the host runs it only for 2 nu1 + NS <= 7)
+template <int NS> void launch_up_down0_ns(Ctx *c, Lvl<float> &f, Lvl<float> &g, int nu1) {
+  if (nu1 == 1) launch_up_down0_k<1, NS>(c, f, g);
+  if constexpr (4 + NS <= 7) {
+    if (nu1 == 2) launch_up_down0_k<2, NS>(c, f, g);
+  }
+  if constexpr (6 + NS <= 7) {
+    if (nu1 == 3) launch_up_down0_k<3, NS>(c, f, g);
+  }
+}
+
+template <int NS> int launch_down_r64_ns(Ctx *c, Lvl<float> &f, Lvl<float> &g, const MixView &mx) {
+  constexpr int H = s_down_halo(NS), P = s_period(NS);
+  using K = void (*)(LevelView<float>, float *, int, int, int, MixView, double *);
+  const K k = f.eq ? (K)k_sdown_r64<NS, true> : (K)k_sdown_r64<NS, false>;
+  const LevelView<float> a = sview(c, (const void *)k, f, f.u, f.u2, H, P);
+  hipLaunchKernelGGL(k, dim3(8 * a.chunk), dim3(64 * kWaves), 0, c->st, a, g.rhs, g.nx, g.ny, g.pitch, mx, c->part);
+  return a.tiles_i * a.tiles_j;
+}
+
 }  // namespace vlp2d

@@ -418,15 +418,17 @@ void down_r64(Ctx *c, Lvl<float> &f, Lvl<float> &g, const MixView &mx);
 // r64 (FP32 hierarchies, zero0, from == 0): the level-0 descent takes the FP64
 // residual of r64->u64in as its right-hand side (down_r64).  stored0 (from ==
 // 1): level 0's pre-smoothed iterate is in its u2 (written by up_down0), and
-// the level-0 ascent stages it.
+// the level-0 ascent stages it.  recomputed0 (to == 1): level 0's ascent
+// (up_down0) recomputes its pre-smoothing, so its descent does not store it.
 template <typename T>
 int vcycle_fused(Ctx *c, Hier<T> &H, bool zero0, int epi, const MixView &mx, double *snap, int from = 0, int to = 0,
-                 const MixView *r64 = nullptr, bool stored0 = false) {
+                 const MixView *r64 = nullptr, bool stored0 = false, bool recomputed0 = false) {
   const int lc = H.lc;
   auto zi = [&](int l) { return l > 0 || zero0; };
   auto ep = [&](int l) { return l == 0 ? epi : (int)EPI_NONE; };
   auto rz = [&](int l) {
-    return !(l == 0 && stored0) && recompute_presmooth(c, zi(l), ep(l)) && !lds_ascent(c, H, l);
+    if (l == 0 && (stored0 || recomputed0)) return recomputed0;
+    return recompute_presmooth(c, zi(l), ep(l)) && !lds_ascent(c, H, l);
   };
   for (int l = from; l < lc; ++l) {
     if constexpr (sizeof(T) == 4) {
@@ -493,20 +495,41 @@ void reduce_part(Ctx *c, const double *part, int np, int slot, bool mx) {
 // right-hand side); ||r0||^2 -> dscal[2].
 void down_r64(Ctx *c, Lvl<float> &f, Lvl<float> &g, const MixView &mx) {
   int np = 0;
-  launch(c, KC_DOWN_R64, 0, 20.0 * nodes(f) + 4.0 * nodes(g), [&] { np = launch_down_r64(c, f, g, mx); });
+  launch(c, KC_DOWN_R64, 0, 20.0 * nodes(f) + 4.0 * nodes(g), [&] {
+    switch (c->o.nu1) {
+      case 1: np = launch_down_r64_ns<1>(c, f, g, mx); break;
+      case 2: np = launch_down_r64_ns<2>(c, f, g, mx); break;
+      default: np = launch_down_r64_ns<3>(c, f, g, mx); break;
+    }
+  });
   reduce_part(c, c->part, np, 2, false);
 }
 
+// Whether a mixed step of m FP32 cycles runs level 0 of its two cycles as one
+// launch (up_down0, k_sup epilogue 4): the refinement fused into the level-0
+// ascent, Gauss-Seidel and half weighting, and the boundary kernel's 2(2 nu1 +
+// nu2) half-sweeps within the longest pipeline (2 nu1 + nu2 <= 7).
+bool fused_two_cycle(const Ctx *c, const Lvl<double> &P, int m) {
+  const Hier<float> &Z = c->h32;
+  if (m != 2 || c->simple || Z.lv.empty() || Z.lc == 0) return false;
+  const Lvl<float> &z = Z.lv[0];
+  return !(c->o.flags & (VLP2D_FLAG_SEPARATE_REFINE | VLP2D_FLAG_STORE_PRESMOOTH | VLP2D_FLAG_NO_CYCLE_FUSION)) &&
+         z.pw2 == P.pw2 && z.pitch == P.pitch && c->o.smoother == VLP2D_SMOOTH_RBGS &&
+         c->o.restriction == VLP2D_RESTRICT_HALF && c->o.nu2 <= 3 && 2 * c->o.nu1 + c->o.nu2 <= 7;
+}
+
 // Whether the first refinement step of a mixed solve computes the initial
-// FP64 residual inside its first descent (one pre-sweep on fused levels, the
-// FP32 and FP64 level 0 sharing pitch and power-of-two steps) instead of a
-// separate residual pass before it.
+// FP64 residual inside its first descent (the descent stores no pre-smoothed
+// iterate: one pre-sweep, which the ascent recomputes, or the first cycle of a
+// two-cycle step, whose up_down0 recomputes it; the FP32 and FP64 level 0
+// sharing pitch and power-of-two steps) instead of a separate residual pass
+// before it.
 bool fused_first_residual(const Ctx *c, const Lvl<double> &P) {
   const Hier<float> &Z = c->h32;
   if (c->simple || Z.lv.empty() || Z.lc == 0) return false;
   const Lvl<float> &z = Z.lv[0];
-  return z.pw2 == P.pw2 && z.pitch == P.pitch && recompute_presmooth(c, true, EPI_NONE) &&
-         c->o.restriction == VLP2D_RESTRICT_HALF;
+  return z.pw2 == P.pw2 && z.pitch == P.pitch && c->o.restriction == VLP2D_RESTRICT_HALF &&
+         (recompute_presmooth(c, true, EPI_NONE) || fused_two_cycle(c, P, c->o.max_inner));
 }
 
 // max |u - snap| over interior nodes -> dscal[0]; pub: published for the
@@ -547,9 +570,9 @@ void cycle64(Ctx *c, Hier<double> &H, int epi, bool pub = false) {
 void up_down0(Ctx *c, Lvl<float> &f, Lvl<float> &g) {
   launch(c, KC_UP_DOWN, 0, 4.0 * (2.0 * nodes(f) + 2.0 * nodes(g)), [&] {
     switch (c->o.nu2) {
-      case 1: launch_up_down0_ns<1>(c, f, g); break;
-      case 2: launch_up_down0_ns<2>(c, f, g); break;
-      default: launch_up_down0_ns<3>(c, f, g); break;
+      case 1: launch_up_down0_ns<1>(c, f, g, c->o.nu1); break;
+      case 2: launch_up_down0_ns<2>(c, f, g, c->o.nu1); break;
+      default: launch_up_down0_ns<3>(c, f, g, c->o.nu1); break;
     }
   });
 }
@@ -583,12 +606,12 @@ void mixed_step(Ctx *c, Lvl<double> &P, int m = 1, bool pub = false, bool first 
   const MixView *r64 = first && fused_first_residual(c, P) ? &rv : nullptr;
   const bool fused_refine = !c->simple && Z.lc > 0 && !(c->o.flags & VLP2D_FLAG_SEPARATE_REFINE) &&
                             z.pw2 == P.pw2 && z.pitch == P.pitch;
-  if (m == 2 && fused_refine && c->o.nu1 == 1 && c->o.nu2 <= 3 && c->o.smoother == VLP2D_SMOOTH_RBGS &&
-      c->o.restriction == VLP2D_RESTRICT_HALF && !(c->o.flags & (VLP2D_FLAG_STORE_PRESMOOTH | VLP2D_FLAG_NO_CYCLE_FUSION))) {
-    // first cycle down to level 1, level 0 of both cycles in between in one
-    // launch (it stores the second cycle's pre-smoothed iterate in z.u2),
-    // second cycle from level 1 with the refinement ascent staging it
-    vcycle_fused(c, Z, true, EPI_NONE, none, nullptr, 0, 1, r64);
+  if (fused_two_cycle(c, P, m)) {
+    // first cycle down to level 1 (its level-0 pre-smoothing recomputed by
+    // up_down0, not stored), level 0 of both cycles in between in one launch
+    // (it stores the second cycle's pre-smoothed iterate in z.u2), second
+    // cycle from level 1 with the refinement ascent staging it
+    vcycle_fused(c, Z, true, EPI_NONE, none, nullptr, 0, 1, r64, false, true);
     up_down0(c, z, Z.lv[1]);
     const int np = vcycle_fused(c, Z, false, EPI_MIXED, mx, nullptr, 1, 0, nullptr, true);
     finish(c, c->part2, np, c->part, np, pub);
