@@ -187,21 +187,25 @@ inline size_t coarse_lds_bytes(const int *nx, const int *ny, int l0, int L, size
 
 __device__ __forceinline__ int ilog2(int x) { return 31 - __builtin_clz(x); }
 
-// Level l's arrays in LDS: levels l0 .. l-1 precede it (the offsets are
-// uniform scalar arithmetic on the kernel arguments).
-template <typename T> __device__ inline LdsLevel<T> cl_level(const CoarseArgs<T> &a, T *sm, int l) {
-  int o = 0;
-  for (int k = a.l0; k < l; ++k) o += 2 * a.lv[k].nx * a.lv[k].ny;
-  const int n = a.lv[l].nx * a.lv[l].ny;
-  return LdsLevel<T>{sm + o, sm + o + n, a.lv[l].nx, a.lv[l].ny, ilog2(a.lv[l].ny - 1)};
+// Level l's arrays in LDS (levels l0 .. l-1 precede it) from the table the
+// kernel builds once at its start: offset of u, nx, ny per level.
+struct ClTab {
+  int off[kMaxLevels], nx[kMaxLevels], ny[kMaxLevels];
+};
+template <typename T> __device__ inline LdsLevel<T> cl_level(const ClTab &tb, T *sm, int l) {
+  const int o = tb.off[l], nx = tb.nx[l], ny = tb.ny[l];
+  return LdsLevel<T>{sm + o, sm + o + nx * ny, nx, ny, ilog2(ny - 1)};
 }
 
 // Work items are mapped to nodes with shifts and masks only (row lengths are
 // powers of two plus one): one item per node of one colour in a half-sweep,
 // one per interior node (column 0 of each row idle) elsewhere.
+// The LDS kernel's phases take the level's parameters and the smoother by
+// value: copied once per phase, they stay in scalar registers instead of
+// being reloaded from the kernel arguments inside the item loops.
 template <typename T>
-__device__ inline void cl_smooth(const LdsLevel<T> &v, const CoarseLevel<T> &g, int n, const CoarseArgs<T> &a, T *tmp) {
-  if (a.jac) {
+__device__ inline void cl_smooth(const LdsLevel<T> &v, const CoarseLevel<T> g, int n, const int jac, const T w, T *tmp) {
+  if (jac) {
     // weighted Jacobi: new values into tmp (LDS after the levels), then back
     const int items = (v.nx - 2) << v.ly;
     for (int s = 0; s < n; ++s) {
@@ -210,7 +214,7 @@ __device__ inline void cl_smooth(const LdsLevel<T> &v, const CoarseLevel<T> &g, 
         if (j == 0) continue;
         const int c = i * v.ny + j;
         const T o = v.u[c];
-        tmp[c] = o + a.w * (c_point(v.u, v.f, c, v.ny, g) - o);
+        tmp[c] = o + w * (c_point(v.u, v.f, c, v.ny, g) - o);
       }
       __syncthreads();
       for (int p = threadIdx.x; p < items; p += CNT) {
@@ -249,7 +253,7 @@ __device__ inline void cl_smooth(const LdsLevel<T> &v, const CoarseLevel<T> &g, 
     }
 }
 
-template <typename T> __device__ inline T cl_resid(const LdsLevel<T> &v, const CoarseLevel<T> &g, int c) {
+template <typename T> __device__ inline T cl_resid(const LdsLevel<T> &v, const CoarseLevel<T> g, int c) {
   const T *u = v.u;
   const int P = v.ny;
   const T u2 = u[c] * T(2);
@@ -261,13 +265,28 @@ template <typename T>
 __global__ void __launch_bounds__(CNT) k_coarse_lds(CoarseArgs<T> a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T *sm = reinterpret_cast<T *>(smem_raw);
+  // the level table in LDS, built once (the kernel arguments are read here
+  // only, not inside every phase)
+  __shared__ ClTab tb;
+  if (threadIdx.x == 0) {
+    int o = 0;
+    for (int k = a.l0; k < a.L; ++k) {
+      tb.off[k] = o;
+      tb.nx[k] = a.lv[k].nx;
+      tb.ny[k] = a.lv[k].ny;
+      o += 2 * a.lv[k].nx * a.lv[k].ny;
+    }
+  }
+  __syncthreads();
   // Jacobi's new values: after the last level
-  T *const tmp = sm + (cl_level(a, sm, a.L - 1).f - sm) + a.lv[a.L - 1].nx * a.lv[a.L - 1].ny;
+  T *const tmp = cl_level(tb, sm, a.L - 1).f + tb.nx[a.L - 1] * tb.ny[a.L - 1];
+  const int jac = a.jac, fw = a.fw;
+  const T w = a.w;
   {
     // level l0: the whole iterate (its boundary feeds the stencils; interior
     // zeroed when zero_first) and the right-hand side (0 on Laplace levels)
     const CoarseLevel<T> &g = a.lv[a.l0];
-    const LdsLevel<T> v = cl_level(a, sm, a.l0);
+    const LdsLevel<T> v = cl_level(tb, sm, a.l0);
     const int lr = v.ly + 1;  // 2^lr >= ny items per row
     for (int p = threadIdx.x; p < v.nx << lr; p += CNT) {
       const int i = p >> lr, j = p & ((1 << lr) - 1);
@@ -279,23 +298,23 @@ __global__ void __launch_bounds__(CNT) k_coarse_lds(CoarseArgs<T> a) {
     __syncthreads();
   }
   for (int l = a.l0; l <= a.L - 2; ++l) {
-    const LdsLevel<T> f = cl_level(a, sm, l), g = cl_level(a, sm, l + 1);
-    const CoarseLevel<T> &fg = a.lv[l];
-    cl_smooth(f, fg, a.nu1, a, tmp);
+    const LdsLevel<T> f = cl_level(tb, sm, l), g = cl_level(tb, sm, l + 1);
+    const CoarseLevel<T> fg = a.lv[l];
+    cl_smooth(f, fg, a.nu1, jac, w, tmp);
     for (int p = threadIdx.x; p < (g.nx - 2) << g.ly; p += CNT) {
       const int I = 1 + (p >> g.ly), J = p & ((1 << g.ly) - 1);
       if (J == 0) continue;
       const int c = 2 * I * f.ny + 2 * J;
-      g.f[I * g.ny + J] = c_restrict<T>([&](int k) { return cl_resid(f, fg, k); }, c, f.ny, a.fw);
+      g.f[I * g.ny + J] = c_restrict<T>([&](int k) { return cl_resid(f, fg, k); }, c, f.ny, fw);
     }
     // the coarse iterate, boundary included, starts at 0 (the coarse
     // right-hand side's boundary is never read)
     for (int p = threadIdx.x; p < g.nx * g.ny; p += CNT) g.u[p] = T(0);
     __syncthreads();
   }
-  cl_smooth(cl_level(a, sm, a.L - 1), a.lv[a.L - 1], a.nuc, a, tmp);
+  cl_smooth(cl_level(tb, sm, a.L - 1), a.lv[a.L - 1], a.nuc, jac, w, tmp);
   for (int l = a.L - 2; l >= a.l0; --l) {
-    const LdsLevel<T> f = cl_level(a, sm, l), g = cl_level(a, sm, l + 1);
+    const LdsLevel<T> f = cl_level(tb, sm, l), g = cl_level(tb, sm, l + 1);
     for (int p = threadIdx.x; p < (f.nx - 2) << f.ly; p += CNT) {
       const int i = 1 + (p >> f.ly), j = p & ((1 << f.ly) - 1);
       if (j == 0) continue;
@@ -305,13 +324,13 @@ __global__ void __launch_bounds__(CNT) k_coarse_lds(CoarseArgs<T> a) {
       if (l == 0 && a.snap) a.snap[i * a.lv[0].pitch + j] = (double)v;
     }
     __syncthreads();
-    cl_smooth(f, a.lv[l], a.nu2, a, tmp);
+    cl_smooth(f, a.lv[l], a.nu2, jac, w, tmp);
   }
   {
     // level l0's interior iterate back to global memory; the coarser levels
     // live and die in LDS
     const CoarseLevel<T> &g = a.lv[a.l0];
-    const LdsLevel<T> v = cl_level(a, sm, a.l0);
+    const LdsLevel<T> v = cl_level(tb, sm, a.l0);
     for (int p = threadIdx.x; p < (v.nx - 2) << v.ly; p += CNT) {
       const int i = 1 + (p >> v.ly), j = p & ((1 << v.ly) - 1);
       if (j == 0) continue;

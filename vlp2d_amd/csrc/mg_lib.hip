@@ -397,7 +397,8 @@ template <typename T> void coarse(Ctx *c, Hier<T> &H, bool zero_first, double *s
     pow2_rows = pow2_rows && ny[l] >= 3 && is_pow2(ny[l] - 1);
   }
   const size_t lds = coarse_lds_bytes(nx, ny, H.lc, L, sizeof(T), a.jac != 0);
-  if (!(c->o.flags & VLP2D_FLAG_COARSE_L2) && pow2_rows && lds <= (size_t)c->max_lds) {
+  // (the kernel's static level table shares the workgroup's LDS)
+  if (!(c->o.flags & VLP2D_FLAG_COARSE_L2) && pow2_rows && lds + sizeof(ClTab) <= (size_t)c->max_lds) {
     if (lds > 65536) (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_coarse_lds<T>),
                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     launch(c, KC_COARSE, H.lc, bytes,
@@ -766,11 +767,12 @@ template <typename T> void level_params(Lvl<T> &v, T hx0, T hy0, int segX, int s
 }
 
 // Levels at or below this size run in the one-workgroup coarse kernel; larger
-// ones in the streaming kernels (measured on MI355X at 8193^2: 33^2 is the
-// fastest split, a streaming launch on a small level costs ~10 us).
+// ones in the streaming kernels (measured on MI355X at 8193^2: with 65^2 the
+// coarse cycle takes 41-44 us against 46-48 us for it at 33^2 plus the two
+// streaming launches of the 65^2 level).
 int coarse_threshold(const Ctx *c) {
   if (c->o.coarse_nodes > 0) return c->o.coarse_nodes;
-  return 33 * 33;
+  return 65 * 65;
 }
 
 template <typename T> void shape_hier(Ctx *c, Hier<T> &H, T hx0, T hy0) {
