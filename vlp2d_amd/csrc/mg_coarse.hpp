@@ -187,12 +187,16 @@ inline size_t coarse_lds_bytes(const int *nx, const int *ny, int l0, int L, size
 
 __device__ __forceinline__ int ilog2(int x) { return 31 - __builtin_clz(x); }
 
-// Level l's arrays in LDS (levels l0 .. l-1 precede it) from the table the
-// kernel builds once at its start: offset of u, nx, ny per level.
-struct ClTab {
+// The table the kernel builds in LDS at its start: each level's parameters
+// (one thread per level copies them from the kernel arguments, all in
+// flight together) and the offset of its arrays in LDS, so that no phase
+// reads the kernel arguments again.  Level l's arrays follow those of levels
+// l0 .. l-1.
+template <typename T> struct ClTab {
   int off[kMaxLevels], nx[kMaxLevels], ny[kMaxLevels];
+  CoarseLevel<T> lv[kMaxLevels];
 };
-template <typename T> __device__ inline LdsLevel<T> cl_level(const ClTab &tb, T *sm, int l) {
+template <typename T> __device__ inline LdsLevel<T> cl_level(const ClTab<T> &tb, T *sm, int l) {
   const int o = tb.off[l], nx = tb.nx[l], ny = tb.ny[l];
   return LdsLevel<T>{sm + o, sm + o + nx * ny, nx, ny, ilog2(ny - 1)};
 }
@@ -267,14 +271,16 @@ __global__ void __launch_bounds__(CNT) k_coarse_lds(CoarseArgs<T> a) {
   T *sm = reinterpret_cast<T *>(smem_raw);
   // the level table in LDS, built once (the kernel arguments are read here
   // only, not inside every phase)
-  __shared__ ClTab tb;
+  __shared__ ClTab<T> tb;
+  if ((int)threadIdx.x < a.L - a.l0) tb.lv[a.l0 + threadIdx.x] = a.lv[a.l0 + threadIdx.x];
+  __syncthreads();
   if (threadIdx.x == 0) {
     int o = 0;
     for (int k = a.l0; k < a.L; ++k) {
       tb.off[k] = o;
-      tb.nx[k] = a.lv[k].nx;
-      tb.ny[k] = a.lv[k].ny;
-      o += 2 * a.lv[k].nx * a.lv[k].ny;
+      tb.nx[k] = tb.lv[k].nx;
+      tb.ny[k] = tb.lv[k].ny;
+      o += 2 * tb.nx[k] * tb.ny[k];
     }
   }
   __syncthreads();
@@ -284,22 +290,44 @@ __global__ void __launch_bounds__(CNT) k_coarse_lds(CoarseArgs<T> a) {
   const T w = a.w;
   {
     // level l0: the whole iterate (its boundary feeds the stencils; interior
-    // zeroed when zero_first) and the right-hand side (0 on Laplace levels)
-    const CoarseLevel<T> &g = a.lv[a.l0];
+    // zeroed when zero_first) and the right-hand side (0 on Laplace levels).
+    // Its parameters are copied once; each thread then issues the global
+    // loads of kLdBatch nodes before it writes any of them to LDS, so the
+    // memory latency is paid once per batch instead of once per node (one
+    // batch covers 65 x 65 nodes).
+    constexpr int kLdBatch = 8;
+    const CoarseLevel<T> g = tb.lv[a.l0];
     const LdsLevel<T> v = cl_level(tb, sm, a.l0);
-    const int lr = v.ly + 1;  // 2^lr >= ny items per row
-    for (int p = threadIdx.x; p < v.nx << lr; p += CNT) {
-      const int i = p >> lr, j = p & ((1 << lr) - 1);
-      if (j >= v.ny) continue;
-      const bool in = i > 0 && i < v.nx - 1 && j > 0 && j < v.ny - 1;
-      v.u[i * v.ny + j] = a.zero_first && in ? T(0) : g.u[i * g.pitch + j];
-      v.f[i * v.ny + j] = g.lap ? T(0) : g.f[i * g.pitch + j];
+    const int n = v.nx * v.ny;
+    const bool zf = a.zero_first != 0;
+    for (int base = 0; base < n; base += kLdBatch * CNT) {
+      T uu[kLdBatch], ff[kLdBatch];
+#pragma unroll
+      for (int k = 0; k < kLdBatch; ++k) {
+        const int p = base + k * CNT + (int)threadIdx.x;
+        const int i = p / v.ny, j = p - i * v.ny;
+        const bool in = i > 0 && i < v.nx - 1 && j > 0 && j < v.ny - 1;
+        uu[k] = T(0);
+        ff[k] = T(0);
+        if (p < n) {
+          if (!(zf && in)) uu[k] = g.u[i * g.pitch + j];
+          if (!g.lap) ff[k] = g.f[i * g.pitch + j];
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < kLdBatch; ++k) {
+        const int p = base + k * CNT + (int)threadIdx.x;
+        if (p < n) {
+          v.u[p] = uu[k];
+          v.f[p] = ff[k];
+        }
+      }
     }
     __syncthreads();
   }
   for (int l = a.l0; l <= a.L - 2; ++l) {
     const LdsLevel<T> f = cl_level(tb, sm, l), g = cl_level(tb, sm, l + 1);
-    const CoarseLevel<T> fg = a.lv[l];
+    const CoarseLevel<T> fg = tb.lv[l];
     cl_smooth(f, fg, a.nu1, jac, w, tmp);
     for (int p = threadIdx.x; p < (g.nx - 2) << g.ly; p += CNT) {
       const int I = 1 + (p >> g.ly), J = p & ((1 << g.ly) - 1);
@@ -312,24 +340,26 @@ __global__ void __launch_bounds__(CNT) k_coarse_lds(CoarseArgs<T> a) {
     for (int p = threadIdx.x; p < g.nx * g.ny; p += CNT) g.u[p] = T(0);
     __syncthreads();
   }
-  cl_smooth(cl_level(tb, sm, a.L - 1), a.lv[a.L - 1], a.nuc, jac, w, tmp);
+  cl_smooth(cl_level(tb, sm, a.L - 1), tb.lv[a.L - 1], a.nuc, jac, w, tmp);
   for (int l = a.L - 2; l >= a.l0; --l) {
     const LdsLevel<T> f = cl_level(tb, sm, l), g = cl_level(tb, sm, l + 1);
+    double *const snap = l == 0 ? a.snap : nullptr;
+    const int sp = a.lv[0].pitch;
     for (int p = threadIdx.x; p < (f.nx - 2) << f.ly; p += CNT) {
       const int i = 1 + (p >> f.ly), j = p & ((1 << f.ly) - 1);
       if (j == 0) continue;
       const int c = i * f.ny + j;
       const T v = f.u[c] + prolong_corr(g.u, g.ny, i, j);
       f.u[c] = v;
-      if (l == 0 && a.snap) a.snap[i * a.lv[0].pitch + j] = (double)v;
+      if (snap) snap[i * sp + j] = (double)v;
     }
     __syncthreads();
-    cl_smooth(f, a.lv[l], a.nu2, jac, w, tmp);
+    cl_smooth(f, tb.lv[l], a.nu2, jac, w, tmp);
   }
   {
     // level l0's interior iterate back to global memory; the coarser levels
     // live and die in LDS
-    const CoarseLevel<T> &g = a.lv[a.l0];
+    const CoarseLevel<T> g = tb.lv[a.l0];
     const LdsLevel<T> v = cl_level(tb, sm, a.l0);
     for (int p = threadIdx.x; p < (v.nx - 2) << v.ly; p += CNT) {
       const int i = 1 + (p >> v.ly), j = p & ((1 << v.ly) - 1);

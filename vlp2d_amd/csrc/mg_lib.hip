@@ -398,7 +398,7 @@ template <typename T> void coarse(Ctx *c, Hier<T> &H, bool zero_first, double *s
   }
   const size_t lds = coarse_lds_bytes(nx, ny, H.lc, L, sizeof(T), a.jac != 0);
   // (the kernel's static level table shares the workgroup's LDS)
-  if (!(c->o.flags & VLP2D_FLAG_COARSE_L2) && pow2_rows && lds + sizeof(ClTab) <= (size_t)c->max_lds) {
+  if (!(c->o.flags & VLP2D_FLAG_COARSE_L2) && pow2_rows && lds + sizeof(ClTab<T>) <= (size_t)c->max_lds) {
     if (lds > 65536) (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_coarse_lds<T>),
                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     launch(c, KC_COARSE, H.lc, bytes,
