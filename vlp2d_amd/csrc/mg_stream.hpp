@@ -7,7 +7,7 @@
 // top to bottom, one staged row per step, LR = R + 2H rows in all.  Every
 // row is loaded from HBM once per tile; all red-black Gauss-Seidel
 // half-sweeps of the launch are pipelined over the rows (temporal blocking in
-// registers): at step q the wavefront loads row q + kPf (prefetch), applies
+// registers): at step q the wavefront loads row q + PF (prefetch), applies
 // half-sweep s to row q - s for s = 1..S, and runs the epilogue on the rows
 // the last half-sweep has finished.  Vertical neighbours are registers of the
 // same lane; horizontal neighbours of the other lane come through DPP
@@ -93,12 +93,16 @@ __device__ __forceinline__ void group_step() {
 
 // Steps per unrolled loop body: a multiple of the colour period (2), the
 // level rings (6) and the right-hand-side ring (S + 4 rows live).
-// Prefetch distance: staged row q + kPf is loaded at step q.
+// Prefetch distance: staged row q + PF is loaded at step q.  kPf is the
+// default of the streaming kernels' PF parameter; a launcher may pick another
+// per kernel (measured: 3 suits the latency-bound FP32 level-1 ascent and the
+// first FP32 descent with the FP64 residual, 2 everything else).
 #ifndef VLP2D_PF
 #define VLP2D_PF 2
 #endif
 constexpr int kPf = VLP2D_PF;
-constexpr int kNru = kPf <= 2 ? 3 : 6;  // prefetch ring slots (a divisor of every period)
+__host__ __device__ constexpr int pf_ring(int pf) { return pf <= 2 ? 3 : 6; }  // prefetch ring slots (a divisor of every period)
+constexpr int kNru = pf_ring(kPf);
 static_assert(kPf >= 1 && kPf <= 5, "prefetch distance");
 // Smoothers of the streaming kernels: SM_GS, red-black Gauss-Seidel (the
 // reference's), 2 ns half-sweep levels; SM_JAC, weighted Jacobi (an
@@ -109,8 +113,11 @@ static_assert(kPf >= 1 && kPf <= 5, "prefetch distance");
 constexpr int SM_GS = 0, SM_JAC = 1;
 __host__ __device__ constexpr int sweep_levels(int ns, int sm) { return sm == SM_JAC ? ns : 2 * ns; }
 __host__ __device__ constexpr int even_up(int h) { return (h + 1) / 2 * 2; }
-__host__ __device__ constexpr int s_period(int ns, int sm = SM_GS) {
-  return sweep_levels(ns, sm) + kPf + 2 <= 6 ? 6 : sweep_levels(ns, sm) + kPf + 2 <= 12 ? 12 : 18;
+__host__ __device__ constexpr int s_period(int ns, int sm = SM_GS, int pf = kPf) {
+  return sweep_levels(ns, sm) + pf + 2 <= 6    ? 6
+         : sweep_levels(ns, sm) + pf + 2 <= 12 ? 12
+         : sweep_levels(ns, sm) + pf + 2 <= 18 ? 18
+                                               : 24;
 }
 __host__ __device__ constexpr int s_down_halo(int ns, int sm = SM_GS) { return even_up(sweep_levels(ns, sm) + 2); }
 __host__ __device__ constexpr int s_up_halo(int ns, int epi, int sm = SM_GS) {
@@ -261,13 +268,14 @@ __device__ __forceinline__ void s_store(T *uout, int pitch, int gi, const Lane &
 // rows above the first staged row then read zeros, and their results lie in
 // the halo.
 // ---------------------------------------------------------------------------
-template <typename T, int NS> struct SweepState {
+template <typename T, int NS, int PFD = kPf> struct SweepState {
   static constexpr int S = 2 * NS;
-  static constexpr int P = s_period(NS);
-  static_assert(S + kPf + 2 <= P, "right-hand-side ring too short for the pipeline");
+  static constexpr int PF = PFD, NRU = pf_ring(PFD);
+  static constexpr int P = s_period(NS, SM_GS, PF);
+  static_assert(S + PF + 2 <= P, "right-hand-side ring too short for the pipeline");
   T U[S + 2][6];  // U[s + 1][p % 6]: level s in -1..S
   T F[P][2];      // right-hand side of row p (0 on Laplace levels and outside the domain)
-  T RU[kNru][2];  // prefetched staged rows (raw)
+  T RU[NRU][2];   // prefetched staged rows (raw)
   __device__ void zero() {
 #pragma unroll
     for (int s = 0; s < S + 2; ++s)
@@ -276,7 +284,7 @@ template <typename T, int NS> struct SweepState {
 #pragma unroll
     for (int k = 0; k < P; ++k) F[k][0] = F[k][1] = T(0);
 #pragma unroll
-    for (int k = 0; k < kNru; ++k) RU[k][0] = RU[k][1] = T(0);
+    for (int k = 0; k < NRU; ++k) RU[k][0] = RU[k][1] = T(0);
   }
   // full value of element e of row p = q - d (J the step phase) after
   // half-sweep s
@@ -297,13 +305,14 @@ template <typename T, int NS> struct SweepState {
 // Weighted-Jacobi sweep state: level s (0 staged, s after sweep s) keeps
 // both elements of each row, since a Jacobi update reads the previous
 // iterate at all four neighbours; the same ring slots as SweepState.
-template <typename T, int NS> struct JacState {
+template <typename T, int NS, int PFD = kPf> struct JacState {
   static constexpr int S = NS;
-  static constexpr int P = s_period(NS, SM_JAC);
-  static_assert(S + kPf + 2 <= P, "right-hand-side ring too short for the pipeline");
+  static constexpr int PF = PFD, NRU = pf_ring(PFD);
+  static constexpr int P = s_period(NS, SM_JAC, PF);
+  static_assert(S + PF + 2 <= P, "right-hand-side ring too short for the pipeline");
   T U[S + 1][6][2];
   T F[P][2];
-  T RU[kNru][2];
+  T RU[NRU][2];
   __device__ void zero() {
 #pragma unroll
     for (int s = 0; s < S + 1; ++s)
@@ -312,7 +321,7 @@ template <typename T, int NS> struct JacState {
 #pragma unroll
     for (int k = 0; k < P; ++k) F[k][0] = F[k][1] = T(0);
 #pragma unroll
-    for (int k = 0; k < kNru; ++k) RU[k][0] = RU[k][1] = T(0);
+    for (int k = 0; k < NRU; ++k) RU[k][0] = RU[k][1] = T(0);
   }
   template <int s, int J, int d, int e> __device__ __forceinline__ T full() { return U[s][md<6>(J - d)][e]; }
   template <int J> __device__ __forceinline__ void stage(T x0, T x1) {
@@ -320,15 +329,16 @@ template <typename T, int NS> struct JacState {
     U[0][md<6>(J)][1] = x1;
   }
 };
-template <typename T, int NS, int SM>
-using SState = typename std::conditional<SM == SM_JAC, JacState<T, NS>, SweepState<T, NS>>::type;
+template <typename T, int NS, int SM, int PF = kPf>
+using SState = typename std::conditional<SM == SM_JAC, JacState<T, NS, PF>, SweepState<T, NS, PF>>::type;
 
 // Apply half-sweeps SA..SB (default 1..S) at step J (row q = q0 + J):
 // half-sweep s updates row q - s, using level s-1 rows q-s-1, q-s, q-s+1.
 // On edge tiles nodes outside the domain interior keep their value.
-template <typename T, int NS, bool EQ, bool PW2, bool EDGE, bool ROWS, int J, int SA = 1, int SB = 2 * NS>
-__device__ __forceinline__ void s_sweeps(SweepState<T, NS> &st, const LevelView<T> &a, const Lane &L, int gi0, int q) {
-  constexpr int P = SweepState<T, NS>::P;
+template <typename T, int NS, int PF, bool EQ, bool PW2, bool EDGE, bool ROWS, int J, int SA = 1, int SB = 2 * NS>
+__device__ __forceinline__ void s_sweeps(SweepState<T, NS, PF> &st, const LevelView<T> &a, const Lane &L, int gi0,
+                                         int q) {
+  constexpr int P = SweepState<T, NS, PF>::P;
   static_for<SA, SB + 1>([&](auto sc) {
     constexpr int s = decltype(sc)::value;
     constexpr int p = J - s;  // phase of the row
@@ -363,9 +373,10 @@ __device__ __forceinline__ void s_sweeps(SweepState<T, NS> &st, const LevelView<
 // Weighted-Jacobi sweeps SA..SB at step J: sweep s updates both elements of
 // row q - s from level s-1 rows q-s-1, q-s, q-s+1, u + w (g - u) with g the
 // Gauss-Seidel point formula (the oracle's jacobi_sweep).
-template <typename T, int NS, bool EQ, bool PW2, bool EDGE, bool ROWS, int J, int SA, int SB>
-__device__ __forceinline__ void j_sweeps(JacState<T, NS> &st, const LevelView<T> &a, const Lane &L, int gi0, int q) {
-  constexpr int P = JacState<T, NS>::P;
+template <typename T, int NS, int PF, bool EQ, bool PW2, bool EDGE, bool ROWS, int J, int SA, int SB>
+__device__ __forceinline__ void j_sweeps(JacState<T, NS, PF> &st, const LevelView<T> &a, const Lane &L, int gi0,
+                                         int q) {
+  constexpr int P = JacState<T, NS, PF>::P;
   static_for<SA, SB + 1>([&](auto sc) {
     constexpr int s = decltype(sc)::value;
     constexpr int p = J - s;
@@ -396,17 +407,19 @@ __device__ __forceinline__ void j_sweeps(JacState<T, NS> &st, const LevelView<T>
 }
 
 // Sweep levels SA..SB of either smoother at step J.
-template <bool EQ, bool PW2, bool EDGE, bool ROWS, int J, int SA, int SB, typename T, int NS>
-__device__ __forceinline__ void sweeps(SweepState<T, NS> &st, const LevelView<T> &a, const Lane &L, int gi0, int q) {
-  s_sweeps<T, NS, EQ, PW2, EDGE, ROWS, J, SA, SB>(st, a, L, gi0, q);
+template <bool EQ, bool PW2, bool EDGE, bool ROWS, int J, int SA, int SB, typename T, int NS, int PF>
+__device__ __forceinline__ void sweeps(SweepState<T, NS, PF> &st, const LevelView<T> &a, const Lane &L, int gi0,
+                                       int q) {
+  s_sweeps<T, NS, PF, EQ, PW2, EDGE, ROWS, J, SA, SB>(st, a, L, gi0, q);
 }
-template <bool EQ, bool PW2, bool EDGE, bool ROWS, int J, int SA, int SB, typename T, int NS>
-__device__ __forceinline__ void sweeps(JacState<T, NS> &st, const LevelView<T> &a, const Lane &L, int gi0, int q) {
-  j_sweeps<T, NS, EQ, PW2, EDGE, ROWS, J, SA, SB>(st, a, L, gi0, q);
+template <bool EQ, bool PW2, bool EDGE, bool ROWS, int J, int SA, int SB, typename T, int NS, int PF>
+__device__ __forceinline__ void sweeps(JacState<T, NS, PF> &st, const LevelView<T> &a, const Lane &L, int gi0,
+                                       int q) {
+  j_sweeps<T, NS, PF, EQ, PW2, EDGE, ROWS, J, SA, SB>(st, a, L, gi0, q);
 }
 
-// Load the raw pair of staged row q + kPf (u unless zero-initialised) and
-// its right-hand side.
+// Load the raw pair of staged row q + PF (u unless zero-initialised) and
+// its right-hand side (PF: the prefetch distance of the state St).
 // ROWS = false: every row this loop body touches is an interior row of the
 // domain, so the load needs no predicate: lanes outside the row allocation
 // read its columns 0..1 (L.boff = 0), values that reach no interior node.
@@ -420,8 +433,8 @@ __device__ __forceinline__ void sweeps(JacState<T, NS> &st, const LevelView<T> &
 // 177 against 218 us, the FP64 level-1 ascent 70 against 76 us).
 template <bool ZI, bool EDGE, bool ROWS, int J, bool FBR, typename St, typename T>
 __device__ __forceinline__ void s_prefetch(St &st, const LevelView<T> &a, const Lane &L, int gi0, int q, int LR) {
-  constexpr int P = St::P;
-  const int qq = q + kPf, gi = gi0 + qq;
+  constexpr int P = St::P, PF = St::PF, NRU = St::NRU;
+  const int qq = q + PF, gi = gi0 + qq;
   Pair<T> u{T(0), T(0)}, f{T(0), T(0)};
   if constexpr (ROWS) {
     bool go = qq < LR && gi >= 0 && gi <= a.nx - 1;
@@ -442,10 +455,10 @@ __device__ __forceinline__ void s_prefetch(St &st, const LevelView<T> &a, const 
       f = ld_pair(row_of(a.f, gi, a.fpitch), L.boff);
     }
   }
-  st.RU[md<kNru>(J + kPf)][0] = u.x0;
-  st.RU[md<kNru>(J + kPf)][1] = u.x1;
-  st.F[md<P>(J + kPf)][0] = f.x0;
-  st.F[md<P>(J + kPf)][1] = f.x1;
+  st.RU[md<NRU>(J + PF)][0] = u.x0;
+  st.RU[md<NRU>(J + PF)][1] = u.x1;
+  st.F[md<P>(J + PF)][0] = f.x0;
+  st.F[md<P>(J + PF)][1] = f.x1;
 }
 
 // Tile of this workgroup: output rows [ti R, ti R + R), columns
@@ -497,23 +510,23 @@ __device__ __forceinline__ void s_idle(int LR) {
 constexpr int kR64D = VLP2D_R64D;  // steps between the FP64 loads of a row and its residual
 static_assert(kR64D >= 1 && kR64D <= 3, "FP64 prefetch distance of the residual descent");
 struct R64State {
-  double X[6][2];  // iterate rows, slot m % 6 (row m lives from step m - kPf - 1 - kR64D to m - kPf + 1)
-  double F[6][2];  // right-hand-side rows, slot m % 6 (step m - kPf - kR64D to m - kPf)
+  double X[6][2];  // iterate rows, slot m % 6 (row m lives from step m - PF - 1 - kR64D to m - PF + 1)
+  double F[6][2];  // right-hand-side rows, slot m % 6 (step m - PF - kR64D to m - PF)
   double acc;      // sum of r^2 over the tile's output nodes
 };
 
-// Step J (row q): loads iterate row q + kPf + 1 + kR64D and right-hand-side
-// row q + kPf + kR64D, then (J >= -kPf) evaluates the residual of staged row
-// m = q + kPf into st.F.  The pre-loop runs J = -kPf - 2 - kR64D .. -1 with
-// q = J.
-template <typename T, int NS, bool EDGE, bool ROWS, bool PW2, int J>
-__device__ __forceinline__ void r64_stage(SweepState<T, NS> &st, R64State &rs, const LevelView<T> &a,
+// Step J (row q): loads iterate row q + PF + 1 + kR64D and right-hand-side
+// row q + PF + kR64D, then (J >= -PF) evaluates the residual of staged row
+// m = q + PF into st.F (PF: the state's prefetch distance).  The pre-loop
+// runs J = -PF - 2 - kR64D .. -1 with q = J.
+template <typename T, int NS, int PF, bool EDGE, bool ROWS, bool PW2, int J>
+__device__ __forceinline__ void r64_stage(SweepState<T, NS, PF> &st, R64State &rs, const LevelView<T> &a,
                                           const MixView &mx, const Lane &L, unsigned boff8, int gi0, int q, int R,
                                           int H) {
-  constexpr int P = SweepState<T, NS>::P;
+  constexpr int P = SweepState<T, NS, PF>::P;
   {
-    const int mu = q + kPf + 1 + kR64D, gu = gi0 + mu;
-    const int mf = q + kPf + kR64D, gf = gi0 + mf;
+    const int mu = q + PF + 1 + kR64D, gu = gi0 + mu;
+    const int mf = q + PF + kR64D, gf = gi0 + mf;
     Pair<double> x{0.0, 0.0}, f{0.0, 0.0};
     if constexpr (ROWS) {
       bool gx = gu >= 0 && gu <= a.nx - 1, gff = mf >= 0 && gf >= 0 && gf <= a.nx - 1;
@@ -527,13 +540,13 @@ __device__ __forceinline__ void r64_stage(SweepState<T, NS> &st, R64State &rs, c
       x = ld_pair(row_of(mx.u64in, gu, a.pitch), boff8);
       f = ld_pair(row_of(mx.fn64, gf, a.pitch), boff8);
     }
-    rs.X[md<6>(J + kPf + 1 + kR64D)][0] = x.x0;
-    rs.X[md<6>(J + kPf + 1 + kR64D)][1] = x.x1;
-    rs.F[md<6>(J + kPf + kR64D)][0] = f.x0;
-    rs.F[md<6>(J + kPf + kR64D)][1] = f.x1;
+    rs.X[md<6>(J + PF + 1 + kR64D)][0] = x.x0;
+    rs.X[md<6>(J + PF + 1 + kR64D)][1] = x.x1;
+    rs.F[md<6>(J + PF + kR64D)][0] = f.x0;
+    rs.F[md<6>(J + PF + kR64D)][1] = f.x1;
   }
-  if constexpr (J + kPf >= 0) {
-    constexpr int jm = J + kPf;
+  if constexpr (J + PF >= 0) {
+    constexpr int jm = J + PF;
     const double u0 = rs.X[md<6>(jm)][0], u1 = rs.X[md<6>(jm)][1];
     const double a0 = rs.X[md<6>(jm - 1)][0], a1 = rs.X[md<6>(jm - 1)][1];
     const double c0 = rs.X[md<6>(jm + 1)][0], c1 = rs.X[md<6>(jm + 1)][1];
@@ -550,7 +563,7 @@ __device__ __forceinline__ void r64_stage(SweepState<T, NS> &st, R64State &rs, c
     const float r0f = (float)res0, r1f = (float)res1;
     st.F[md<P>(jm)][0] = r0f;
     st.F[md<P>(jm)][1] = r1f;
-    const int m = q + kPf, gm = gi0 + m;
+    const int m = q + PF, gm = gi0 + m;
     bool orow = m >= H && m < H + R;
     if constexpr (ROWS) orow = orow && gm >= 1 && gm <= a.nx - 2;
     if (orow && L.oc) {
@@ -570,16 +583,16 @@ __device__ __forceinline__ void r64_stage(SweepState<T, NS> &st, R64State &rs, c
 // ---------------------------------------------------------------------------
 // SM: smoother (SM_GS, SM_JAC); FW: full-weighting restriction (an addition,
 // (1/16)[1 2 1; 2 4 2; 1 2 1], the oracle's restrict_fw) instead of half
-// weighting.
+// weighting; PF: prefetch distance.
 template <typename T, int NS, bool ZI, bool STORE, bool EQ, bool PW2, bool EDGE, bool R64 = false, int SM = SM_GS,
-          bool FW = false>
+          bool FW = false, int PF = kPf>
 __device__ __forceinline__ void sdown_body(const LevelView<T> &a, const STile &tl, T *__restrict__ rhsc, int ncx,
                                            int ncy, int pc, const MixView &mx, double *__restrict__ part) {
   static_assert(!R64 || (ZI && !STORE && sizeof(T) == 4 && SM == SM_GS && !FW),
                 "the FP64-residual descent starts the FP32 correction");
   static_assert(SM == SM_GS || STORE, "a Jacobi-smoothed level is stored (the ascent recomputes Gauss-Seidel only)");
-  using St = SState<T, NS, SM>;
-  constexpr int S = St::S, H = s_down_halo(NS, SM), P = St::P, TJ = SCOLS - 2 * H;
+  using St = SState<T, NS, SM, PF>;
+  constexpr int S = St::S, H = s_down_halo(NS, SM), P = St::P, TJ = SCOLS - 2 * H, NRU = St::NRU;
   const int R = a.rows, LR = R + 2 * H;  // LR is a multiple of P
   const int gi0 = tl.gi0, cb = tl.cb;
   const Lane L = make_lane<T>(cb, H, TJ, a.ny, a.pitch);
@@ -596,15 +609,15 @@ __device__ __forceinline__ void sdown_body(const LevelView<T> &a, const STile &t
 #pragma unroll
     for (int k = 0; k < 6; ++k) rs.F[k][0] = rs.F[k][1] = 0.0;
     rs.acc = 0.0;
-    // iterate rows -1 .. kPf + kR64D, right-hand-side rows 0 .. kPf + kR64D - 1,
-    // F of rows 0 .. kPf - 1
-    static_for<-kPf - 2 - kR64D, 0>([&](auto jc) {
+    // iterate rows -1 .. PF + kR64D, right-hand-side rows 0 .. PF + kR64D - 1,
+    // F of rows 0 .. PF - 1
+    static_for<-PF - 2 - kR64D, 0>([&](auto jc) {
       constexpr int J = decltype(jc)::value;
-      r64_stage<T, NS, EDGE, true, PW2, J>(st, rs, a, mx, L, boff8, gi0, J, R, H);
+      r64_stage<T, NS, PF, EDGE, true, PW2, J>(st, rs, a, mx, L, boff8, gi0, J, R, H);
     });
   } else {
-    // rows 0 .. kPf - 1 (the loop prefetches q + kPf)
-    static_for<-kPf, 0>([&](auto jc) {
+    // rows 0 .. PF - 1 (the loop prefetches q + PF)
+    static_for<-PF, 0>([&](auto jc) {
       constexpr int J = decltype(jc)::value;
       s_prefetch<ZI, EDGE, true, J, true>(st, a, L, gi0, J, LR);
     });
@@ -614,18 +627,18 @@ __device__ __forceinline__ void sdown_body(const LevelView<T> &a, const STile &t
   T *const rc = rhsc + Jc;
 #pragma unroll 1
   for (int q0 = 0; q0 < LR; q0 += P) {
-    // rows q0 - S - 3 .. q0 + P + kPf + 4 cover everything this loop body
+    // rows q0 - S - 3 .. q0 + P + PF + 4 cover everything this loop body
     // reads or writes; when all are interior rows it runs without row checks
-    const bool rows_in = gi0 + q0 - (S + 3) >= 1 && gi0 + q0 + P + kPf + 4 <= a.nx - 2;
+    const bool rows_in = gi0 + q0 - (S + 3) >= 1 && gi0 + q0 + P + PF + 4 <= a.nx - 2;
     auto body = [&](auto rows_c) {
     constexpr bool ROWS = decltype(rows_c)::value;
     static_for<0, P>([&](auto jc) {
       constexpr int J = decltype(jc)::value;
       const int q = q0 + J;
       group_step();
-      if constexpr (R64) r64_stage<T, NS, EDGE, ROWS, PW2, J>(st, rs, a, mx, L, boff8, gi0, q, R, H);
+      if constexpr (R64) r64_stage<T, NS, PF, EDGE, ROWS, PW2, J>(st, rs, a, mx, L, boff8, gi0, q, R, H);
       else s_prefetch<ZI, EDGE, ROWS, J, true>(st, a, L, gi0, q, LR);
-      st.template stage<J>(st.RU[md<kNru>(J)][0], st.RU[md<kNru>(J)][1]);
+      st.template stage<J>(st.RU[md<NRU>(J)][0], st.RU[md<NRU>(J)][1]);
       sweeps<EQ, PW2, EDGE, ROWS, J, 1, S>(st, a, L, gi0, q);
       // store row q - S (unless the ascent recomputes it)
       if constexpr (STORE) {
@@ -681,8 +694,8 @@ __device__ __forceinline__ void sdown_body(const LevelView<T> &a, const STile &t
 }
 
 // STORE = false: the smoothed iterate is not written (the ascent recomputes
-// it, k_sup with N1 = NS).  SM, FW as sdown_body.
-template <typename T, int NS, bool ZI, bool EQ, bool STORE = true, int SM = SM_GS, bool FW = false>
+// it, k_sup with N1 = NS).  SM, FW, PF as sdown_body.
+template <typename T, int NS, bool ZI, bool EQ, bool STORE = true, int SM = SM_GS, bool FW = false, int PF = kPf>
 __global__ void __launch_bounds__(64 * kWaves) k_sdown(LevelView<T> a, T *__restrict__ rhsc, int ncx, int ncy, int pc) {
   constexpr int H = s_down_halo(NS, SM);
   STile tl;
@@ -692,9 +705,9 @@ __global__ void __launch_bounds__(64 * kWaves) k_sdown(LevelView<T> a, T *__rest
   }
   // every tile runs the column-predicated body
   if (a.pw2)
-    sdown_body<T, NS, ZI, STORE, EQ, true, true, false, SM, FW>(a, tl, rhsc, ncx, ncy, pc, MixView{}, nullptr);
+    sdown_body<T, NS, ZI, STORE, EQ, true, true, false, SM, FW, PF>(a, tl, rhsc, ncx, ncy, pc, MixView{}, nullptr);
   else
-    sdown_body<T, NS, ZI, STORE, EQ, false, true, false, SM, FW>(a, tl, rhsc, ncx, ncy, pc, MixView{}, nullptr);
+    sdown_body<T, NS, ZI, STORE, EQ, false, true, false, SM, FW, PF>(a, tl, rhsc, ncx, ncy, pc, MixView{}, nullptr);
 }
 
 // The first FP32 descent of a mixed-precision solve on level 0 (zero
@@ -703,7 +716,8 @@ __global__ void __launch_bounds__(64 * kWaves) k_sdown(LevelView<T> a, T *__rest
 // (float)(fn - Lap u64) into mx.r32out on interior nodes, the restricted
 // residual into rhsc, and one partial sum of r^2 per tile into part.  The
 // FP64 steps of mx are powers of two exactly when the FP32 steps of a are.
-template <int NS, bool EQ>
+// PF: prefetch distance.
+template <int NS, bool EQ, int PF = kPf>
 __global__ void __launch_bounds__(64 * kWaves) k_sdown_r64(LevelView<float> a, float *__restrict__ rhsc, int ncx,
                                                            int ncy, int pc, MixView mx, double *__restrict__ part) {
   constexpr int H = s_down_halo(NS);
@@ -712,14 +726,17 @@ __global__ void __launch_bounds__(64 * kWaves) k_sdown_r64(LevelView<float> a, f
     s_idle(a.rows + 2 * H);
     return;
   }
-  if (a.pw2) sdown_body<float, NS, true, false, EQ, true, true, true>(a, tl, rhsc, ncx, ncy, pc, mx, part);
-  else sdown_body<float, NS, true, false, EQ, false, true, true>(a, tl, rhsc, ncx, ncy, pc, mx, part);
+  if (a.pw2)
+    sdown_body<float, NS, true, false, EQ, true, true, true, SM_GS, false, PF>(a, tl, rhsc, ncx, ncy, pc, mx, part);
+  else
+    sdown_body<float, NS, true, false, EQ, false, true, true, SM_GS, false, PF>(a, tl, rhsc, ncx, ncy, pc, mx, part);
 }
 
 // ---------------------------------------------------------------------------
 // Ascent
 // ---------------------------------------------------------------------------
-template <typename T, int NS, int N1, int EPI, bool EQ, bool PW2, bool EDGE, bool ZS, int NB, int SM = SM_GS>
+template <typename T, int NS, int N1, int EPI, bool EQ, bool PW2, bool EDGE, bool ZS, int NB, int SM = SM_GS,
+          int PF = kPf>
 __device__ __forceinline__ void sup_body(const LevelView<T> &a, const STile &tl, const T *__restrict__ uc, int ncx,
                                          int ncy, int pc, double *__restrict__ part, const MixView &mx) {
   // N1 > 0: the level was pre-smoothed with N1 sweeps by the descent, which
@@ -731,8 +748,8 @@ __device__ __forceinline__ void sup_body(const LevelView<T> &a, const STile &tl,
   // next cycle's pre-smoothing (EPI 4)
   // (SM_JAC: NS Jacobi sweep levels, no recomputed or next-cycle sweeps)
   constexpr int NT = N1 + NS + NB, S1 = 2 * N1, ST = S1 + sweep_levels(NS, SM), SE = ST + 2 * NB;
-  using St = SState<T, NT, SM>;
-  constexpr int H = s_up_halo(NT, EPI, SM), P = St::P, TJ = SCOLS - 2 * H;
+  using St = SState<T, NT, SM, PF>;
+  constexpr int H = s_up_halo(NT, EPI, SM), P = St::P, TJ = SCOLS - 2 * H, NRU = St::NRU;
   static_assert(St::S == SE, "sweep levels of the state");
   static_assert(N1 == 0 || EPI != 2, "the update metric needs the staged iterate");
   static_assert((NB > 0) == (EPI == 4), "pre-sweeps of the next cycle belong to epilogue 4");
@@ -804,16 +821,16 @@ __device__ __forceinline__ void sup_body(const LevelView<T> &a, const STile &tl,
 #pragma unroll
   for (int k = 0; k < N4; ++k) RR[k][0] = RR[k][1] = T(0);
   const bool oc_c = EDGE ? L.oc && Jc >= 1 && Jc <= ncy - 2 : L.oc;
-  static_for<-kPf, 0>([&](auto jc) {
+  static_for<-PF, 0>([&](auto jc) {
     constexpr int J = decltype(jc)::value;
     s_prefetch<ZS, EDGE, true, J, FBR>(st, a, L, gi0, J, LR);
   });
 #pragma unroll 1
   for (int q0 = 0; q0 < LR; q0 += P) {
-    // rows q0 - SE - 3 .. q0 + P + kPf + 4 cover everything this loop body
+    // rows q0 - SE - 3 .. q0 + P + PF + 4 cover everything this loop body
     // reads or writes (coarse rows included); when all are interior rows it
     // runs without row checks
-    const bool rows_in = gi0 + q0 - (SE + 3) >= 1 && gi0 + q0 + P + kPf + 4 <= a.nx - 2;
+    const bool rows_in = gi0 + q0 - (SE + 3) >= 1 && gi0 + q0 + P + PF + 4 <= a.nx - 2;
     auto body = [&](auto rows_c) {
     constexpr bool ROWS = decltype(rows_c)::value;
     static_for<0, P>([&](auto jc) {
@@ -823,7 +840,7 @@ __device__ __forceinline__ void sup_body(const LevelView<T> &a, const STile &tl,
       s_prefetch<ZS, EDGE, ROWS, J, FBR>(st, a, L, gi0, q, LR);
       // staged row q (Gauss-Seidel: level -1 the red element, 0 the black)
       if constexpr (N1 > 0) {
-        st.template stage<J>(st.RU[md<kNru>(J)][0], st.RU[md<kNru>(J)][1]);
+        st.template stage<J>(st.RU[md<NRU>(J)][0], st.RU[md<NRU>(J)][1]);
         sweeps<EQ, PW2, EDGE, ROWS, J, 1, S1>(st, a, L, gi0, q);
       }
       // prolongation of row pr = q - S1, whose level S1 is now complete
@@ -858,7 +875,7 @@ __device__ __forceinline__ void sup_body(const LevelView<T> &a, const STile &tl,
           a1 = a1 && rin;
         }
         if constexpr (N1 == 0) {
-          T v0 = st.RU[md<kNru>(J)][0], v1 = st.RU[md<kNru>(J)][1];
+          T v0 = st.RU[md<NRU>(J)][0], v1 = st.RU[md<NRU>(J)][1];
           v0 = a0 ? v0 + p0 : v0;
           v1 = a1 ? v1 + p1 : v1;
           st.template stage<J>(v0, v1);
@@ -1074,8 +1091,8 @@ __device__ __forceinline__ void sup_body(const LevelView<T> &a, const STile &tl,
 // N1 > 0 pairs with k_sdown<T, N1, ZS, EQ, false> on the same level: ZS, the
 // descent started from zero; otherwise from a.uin, which the ascent then
 // stages (it may not write a.uin: only EPI 3, which writes no iterate).
-// SM: the smoother (SM_JAC with N1 = NB = 0 only).
-template <typename T, int NS, int EPI, bool EQ, int N1 = 0, bool ZS = true, int NB = 0, int SM = SM_GS>
+// SM: the smoother (SM_JAC with N1 = NB = 0 only); PF: prefetch distance.
+template <typename T, int NS, int EPI, bool EQ, int N1 = 0, bool ZS = true, int NB = 0, int SM = SM_GS, int PF = kPf>
 __global__ void __launch_bounds__(64 * kWaves) k_sup(LevelView<T> a, const T *__restrict__ uc, int ncx, int ncy, int pc,
                                             double *__restrict__ part, MixView mx) {
   constexpr int H = s_up_halo(N1 + NS + NB, EPI, SM);
@@ -1090,9 +1107,9 @@ __global__ void __launch_bounds__(64 * kWaves) k_sup(LevelView<T> a, const T *__
   // checks wherever all their rows are interior rows (measured faster than a
   // separate body for interior tiles, which also needs more registers).
   if (a.pw2) {
-    sup_body<T, NS, N1, EPI, EQ, true, true, Z, NB, SM>(a, tl, uc, ncx, ncy, pc, part, mx);
+    sup_body<T, NS, N1, EPI, EQ, true, true, Z, NB, SM, PF>(a, tl, uc, ncx, ncy, pc, part, mx);
   } else {
-    sup_body<T, NS, N1, EPI, EQ, false, true, Z, NB, SM>(a, tl, uc, ncx, ncy, pc, part, mx);
+    sup_body<T, NS, N1, EPI, EQ, false, true, Z, NB, SM, PF>(a, tl, uc, ncx, ncy, pc, part, mx);
   }
 }
 
