@@ -85,6 +85,10 @@ extern "C" {
 #define VLP2D_FLAG_STREAM_SMALL 64u /* levels below level 0 with <= 1025^2
                                      nodes ascend with the register-streaming
                                      kernel (default: the LDS-tiled one)    */
+#define VLP2D_FLAG_NO_STEP_FOLD 512u /* mixed solves: each refinement step's
+                                     first descent in a launch of its own
+                                     (default with one pre-sweep: folded into
+                                     the previous step's refinement ascent) */
 
 typedef struct vlp2d_mg_ctx *vlp2d_mg_handle;
 

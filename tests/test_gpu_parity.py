@@ -75,6 +75,7 @@ KERNEL_PATHS = {
     "store_presmooth": {"flags": _lib.FLAG_STORE_PRESMOOTH},  # ascent reloads the pre-smoothed level
     "stream_small": {"flags": _lib.FLAG_STREAM_SMALL},  # streaming ascent on levels <= 1025^2 (not k_lt_up)
     "no_cycle_fusion": {"flags": _lib.FLAG_NO_CYCLE_FUSION},  # two-cycle steps: level 0 of each cycle apart
+    "no_step_fold": {"flags": _lib.FLAG_NO_STEP_FOLD},  # each step's first descent in its own launch
     "one_op_per_launch": {"flags": _lib.FLAG_SIMPLE},
     "no_graph": {"flags": _lib.FLAG_NO_GRAPH},
     "graph_solve": {"flags": _lib.FLAG_GRAPH_SOLVE},  # the solve loop replays captured graphs
@@ -136,23 +137,28 @@ def test_recomputed_presmoothing_bitwise(seg_x, seg_y, hx, hy, laplace, mixed, n
     pre-smoothed iterate and the ascent recomputes it (streaming levels), or
     stores it for the LDS-tiled ascent (levels <= 1025^2); mixed solves also
     with two FP32 cycles per step, whose level-0 kernels between the cycles
-    run as one launch (ascent continued into the next descent).  Same
+    run as one launch (ascent continued into the next descent), and with the
+    next step's first descent folded into each step's refinement ascent
+    (one, two or three FP32 cycles per step: with three only a step the
+    schedule shortens to one or two cycles hands a descent on).  Same
     cycles and the same iterate, bit for bit, as the oracle, as the path
     that stores and reloads every level, as the streaming ascent on every
-    level and as separate level-0 launches.  V(1,1) is left out on the 128 x 256 grid with
+    level, as separate level-0 launches and as each step's first descent in
+    a launch of its own.  V(1,1) is left out on the 128 x 256 grid with
     steps e/128, pi/256 ((hx/hy)^2 = 3): point red-black smoothing diverges
     there (the oracle's residual grows by 1.76 per cycle)."""
     if nu2 == 1 and hx != hy:
         pytest.skip("V(1,1) diverges on the anisotropic grid (oracle included)")
     f, g = _random(seg_x, seg_y, 5 + seg_y, laplace)
     z = np.zeros_like(g) if laplace else f
-    for inner in ((1, 2) if mixed else (1,)):
+    for inner in ((1, 2, 3) if mixed else (1,)):
         _say(f"presmooth {seg_x}x{seg_y} mixed={mixed} nu=(1,{nu2}) inner={inner}")
         if mixed:
             ur, kr, _ = mixed_solve(seg_x, seg_y, hx, hy, z, g, 1, nu2, 3, 1e-10, 60, inner=inner)
         else:
             ur, kr, _ = plain_solve(seg_x, seg_y, hx, hy, z, g, 1, nu2, 3, 1e-10, 60)
-        for flags in (0, _lib.FLAG_STORE_PRESMOOTH, _lib.FLAG_STREAM_SMALL, _lib.FLAG_NO_CYCLE_FUSION):
+        for flags in (0, _lib.FLAG_STORE_PRESMOOTH, _lib.FLAG_STREAM_SMALL, _lib.FLAG_NO_CYCLE_FUSION,
+                      _lib.FLAG_NO_STEP_FOLD):
             s = MultiGridSchemeHIP(seg_x, seg_y, hx, hy, z,
                                    options=Options.performance(mixed, nu1=1, nu2=nu2, flags=flags, max_inner=inner))
             s.un0[:] = g

@@ -197,9 +197,11 @@ LevelView<T> sview(Ctx *c, const void *fn, const Lvl<T> &v, const T *uin, T *uou
 // the pre-smoothed iterate is written (else the ascent recomputes it).
 template <typename T, int NS> void launch_down_ns(Ctx *c, Lvl<T> &f, Lvl<T> &g, bool zi, bool store);
 // Ascent of level f with epilogue epi (Epi); rz: the pre-smoothed iterate is
-// recomputed; zi: f started from zero.  Returns the partials written.
+// recomputed; zi: f started from zero; nd = 1: the next refinement step's
+// first descent folded into the refinement ascent (its restriction into
+// g.rhs).  Returns the partials written.
 template <typename T, int NS>
-int launch_up_ns(Ctx *c, Lvl<T> &f, const Lvl<T> &g, int epi, bool rz, bool zi, const MixView &mx);
+int launch_up_ns(Ctx *c, Lvl<T> &f, const Lvl<T> &g, int epi, bool rz, bool zi, const MixView &mx, int nd);
 // Level 0 between the two FP32 cycles of a mixed step (k_sup epilogue 4;
 // nu1 pre-sweeps recomputed and run on the next cycle, 2 nu1 + NS <= 7).
 template <int NS> void launch_up_down0_ns(Ctx *c, Lvl<float> &f, Lvl<float> &g, int nu1);

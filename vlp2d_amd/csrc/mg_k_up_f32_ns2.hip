@@ -5,6 +5,6 @@
 namespace vlp2d {
 
 template void launch_up_down0_ns<2>(Ctx *, Lvl<float> &, Lvl<float> &, int);
-template int launch_up_ns<float, 2>(Ctx *, Lvl<float> &, const Lvl<float> &, int, bool, bool, const MixView &);
+template int launch_up_ns<float, 2>(Ctx *, Lvl<float> &, const Lvl<float> &, int, bool, bool, const MixView &, int);
 
 }  // namespace vlp2d

@@ -4,6 +4,6 @@
 
 namespace vlp2d {
 
-template int launch_up_ns<double, 1>(Ctx *, Lvl<double> &, const Lvl<double> &, int, bool, bool, const MixView &);
+template int launch_up_ns<double, 1>(Ctx *, Lvl<double> &, const Lvl<double> &, int, bool, bool, const MixView &, int);
 
 }  // namespace vlp2d

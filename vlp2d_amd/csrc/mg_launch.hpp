@@ -44,13 +44,17 @@ void launch_down_ns(Ctx *c, Lvl<T> &f, Lvl<T> &g, bool zi, bool store) {
   }
 }
 
-template <typename T, int NS, int EPI, int N1, bool ZS = true, int SM = SM_GS, int PF = kPf>
+// ND > 0: the next refinement step's first descent folded into the
+// refinement ascent, restricting into g's right-hand side.
+template <typename T, int NS, int EPI, int N1, bool ZS = true, int SM = SM_GS, int PF = kPf, int ND = 0>
 int launch_sup(Ctx *c, Lvl<T> &f, const Lvl<T> &g, const MixView &mx) {
-  constexpr int H = s_up_halo(N1 + NS, EPI, SM), P = s_period(N1 + NS, SM, PF);
+  constexpr int H = s_up_halo(N1 + NS, EPI, SM, ND), P = s_period(N1 + NS, SM, PF);
   using K = void (*)(LevelView<T>, const T *, int, int, int, double *, MixView);
-  const K k = f.eq ? (K)k_sup<T, NS, EPI, true, N1, ZS, 0, SM, PF> : (K)k_sup<T, NS, EPI, false, N1, ZS, 0, SM, PF>;
+  const K k = f.eq ? (K)k_sup<T, NS, EPI, true, N1, ZS, 0, SM, PF, ND>
+                   : (K)k_sup<T, NS, EPI, false, N1, ZS, 0, SM, PF, ND>;
   // a recomputed non-zero start is staged from the descent's input f.u
-  const LevelView<T> a = sview(c, (const void *)k, f, ZS ? f.u2 : f.u, f.u, H, P);
+  LevelView<T> a = sview(c, (const void *)k, f, ZS ? f.u2 : f.u, f.u, H, P);
+  if constexpr (ND > 0) a.rc = g.rhs;
   hipLaunchKernelGGL(k, dim3(8 * a.chunk), dim3(64 * kWaves), 0, c->st, a, g.u, g.nx, g.ny, g.pitch, c->part, mx);
   return a.tiles_i * a.tiles_j;
 }
@@ -58,14 +62,16 @@ int launch_sup(Ctx *c, Lvl<T> &f, const Lvl<T> &g, const MixView &mx) {
 // FP64 levels run epilogues none / residual norm / update norm; FP32 levels
 // (the correction cycle of a mixed step) none / fused refinement.  rz: the
 // level's pre-smoothed iterate is recomputed (one Gauss-Seidel pre-sweep; a
-// Jacobi-smoothed level is always stored, rz false).
+// Jacobi-smoothed level is always stored, rz false).  nd = 1 (FP32 refinement
+// ascents that start from zero or stage the stored pre-smoothed iterate, one
+// pre-sweep): the next step's first descent is folded in.
 // The Gauss-Seidel FP32 ascents without an epilogue on levels of 4097^2 up to
 // (not including) 8193^2 nodes prefetch three rows ahead instead of two:
 // measured on MI355X, the 4097^2 level 49 -> 43.5 us per launch, while 2049^2
 // went 28 -> 29 us (larger levels are bandwidth-bound and keep two).
 constexpr size_t kDeepPfMin = (size_t)4097 * 4097, kDeepPfEnd = (size_t)8193 * 8193;
 template <typename T, int NS>
-int launch_up_ns(Ctx *c, Lvl<T> &f, const Lvl<T> &g, int epi, bool rz, bool zi, const MixView &mx) {
+int launch_up_ns(Ctx *c, Lvl<T> &f, const Lvl<T> &g, int epi, bool rz, bool zi, const MixView &mx, int nd) {
   if (c->o.smoother == VLP2D_SMOOTH_WJACOBI) {
     if constexpr (sizeof(T) == 8) {
       switch (epi) {
@@ -87,6 +93,8 @@ int launch_up_ns(Ctx *c, Lvl<T> &f, const Lvl<T> &g, int epi, bool rz, bool zi, 
     }
   } else {
     if (epi == EPI_MIXED) {
+      if (nd == 1 && rz && zi) return launch_sup<T, NS, EPI_MIXED, 1, true, SM_GS, kPf, 1>(c, f, g, mx);
+      if (nd == 1 && !rz) return launch_sup<T, NS, EPI_MIXED, 0, true, SM_GS, kPf, 1>(c, f, g, mx);
       if (rz && !zi) return launch_sup<T, NS, EPI_MIXED, 1, false>(c, f, g, mx);
       return rz ? launch_sup<T, NS, EPI_MIXED, 1>(c, f, g, mx) : launch_sup<T, NS, EPI_MIXED, 0>(c, f, g, mx);
     }

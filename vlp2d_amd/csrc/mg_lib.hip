@@ -327,19 +327,21 @@ template <typename T> void down(Ctx *c, Lvl<T> &f, Lvl<T> &g, int lvl, bool zi, 
 }
 
 
-// returns the number of partials the epilogue writes
+// returns the number of partials the epilogue writes; nd = 1: the next
+// refinement step's first descent folded in (writes g.rhs)
 template <typename T>
-int up(Ctx *c, Lvl<T> &f, Lvl<T> &g, int lvl, int epi, bool rz, bool zi, const MixView &mx) {
+int up(Ctx *c, Lvl<T> &f, Lvl<T> &g, int lvl, int epi, bool rz, bool zi, const MixView &mx, int nd = 0) {
   const double s = sizeof(T);
   const double rin = rz && zi ? 0.0 : 1.0;  // the staged iterate, unless recomputed from zero
   double bytes = s * ((rin + 2.0) * nodes(f) + nodes(g));
   if (epi == EPI_MIXED) bytes = s * ((rin + 1.0) * nodes(f) + nodes(g)) + 28.0 * nodes(f);
+  if (nd) bytes += s * nodes(g);
   int np = 0;
   launch(c, epi == EPI_MIXED ? KC_UP_REFINE : KC_UP, lvl, bytes, [&] {
     switch (c->o.nu2) {
-      case 1: np = launch_up_ns<T, 1>(c, f, g, epi, rz, zi, mx); break;
-      case 2: np = launch_up_ns<T, 2>(c, f, g, epi, rz, zi, mx); break;
-      default: np = launch_up_ns<T, 3>(c, f, g, epi, rz, zi, mx); break;
+      case 1: np = launch_up_ns<T, 1>(c, f, g, epi, rz, zi, mx, nd); break;
+      case 2: np = launch_up_ns<T, 2>(c, f, g, epi, rz, zi, mx, nd); break;
+      default: np = launch_up_ns<T, 3>(c, f, g, epi, rz, zi, mx, nd); break;
     }
   });
   return np;
@@ -421,9 +423,11 @@ void down_r64(Ctx *c, Lvl<float> &f, Lvl<float> &g, const MixView &mx);
 // 1): level 0's pre-smoothed iterate is in its u2 (written by up_down0), and
 // the level-0 ascent stages it.  recomputed0 (to == 1): level 0's ascent
 // (up_down0) recomputes its pre-smoothing, so its descent does not store it.
+// nd0 = 1 (epi EPI_MIXED): the level-0 ascent also runs the next refinement
+// step's first descent into level 1's right-hand side (step_fold).
 template <typename T>
 int vcycle_fused(Ctx *c, Hier<T> &H, bool zero0, int epi, const MixView &mx, double *snap, int from = 0, int to = 0,
-                 const MixView *r64 = nullptr, bool stored0 = false, bool recomputed0 = false) {
+                 const MixView *r64 = nullptr, bool stored0 = false, bool recomputed0 = false, int nd0 = 0) {
   const int lc = H.lc;
   auto zi = [&](int l) { return l > 0 || zero0; };
   auto ep = [&](int l) { return l == 0 ? epi : (int)EPI_NONE; };
@@ -447,7 +451,7 @@ int vcycle_fused(Ctx *c, Hier<T> &H, bool zero0, int epi, const MixView &mx, dou
       up_lds(c, H.lv[l], H.lv[l + 1], l);
       continue;
     }
-    const int n = up(c, H.lv[l], H.lv[l + 1], l, ep(l), rz(l), zi(l), mx);
+    const int n = up(c, H.lv[l], H.lv[l + 1], l, ep(l), rz(l), zi(l), mx, l == 0 ? nd0 : 0);
     if (l == 0) np = n;
   }
   return np;
@@ -533,6 +537,27 @@ bool fused_first_residual(const Ctx *c, const Lvl<double> &P) {
          (recompute_presmooth(c, true, EPI_NONE) || fused_two_cycle(c, P, c->o.max_inner));
 }
 
+// Whether a mixed step's refinement ascent also runs the next step's first
+// FP32 descent (zero start, one pre-sweep not stored, restriction into level
+// 1's right-hand side), so that the next step starts at level 1: fused
+// refinement, Gauss-Seidel with one pre-sweep, half weighting, and a next
+// step whose level-0 ascent recomputes that pre-sweep (up_down0, or the
+// refinement ascent with one recomputed sweep).
+bool step_fold(const Ctx *c, const Lvl<double> &P) {
+  const Hier<float> &Z = c->h32;
+  if (c->simple || Z.lv.empty() || Z.lc == 0) return false;
+  const Lvl<float> &z = Z.lv[0];
+  return !(c->o.flags & (VLP2D_FLAG_SEPARATE_REFINE | VLP2D_FLAG_NO_STEP_FOLD)) && z.pw2 == P.pw2 &&
+         z.pitch == P.pitch && c->o.restriction == VLP2D_RESTRICT_HALF && recompute_presmooth(c, true, EPI_MIXED);
+}
+
+// Whether a step of m FP32 cycles folds the next step's first descent into
+// its refinement ascent: that ascent starts from zero (m = 1) or stages the
+// stored pre-smoothed iterate (the two-cycle step).
+bool step_folds_out(const Ctx *c, const Lvl<double> &P, int m) {
+  return step_fold(c, P) && (m == 1 || fused_two_cycle(c, P, m));
+}
+
 // max |u - snap| over interior nodes -> dscal[0]; pub: published for the
 // host (publish_wait)
 void maxdiff(Ctx *c, Lvl<double> &v, const double *snap, bool pub) {
@@ -587,7 +612,12 @@ void up_down0(Ctx *c, Lvl<float> &f, Lvl<float> &g) {
 // first: the first step of a solve; when fused_first_residual holds, its first
 // descent also computes the initial FP64 residual (r32 into z.rhs, ||r0||^2
 // into dscal[2]), which the caller then does not compute beforehand.
-void mixed_step(Ctx *c, Lvl<double> &P, int m = 1, bool pub = false, bool first = false) {
+// fold_in: the previous step's refinement ascent left this step's first
+// restriction in level 1's right-hand side (step_folds_out), so the step
+// starts at level 1.  fold_out: this step's refinement ascent does the same
+// for the next step when step_folds_out(c, P, m) holds.
+void mixed_step(Ctx *c, Lvl<double> &P, int m = 1, bool pub = false, bool first = false, bool fold_in = false,
+                bool fold_out = false) {
   Hier<float> &Z = c->h32;
   Lvl<float> &z = Z.lv[0];
   MixView none{};
@@ -604,24 +634,26 @@ void mixed_step(Ctx *c, Lvl<double> &P, int m = 1, bool pub = false, bool first 
   mx.part_max = c->part2;
   MixView rv = mx;
   rv.r32out = z.rhs;
-  const MixView *r64 = first && fused_first_residual(c, P) ? &rv : nullptr;
+  const MixView *r64 = first && !fold_in && fused_first_residual(c, P) ? &rv : nullptr;
   const bool fused_refine = !c->simple && Z.lc > 0 && !(c->o.flags & VLP2D_FLAG_SEPARATE_REFINE) &&
                             z.pw2 == P.pw2 && z.pitch == P.pitch;
+  const int from0 = fold_in ? 1 : 0;  // the first cycle's first descent level
+  const int nd = fold_out && step_folds_out(c, P, m) ? 1 : 0;
   if (fused_two_cycle(c, P, m)) {
     // first cycle down to level 1 (its level-0 pre-smoothing recomputed by
     // up_down0, not stored), level 0 of both cycles in between in one launch
     // (it stores the second cycle's pre-smoothed iterate in z.u2), second
     // cycle from level 1 with the refinement ascent staging it
-    vcycle_fused(c, Z, true, EPI_NONE, none, nullptr, 0, 1, r64, false, true);
+    vcycle_fused(c, Z, true, EPI_NONE, none, nullptr, from0, 1, r64, false, true);
     up_down0(c, z, Z.lv[1]);
-    const int np = vcycle_fused(c, Z, false, EPI_MIXED, mx, nullptr, 1, 0, nullptr, true);
+    const int np = vcycle_fused(c, Z, false, EPI_MIXED, mx, nullptr, 1, 0, nullptr, true, false, nd);
     finish(c, c->part2, np, c->part, np, pub);
     return;
   }
   // all cycles but the last (correction e -> z.u)
   for (int k = 0; k + 1 < m; ++k) {
     if (c->simple) vcycle_simple(c, Z, k == 0, nullptr);
-    else vcycle_fused(c, Z, k == 0, EPI_NONE, none, nullptr, 0, 0, k == 0 ? r64 : nullptr);
+    else vcycle_fused(c, Z, k == 0, EPI_NONE, none, nullptr, k == 0 ? from0 : 0, 0, k == 0 ? r64 : nullptr);
   }
   const bool zero_last = m == 1;
   if (c->simple || Z.lc == 0) {
@@ -640,7 +672,8 @@ void mixed_step(Ctx *c, Lvl<double> &P, int m = 1, bool pub = false, bool first 
     // refinement update inside the level-0 ascent of the FP32 cycle (the
     // kernel evaluates both precisions' divisions the same way, as
     // multiplications when the steps are powers of two)
-    const int np = vcycle_fused(c, Z, zero_last, EPI_MIXED, mx, nullptr, 0, 0, m == 1 ? r64 : nullptr);
+    const int np = vcycle_fused(c, Z, zero_last, EPI_MIXED, mx, nullptr, m == 1 ? from0 : 0, 0, m == 1 ? r64 : nullptr,
+                                false, false, nd);
     finish(c, c->part2, np, c->part, np, pub);
     return;
   }
@@ -696,6 +729,7 @@ template <typename F> int run_program(Ctx *c, int key, F &&body) {
 // graph keys: program id * 4 + swap parities of the buffers it captured
 enum Prog { PG_CYCLE_METRIC = 0, PG_CYCLE_NORM = 1, PG_MIXED = 2, PG_ADJ_CYCLE_NORM = 3, PG_ADJ_MIXED = 4 };
 constexpr int kProgFirst = 8;  // or-ed into a mixed program: the first step of a solve
+constexpr int kProgFold = 16;  // or-ed into a mixed program: the step starts at level 1 (fold_in)
 int gkey(int prog, int par, int rpar, int m = 1) { return ((prog * 16 + m) * 2 + par) * 2 + rpar; }
 
 int sync_read(Ctx *c, int nslots) {
@@ -941,16 +975,24 @@ int solve_mixed(Ctx *c, Lvl<double> &P, int prog_mixed, double rtol, int max_cyc
   };
   Step fl[2] = {};
   int nfl = 0;
+  // level 1's right-hand side holds the next step's first restriction (the
+  // last launched step folded it in); a dropped step may have overwritten
+  // it, so its relaunch starts at level 0
+  bool fold_ready = false;
   auto launch_step = [&](int m, bool first) -> int {
     first = first && fuse_r0;
-    RCHK(run_program(c, gkey(first ? prog_mixed | kProgFirst : prog_mixed, P.par, c->h32.lv[0].rpar, m) + kPubKey,
-                     [&] { mixed_step(c, P, m, true, first); }));
+    const bool fin = fold_ready;
+    const int prog = (first ? prog_mixed | kProgFirst : prog_mixed) | (fin ? kProgFold : 0);
+    RCHK(run_program(c, gkey(prog, P.par, c->h32.lv[0].rpar, m) + kPubKey,
+                     [&] { mixed_step(c, P, m, true, first, fin, true); }));
+    fold_ready = step_folds_out(c, P, m);
     swap_mixed(c, P);
     fl[nfl++] = Step{m, ++c->pub_seq};
     return VLP2D_OK;
   };
   auto drop = [&](int n) {
     for (int q = 0; q < n; ++q) swap_mixed(c, P);
+    fold_ready = false;
   };
   k = 0;
   vcycles = 0;
@@ -1040,6 +1082,7 @@ int solve_impl(Ctx *c, Hier<double> &H, int prog_fp64, int prog_mixed, double rt
     rel = r0 > 0 ? 1.0 : 0.0;
     double rel_prev = 1.0;
     int m_prev = 1;
+    bool fold_ready = false;  // as in solve_mixed
     while (rc == VLP2D_OK && r0 > 0 && rel > rtol) {
       if (k >= max_cycles) {
         rc = fail(c, VLP2D_E_NOTCONV, "max_cycles reached before rtol");
@@ -1052,8 +1095,11 @@ int solve_impl(Ctx *c, Hier<double> &H, int prog_fp64, int prog_mixed, double rt
       const bool first = fuse_r0 && k == 0;
       if (mixed) {
         const int m = k == 0 ? c->o.max_inner : inner_cycles(rel, rel_prev, m_prev, c->o.max_inner, rtol);
-        RCHK(run_program(c, gkey(first ? prog_mixed | kProgFirst : prog_mixed, P.par, c->h32.lv[0].rpar, m) + kPubKey,
-                         [&] { mixed_step(c, P, m, true, first); }));
+        const bool fin = fold_ready;
+        const int prog = (first ? prog_mixed | kProgFirst : prog_mixed) | (fin ? kProgFold : 0);
+        RCHK(run_program(c, gkey(prog, P.par, c->h32.lv[0].rpar, m) + kPubKey,
+                         [&] { mixed_step(c, P, m, true, first, fin, true); }));
+        fold_ready = step_folds_out(c, P, m);
         swap_mixed(c, P);
         vcycles += m;
         m_prev = m;

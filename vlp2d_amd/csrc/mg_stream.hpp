@@ -120,9 +120,16 @@ __host__ __device__ constexpr int s_period(int ns, int sm = SM_GS, int pf = kPf)
                                                : 24;
 }
 __host__ __device__ constexpr int s_down_halo(int ns, int sm = SM_GS) { return even_up(sweep_levels(ns, sm) + 2); }
-__host__ __device__ constexpr int s_up_halo(int ns, int epi, int sm = SM_GS) {
-  return (epi == 1 || epi == 3 || epi == 4) ? even_up(sweep_levels(ns, sm) + 1)
-                                            : (sweep_levels(ns, sm) < 2 ? 2 : even_up(sweep_levels(ns, sm)));
+// nd > 0 (epilogue 3 only): the ascent also runs the next refinement step's
+// first descent (nd pre-sweeps from zero, residual, restriction) on the
+// FP32 residual it produces, which needs that residual 2 nd + 1 nodes
+// around the tile's output region, hence the FP64 iterate one node further
+// and the final correction there: 2 nd + 2 more levels of halo.
+__host__ __device__ constexpr int s_up_halo(int ns, int epi, int sm = SM_GS, int nd = 0) {
+  return (epi == 3 && nd > 0) ? even_up(sweep_levels(ns, sm) + 2 * nd + 2)
+         : (epi == 1 || epi == 3 || epi == 4)
+             ? even_up(sweep_levels(ns, sm) + 1)
+             : (sweep_levels(ns, sm) < 2 ? 2 : even_up(sweep_levels(ns, sm)));
 }
 // Output rows per tile R (a launch parameter): the staged rows R + 2H must
 // be a whole number of loop bodies; the smallest such R >= want.
@@ -736,7 +743,7 @@ __global__ void __launch_bounds__(64 * kWaves) k_sdown_r64(LevelView<float> a, f
 // Ascent
 // ---------------------------------------------------------------------------
 template <typename T, int NS, int N1, int EPI, bool EQ, bool PW2, bool EDGE, bool ZS, int NB, int SM = SM_GS,
-          int PF = kPf>
+          int PF = kPf, int ND = 0>
 __device__ __forceinline__ void sup_body(const LevelView<T> &a, const STile &tl, const T *__restrict__ uc, int ncx,
                                          int ncy, int pc, double *__restrict__ part, const MixView &mx) {
   // N1 > 0: the level was pre-smoothed with N1 sweeps by the descent, which
@@ -747,9 +754,21 @@ __device__ __forceinline__ void sup_body(const LevelView<T> &a, const STile &tl,
   // half-sweeps: 1..S1 recompute, S1+1..ST post-smoothing, ST+1..SE the
   // next cycle's pre-smoothing (EPI 4)
   // (SM_JAC: NS Jacobi sweep levels, no recomputed or next-cycle sweeps)
+  // ND > 0 (EPI 3): the next refinement step's first descent runs on the
+  // FP32 residual rows this ascent produces, D = ST + 1 steps behind the
+  // staged row: ND Gauss-Seidel sweeps from zero (state S2), residual and
+  // half-weighting restriction into a.rc, the coarse level's right-hand
+  // side (as k_sdown<T, ND, ZI, EQ, false> on that residual, the same
+  // operations in the same order)
   constexpr int NT = N1 + NS + NB, S1 = 2 * N1, ST = S1 + sweep_levels(NS, SM), SE = ST + 2 * NB;
   using St = SState<T, NT, SM, PF>;
-  constexpr int H = s_up_halo(NT, EPI, SM), P = St::P, TJ = SCOLS - 2 * H, NRU = St::NRU;
+  constexpr int H = s_up_halo(NT, EPI, SM, ND), P = St::P, TJ = SCOLS - 2 * H, NRU = St::NRU;
+  static_assert(ND == 0 || (EPI == 3 && SM == SM_GS), "the next step's descent follows the refinement epilogue");
+  // residual radius around the output region the folded descent reads
+  constexpr int RX = ND > 0 ? 2 * ND + 1 : 0, D = ST + 1, SD = 2 * ND;
+  // (it prefetches nothing: its ring only needs the rows in flight, 1 step)
+  using St2 = SweepState<T, (ND > 0 ? ND : 1), 1>;
+  static_assert(ND == 0 || P % St2::P == 0, "folded descent ring period");
   static_assert(St::S == SE, "sweep levels of the state");
   static_assert(N1 == 0 || EPI != 2, "the update metric needs the staged iterate");
   static_assert((NB > 0) == (EPI == 4), "pre-sweeps of the next cycle belong to epilogue 4");
@@ -808,10 +827,12 @@ __device__ __forceinline__ void sup_body(const LevelView<T> &a, const STile &tl,
 #pragma unroll
   for (int k = 0; k < N3; ++k) X64[k][0] = X64[k][1] = XR[k][0] = XR[k][1] = FN[k][0] = FN[k][1] = 0.0;
   // EPI 3's FP64 rows: only lanes H/2 - 1 .. H/2 + TJ/2 (the output columns
-  // and the neighbour column on each side, which the FP64 residual reads)
-  // need their values; the others load a needed lane's pair instead of their
-  // own halo columns (addresses already requested: no extra traffic)
-  const int m64 = L.m < H / 2 - 1 ? H / 2 - 1 : (L.m > H / 2 + TJ / 2 ? H / 2 + TJ / 2 : L.m);
+  // and the neighbour column on each side, which the FP64 residual reads;
+  // RX more columns on each side with ND) need their values; the others load
+  // a needed lane's pair instead of their own halo columns (addresses
+  // already requested: no extra traffic)
+  constexpr int m64lo = H / 2 - (RX + 2) / 2, m64hi = H / 2 + TJ / 2 + RX / 2;
+  const int m64 = L.m < m64lo ? m64lo : (L.m > m64hi ? m64hi : L.m);
   const int gj64 = cb + 2 * m64;
   const unsigned boff8 = gj64 >= 0 && gj64 <= a.pitch - 2 ? (unsigned)gj64 * 8u : 0u;
   // EPI 4: residual of rows (slot p % 3; element 1 only on even rows) and
@@ -821,16 +842,24 @@ __device__ __forceinline__ void sup_body(const LevelView<T> &a, const STile &tl,
 #pragma unroll
   for (int k = 0; k < N4; ++k) RR[k][0] = RR[k][1] = T(0);
   const bool oc_c = EDGE ? L.oc && Jc >= 1 && Jc <= ncy - 2 : L.oc;
+  // ND: the folded descent's state (staged zeros, right-hand side = the FP32
+  // residual of row q - D) and its residual rows (slot p % 3)
+  St2 s2;
+  s2.zero();
+  T R2[3][2];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) R2[k][0] = R2[k][1] = T(0);
   static_for<-PF, 0>([&](auto jc) {
     constexpr int J = decltype(jc)::value;
     s_prefetch<ZS, EDGE, true, J, FBR>(st, a, L, gi0, J, LR);
   });
 #pragma unroll 1
   for (int q0 = 0; q0 < LR; q0 += P) {
-    // rows q0 - SE - 3 .. q0 + P + PF + 4 cover everything this loop body
-    // reads or writes (coarse rows included); when all are interior rows it
-    // runs without row checks
-    const bool rows_in = gi0 + q0 - (SE + 3) >= 1 && gi0 + q0 + P + PF + 4 <= a.nx - 2;
+    // rows q0 - SL .. q0 + P + PF + 4 cover everything this loop body reads
+    // or writes (coarse rows included); when all are interior rows it runs
+    // without row checks
+    constexpr int SL = ND > 0 ? D + SD + 3 : SE + 3;
+    const bool rows_in = gi0 + q0 - SL >= 1 && gi0 + q0 + P + PF + 4 <= a.nx - 2;
     auto body = [&](auto rows_c) {
     constexpr bool ROWS = decltype(rows_c)::value;
     static_for<0, P>([&](auto jc) {
@@ -898,13 +927,14 @@ __device__ __forceinline__ void sup_body(const LevelView<T> &a, const STile &tl,
       }
       if constexpr (EPI == 3) {
         // prefetch FP64 rows: iterate row q - ST + 2, right-hand side row q - ST + 1
-        // only the rows the FP64 residual of the output rows [H, H + R) reads:
-        // iterate rows H - 1 .. H + R, right-hand-side rows H .. H + R - 1
-        // (the slots of the other rows hold zeros no output uses)
+        // only the rows the FP64 residual of the output rows [H, H + R) reads
+        // (RX more on each side with ND): iterate rows H - 1 - RX .. H + R + RX,
+        // right-hand-side rows H - RX .. H + R - 1 + RX (the slots of the other
+        // rows hold zeros no output uses)
         const int r = q - ST + 2, gr = gi0 + r;
         Pair<double> x{0.0, 0.0}, f{0.0, 0.0};
         const int rf = q - ST + 1, gf = gi0 + rf;
-        const bool need_x = r >= H - 1 && r <= H + R, need_f = rf >= H && rf < H + R;
+        const bool need_x = r >= H - 1 - RX && r <= H + R + RX, need_f = rf >= H - RX && rf < H + R + RX;
         if constexpr (ROWS) {
           bool go = need_x && gr >= 0 && gr <= a.nx - 1;
           if constexpr (EDGE) go = go && L.ld;
@@ -1017,9 +1047,8 @@ __device__ __forceinline__ void sup_body(const LevelView<T> &a, const STile &tl,
         const T e0 = st.template full<ST, J, d, 0>(), e1 = st.template full<ST, J, d, 1>();
         bool orow = p >= H && p < H + R;
         if constexpr (ROWS) orow = orow && gi >= 1 && gi <= a.nx - 2;
-        if (orow && L.oc) {
+        auto fp64_residual = [&](double &res0, double &res1) {
           const double u20 = x0 * 2.0, u21 = x1 * 2.0;
-          double res0, res1;
           if constexpr (PW2) {
             res0 = FN[md<3>(pr)][0] - ((a0 + b0 - u20) * mx.ihx2 + (xl + x1 - u20) * mx.ihy2);
             res1 = FN[md<3>(pr)][1] - ((a1 + b1 - u21) * mx.ihx2 + (x0 + xr - u21) * mx.ihy2);
@@ -1027,6 +1056,18 @@ __device__ __forceinline__ void sup_body(const LevelView<T> &a, const STile &tl,
             res0 = FN[md<3>(pr)][0] - ((a0 + b0 - u20) / mx.hx2 + (xl + x1 - u20) / mx.hy2);
             res1 = FN[md<3>(pr)][1] - ((a1 + b1 - u21) / mx.hx2 + (x0 + xr - u21) / mx.hy2);
           }
+        };
+        double res0 = 0.0, res1 = 0.0;
+        if constexpr (ND > 0) {
+          // every lane: the folded descent reads the residual around the
+          // output region too (values of lanes and rows it does not read are
+          // never used)
+          fp64_residual(res0, res1);
+          s2.F[md<St2::P>(pr)][0] = (float)res0;
+          s2.F[md<St2::P>(pr)][1] = (float)res1;
+        }
+        if (orow && L.oc) {
+          if constexpr (ND == 0) fp64_residual(res0, res1);
           const bool o0 = !EDGE || L.in0, o1 = !EDGE || L.in1;
           double *urow = row_of(mx.u64out, gi, a.pitch);
           float *rrow = row_of(mx.r32out, gi, a.pitch);
@@ -1047,6 +1088,39 @@ __device__ __forceinline__ void sup_body(const LevelView<T> &a, const STile &tl,
           if (o1) {
             acc += res1 * res1;
             emax = nanmax(emax, fabs((double)e1));
+          }
+        }
+      }
+      // the next step's first descent (ND): row q - D was just staged in s2
+      // (its right-hand side above, zeros as its start); sweeps, residual of
+      // row q - D - SD - 1, restriction at even row q - D - SD - 2
+      if constexpr (ND > 0) {
+        constexpr int J2 = J - D;
+        const int q2 = q - D;
+        sweeps<EQ, PW2, EDGE, ROWS, J2, 1, SD>(s2, a, L, gi0, q2);
+        {
+          constexpr int d = SD + 1;
+          constexpr int pr = J2 - d;
+          const T f0 = s2.F[md<St2::P>(pr)][0], f1 = s2.F[md<St2::P>(pr)][1];
+          const T x0 = s2.template full<SD, J2, d, 0>(), x1 = s2.template full<SD, J2, d, 1>();
+          const T a0 = s2.template full<SD, J2, d + 1, 0>(), b0 = s2.template full<SD, J2, d - 1, 0>();
+          R2[md<3>(pr)][0] = s_res<T, PW2>(x0, a0, b0, from_left(x1), x1, f0, a);
+          if constexpr (md<2>(pr) == 0) {
+            const T a1 = s2.template full<SD, J2, d + 1, 1>(), b1 = s2.template full<SD, J2, d - 1, 1>();
+            R2[md<3>(pr)][1] = s_res<T, PW2>(x1, a1, b1, x0, from_right(x0), f1, a);
+          }
+        }
+        {
+          constexpr int px = J2 - SD - 2;
+          if constexpr (md<2>(px) == 0) {
+            const int p = q2 - SD - 2;
+            const T r0 = R2[md<3>(px)][0], rjp = R2[md<3>(px)][1];
+            const T rjm = from_left(rjp);
+            const T rim = R2[md<3>(px - 1)][0], rip = R2[md<3>(px + 1)][0];
+            const int I = (gi0 + p) / 2;
+            bool go = p >= H && p < H + R;
+            if constexpr (ROWS) go = go && I >= 1 && I <= ncx - 2;
+            if (go && oc_c) a.rc[I * pc + Jc] = T(0.125) * (r0 * T(4) + (rip + rim + rjp + rjm));
           }
         }
       }
@@ -1091,11 +1165,13 @@ __device__ __forceinline__ void sup_body(const LevelView<T> &a, const STile &tl,
 // N1 > 0 pairs with k_sdown<T, N1, ZS, EQ, false> on the same level: ZS, the
 // descent started from zero; otherwise from a.uin, which the ascent then
 // stages (it may not write a.uin: only EPI 3, which writes no iterate).
-// SM: the smoother (SM_JAC with N1 = NB = 0 only); PF: prefetch distance.
-template <typename T, int NS, int EPI, bool EQ, int N1 = 0, bool ZS = true, int NB = 0, int SM = SM_GS, int PF = kPf>
+// SM: the smoother (SM_JAC with N1 = NB = 0 only); PF: prefetch distance;
+// ND: the next step's first descent folded in (EPI 3, restriction into a.rc).
+template <typename T, int NS, int EPI, bool EQ, int N1 = 0, bool ZS = true, int NB = 0, int SM = SM_GS, int PF = kPf,
+          int ND = 0>
 __global__ void __launch_bounds__(64 * kWaves) k_sup(LevelView<T> a, const T *__restrict__ uc, int ncx, int ncy, int pc,
                                             double *__restrict__ part, MixView mx) {
-  constexpr int H = s_up_halo(N1 + NS + NB, EPI, SM);
+  constexpr int H = s_up_halo(N1 + NS + NB, EPI, SM, ND);
   constexpr bool Z = ZS && N1 > 0;
   static_assert(ZS || (N1 > 0 && EPI == 3), "a recomputed non-zero start is read back in place: EPI 3 only");
   STile tl;
@@ -1107,9 +1183,9 @@ __global__ void __launch_bounds__(64 * kWaves) k_sup(LevelView<T> a, const T *__
   // checks wherever all their rows are interior rows (measured faster than a
   // separate body for interior tiles, which also needs more registers).
   if (a.pw2) {
-    sup_body<T, NS, N1, EPI, EQ, true, true, Z, NB, SM, PF>(a, tl, uc, ncx, ncy, pc, part, mx);
+    sup_body<T, NS, N1, EPI, EQ, true, true, Z, NB, SM, PF, ND>(a, tl, uc, ncx, ncy, pc, part, mx);
   } else {
-    sup_body<T, NS, N1, EPI, EQ, false, true, Z, NB, SM, PF>(a, tl, uc, ncx, ncy, pc, part, mx);
+    sup_body<T, NS, N1, EPI, EQ, false, true, Z, NB, SM, PF, ND>(a, tl, uc, ncx, ncy, pc, part, mx);
   }
 }
 
