@@ -729,7 +729,8 @@ template <typename F> int run_program(Ctx *c, int key, F &&body) {
 // graph keys: program id * 4 + swap parities of the buffers it captured
 enum Prog { PG_CYCLE_METRIC = 0, PG_CYCLE_NORM = 1, PG_MIXED = 2, PG_ADJ_CYCLE_NORM = 3, PG_ADJ_MIXED = 4 };
 constexpr int kProgFirst = 8;  // or-ed into a mixed program: the first step of a solve
-constexpr int kProgFold = 16;  // or-ed into a mixed program: the step starts at level 1 (fold_in)
+constexpr int kProgFold = 16;     // or-ed into a mixed program: the step starts at level 1 (fold_in)
+constexpr int kProgFoldOut = 32;  // or-ed into a mixed program: the step folds the next one's first descent
 int gkey(int prog, int par, int rpar, int m = 1) { return ((prog * 16 + m) * 2 + par) * 2 + rpar; }
 
 int sync_read(Ctx *c, int nslots) {
@@ -953,6 +954,17 @@ bool speculate_next(double rel, double rel_prev, int m_prev, bool have_rate, int
   return rel * std::pow(rho, m_flight + max_inner - 1) > 100.0 * rtol;
 }
 
+// Whether a step of m cycles launched after m_before cycles still in flight is
+// predicted to reach rtol (the last step of the solve), from the contraction
+// per cycle of the last accepted step: such a step does not fold the next
+// step's first descent into its refinement ascent.  Only a prediction: when
+// another step turns out to be needed, it starts at level 0.
+bool predicted_last(double rel, double rel_prev, int m_prev, bool have_rate, int m_before, int m, double rtol) {
+  if (!have_rate) return false;
+  const double rho = std::pow(rel / rel_prev, 1.0 / m_prev);
+  return rho < 1.0 && rel * std::pow(rho, m_before + m) <= rtol;
+}
+
 // Mixed solve: refinement steps are launched one ahead of the host while the
 // next one is certain to be needed (speculate_next), so the host's wait for a
 // step's result overlaps the next step.  A step writes the iterate and the
@@ -979,13 +991,18 @@ int solve_mixed(Ctx *c, Lvl<double> &P, int prog_mixed, double rtol, int max_cyc
   // last launched step folded it in); a dropped step may have overwritten
   // it, so its relaunch starts at level 0
   bool fold_ready = false;
+  // (rel, rel_prev, m_prev: the loop's state below, read when a step is launched)
+  double rel_prev = 1.0;
+  int m_prev = 1;
   auto launch_step = [&](int m, bool first) -> int {
     first = first && fuse_r0;
     const bool fin = fold_ready;
-    const int prog = (first ? prog_mixed | kProgFirst : prog_mixed) | (fin ? kProgFold : 0);
+    const bool fout = step_folds_out(c, P, m) &&
+                      !predicted_last(rel, rel_prev, m_prev, k > 0, nfl ? fl[0].m : 0, m, rtol);
+    const int prog = (first ? prog_mixed | kProgFirst : prog_mixed) | (fin ? kProgFold : 0) | (fout ? kProgFoldOut : 0);
     RCHK(run_program(c, gkey(prog, P.par, c->h32.lv[0].rpar, m) + kPubKey,
-                     [&] { mixed_step(c, P, m, true, first, fin, true); }));
-    fold_ready = step_folds_out(c, P, m);
+                     [&] { mixed_step(c, P, m, true, first, fin, fout); }));
+    fold_ready = fout;
     swap_mixed(c, P);
     fl[nfl++] = Step{m, ++c->pub_seq};
     return VLP2D_OK;
@@ -998,8 +1015,6 @@ int solve_mixed(Ctx *c, Lvl<double> &P, int prog_mixed, double rtol, int max_cyc
   vcycles = 0;
   rel = 1.0;
   r0 = NAN;
-  double rel_prev = 1.0;
-  int m_prev = 1;
   for (;;) {
     if (c->cancel.load()) {
       drop(nfl);
@@ -1096,10 +1111,12 @@ int solve_impl(Ctx *c, Hier<double> &H, int prog_fp64, int prog_mixed, double rt
       if (mixed) {
         const int m = k == 0 ? c->o.max_inner : inner_cycles(rel, rel_prev, m_prev, c->o.max_inner, rtol);
         const bool fin = fold_ready;
-        const int prog = (first ? prog_mixed | kProgFirst : prog_mixed) | (fin ? kProgFold : 0);
+        const bool fout = step_folds_out(c, P, m) && !predicted_last(rel, rel_prev, m_prev, k > 0, 0, m, rtol);
+        const int prog =
+            (first ? prog_mixed | kProgFirst : prog_mixed) | (fin ? kProgFold : 0) | (fout ? kProgFoldOut : 0);
         RCHK(run_program(c, gkey(prog, P.par, c->h32.lv[0].rpar, m) + kPubKey,
-                         [&] { mixed_step(c, P, m, true, first, fin, true); }));
-        fold_ready = step_folds_out(c, P, m);
+                         [&] { mixed_step(c, P, m, true, first, fin, fout); }));
+        fold_ready = fout;
         swap_mixed(c, P);
         vcycles += m;
         m_prev = m;
