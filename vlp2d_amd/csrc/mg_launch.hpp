@@ -7,26 +7,36 @@
 
 namespace vlp2d {
 
+// The Gauss-Seidel FP32 ascents without an epilogue on levels of 4097^2 up to
+// (not including) 8193^2 nodes prefetch three rows ahead instead of two:
+// measured on MI355X, the 4097^2 level 49 -> 43.5 us per launch, while 2049^2
+// went 28 -> 29 us (larger levels are bandwidth-bound and keep two).  The FP32
+// Gauss-Seidel descents below 8193^2 do too (launch_down_ns).
+constexpr size_t kDeepPfMin = (size_t)4097 * 4097, kDeepPfEnd = (size_t)8193 * 8193;
+
 // Descent of level f into g's right-hand side (zi: f starts from zero;
-// store: the pre-smoothed iterate is written to f.u2).
-template <typename T, int NS, int SM, bool FW>
+// store: the pre-smoothed iterate is written to f.u2); PF: prefetch distance.
+template <typename T, int NS, int SM, bool FW, int PF = kPf>
 void launch_down_k(Ctx *c, Lvl<T> &f, Lvl<T> &g, bool zi, bool store) {
-  constexpr int H = s_down_halo(NS, SM), P = s_period(NS, SM);
+  constexpr int H = s_down_halo(NS, SM), P = s_period(NS, SM, PF);
   using K = void (*)(LevelView<T>, T *, int, int, int);
   K k;
   if constexpr (SM == SM_JAC) {
     // a Jacobi-smoothed level is always stored (the host never asks otherwise)
     (void)store;
-    if (!zi) k = f.eq ? (K)k_sdown<T, NS, false, true, true, SM, FW> : (K)k_sdown<T, NS, false, false, true, SM, FW>;
-    else k = f.eq ? (K)k_sdown<T, NS, true, true, true, SM, FW> : (K)k_sdown<T, NS, true, false, true, SM, FW>;
+    if (!zi)
+      k = f.eq ? (K)k_sdown<T, NS, false, true, true, SM, FW, PF> : (K)k_sdown<T, NS, false, false, true, SM, FW, PF>;
+    else k = f.eq ? (K)k_sdown<T, NS, true, true, true, SM, FW, PF> : (K)k_sdown<T, NS, true, false, true, SM, FW, PF>;
   } else {
     if (!zi && store)
-      k = f.eq ? (K)k_sdown<T, NS, false, true, true, SM, FW> : (K)k_sdown<T, NS, false, false, true, SM, FW>;
+      k = f.eq ? (K)k_sdown<T, NS, false, true, true, SM, FW, PF> : (K)k_sdown<T, NS, false, false, true, SM, FW, PF>;
     else if (!zi)
-      k = f.eq ? (K)k_sdown<T, NS, false, true, false, SM, FW> : (K)k_sdown<T, NS, false, false, false, SM, FW>;
+      k = f.eq ? (K)k_sdown<T, NS, false, true, false, SM, FW, PF>
+               : (K)k_sdown<T, NS, false, false, false, SM, FW, PF>;
     else if (store)
-      k = f.eq ? (K)k_sdown<T, NS, true, true, true, SM, FW> : (K)k_sdown<T, NS, true, false, true, SM, FW>;
-    else k = f.eq ? (K)k_sdown<T, NS, true, true, false, SM, FW> : (K)k_sdown<T, NS, true, false, false, SM, FW>;
+      k = f.eq ? (K)k_sdown<T, NS, true, true, true, SM, FW, PF> : (K)k_sdown<T, NS, true, false, true, SM, FW, PF>;
+    else
+      k = f.eq ? (K)k_sdown<T, NS, true, true, false, SM, FW, PF> : (K)k_sdown<T, NS, true, false, false, SM, FW, PF>;
   }
   const LevelView<T> a = sview(c, (const void *)k, f, f.u, f.u2, H, P);
   hipLaunchKernelGGL(k, dim3(8 * a.chunk), dim3(64 * kWaves), 0, c->st, a, g.rhs, g.nx, g.ny, g.pitch);
@@ -38,9 +48,19 @@ void launch_down_ns(Ctx *c, Lvl<T> &f, Lvl<T> &g, bool zi, bool store) {
   if (c->o.smoother == VLP2D_SMOOTH_WJACOBI) {
     if (fw) launch_down_k<T, NS, SM_JAC, true>(c, f, g, zi, store);
     else launch_down_k<T, NS, SM_JAC, false>(c, f, g, zi, store);
+  } else if (fw) {
+    launch_down_k<T, NS, SM_GS, true>(c, f, g, zi, store);
   } else {
-    if (fw) launch_down_k<T, NS, SM_GS, true>(c, f, g, zi, store);
-    else launch_down_k<T, NS, SM_GS, false>(c, f, g, zi, store);
+    if constexpr (sizeof(T) == 4) {
+      // FP32 Gauss-Seidel descents below 8193^2 nodes prefetch three rows
+      // ahead (measured on MI355X: 4097^2 25.4 -> 25.0 us, 2049^2 12.9 -> 11.9
+      // us per launch; the bandwidth-bound 8193^2 descent 81 -> 89 us keeps two)
+      if ((size_t)f.nx * (size_t)f.ny < kDeepPfEnd) {
+        launch_down_k<T, NS, SM_GS, false, 3>(c, f, g, zi, store);
+        return;
+      }
+    }
+    launch_down_k<T, NS, SM_GS, false>(c, f, g, zi, store);
   }
 }
 
@@ -65,11 +85,6 @@ int launch_sup(Ctx *c, Lvl<T> &f, const Lvl<T> &g, const MixView &mx) {
 // Jacobi-smoothed level is always stored, rz false).  nd = 1 (FP32 refinement
 // ascents that start from zero or stage the stored pre-smoothed iterate, one
 // pre-sweep): the next step's first descent is folded in.
-// The Gauss-Seidel FP32 ascents without an epilogue on levels of 4097^2 up to
-// (not including) 8193^2 nodes prefetch three rows ahead instead of two:
-// measured on MI355X, the 4097^2 level 49 -> 43.5 us per launch, while 2049^2
-// went 28 -> 29 us (larger levels are bandwidth-bound and keep two).
-constexpr size_t kDeepPfMin = (size_t)4097 * 4097, kDeepPfEnd = (size_t)8193 * 8193;
 template <typename T, int NS>
 int launch_up_ns(Ctx *c, Lvl<T> &f, const Lvl<T> &g, int epi, bool rz, bool zi, const MixView &mx, int nd) {
   if (c->o.smoother == VLP2D_SMOOTH_WJACOBI) {
