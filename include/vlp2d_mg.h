@@ -89,6 +89,10 @@ extern "C" {
                                      first descent in a launch of its own
                                      (default with one pre-sweep: folded into
                                      the previous step's refinement ascent) */
+#define VLP2D_FLAG_NO_LT_PAIR 1024u /* FP32 cycles: one launch per LDS-
+                                     tiled ascent (default: two such levels
+                                     per launch, the coarser recomputed per
+                                     tile)                                  */
 
 typedef struct vlp2d_mg_ctx *vlp2d_mg_handle;
 

@@ -76,6 +76,7 @@ KERNEL_PATHS = {
     "stream_small": {"flags": _lib.FLAG_STREAM_SMALL},  # streaming ascent on levels <= 1025^2 (not k_lt_up)
     "no_cycle_fusion": {"flags": _lib.FLAG_NO_CYCLE_FUSION},  # two-cycle steps: level 0 of each cycle apart
     "no_step_fold": {"flags": _lib.FLAG_NO_STEP_FOLD},  # each step's first descent in its own launch
+    "no_lt_pair": {"flags": _lib.FLAG_NO_LT_PAIR},  # FP32: one launch per LDS-tiled ascent level
     "one_op_per_launch": {"flags": _lib.FLAG_SIMPLE},
     "no_graph": {"flags": _lib.FLAG_NO_GRAPH},
     "graph_solve": {"flags": _lib.FLAG_GRAPH_SOLVE},  # the solve loop replays captured graphs
@@ -158,7 +159,7 @@ def test_recomputed_presmoothing_bitwise(seg_x, seg_y, hx, hy, laplace, mixed, n
         else:
             ur, kr, _ = plain_solve(seg_x, seg_y, hx, hy, z, g, 1, nu2, 3, 1e-10, 60)
         for flags in (0, _lib.FLAG_STORE_PRESMOOTH, _lib.FLAG_STREAM_SMALL, _lib.FLAG_NO_CYCLE_FUSION,
-                      _lib.FLAG_NO_STEP_FOLD):
+                      _lib.FLAG_NO_STEP_FOLD, _lib.FLAG_NO_LT_PAIR):
             s = MultiGridSchemeHIP(seg_x, seg_y, hx, hy, z,
                                    options=Options.performance(mixed, nu1=1, nu2=nu2, flags=flags, max_inner=inner))
             s.un0[:] = g

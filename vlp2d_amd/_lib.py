@@ -23,6 +23,7 @@ FLAG_STREAM_SMALL = 64
 FLAG_NO_CYCLE_FUSION = 128
 FLAG_GRAPH_SOLVE = 256
 FLAG_NO_STEP_FOLD = 512
+FLAG_NO_LT_PAIR = 1024
 SMOOTH_RBGS, SMOOTH_WJACOBI = 0, 1
 RESTRICT_HALF, RESTRICT_FULL = 0, 1
 CYCLE_V, CYCLE_FMG = 0, 1

@@ -368,6 +368,42 @@ template <typename T> void up_lds(Ctx *c, Lvl<T> &f, Lvl<T> &g, int lvl) {
   });
 }
 
+// Levels f (l) and g (l + 1), both LDS-tiled ascents, in one launch
+// (k_lt_up2): g's ascent recomputed per tile of f in LDS, h = level l + 2.
+template <typename T, int NS> void launch_lt_up2_ns(Ctx *c, Lvl<T> &f, const Lvl<T> &g, const Lvl<T> &h) {
+  constexpr int H = lt_halo(NS);
+  const LtGrid lg = lt_grid(f.nx, f.ny, H);
+  const LevelView<T> a = view(f, f.u2, f.u, H);
+  const LevelView<T> b = view(g, g.u2, g.u, H);
+  using K = void (*)(LevelView<T>, LevelView<T>, const T *, int, int, int, int);
+  const K k = f.eq ? (K)k_lt_up2<T, NS, true> : (K)k_lt_up2<T, NS, false>;
+  hipLaunchKernelGGL(k, dim3(lg.tiles), dim3(64 * kLtWaves), 0, c->st, a, b, h.u, h.nx, h.ny, h.pitch, lg.tiles_j);
+}
+
+// Whether the LDS-tiled ascents of levels l and l + 1 run as one launch: FP32
+// levels (the FP32 cycles of a mixed solve), both LDS-tiled, the same step
+// flags, and not VLP2D_FLAG_NO_LT_PAIR.  Measured on MI355X at 8193^2 (levels
+// 3-6 as two pairs): the mixed solve 3.652 -> 3.638 ms; in FP64 the pairs are
+// slower (the FP64 solve 6.216 -> 6.256 ms: the recomputed coarse tile and
+// 64 KB of LDS per workgroup cost more than the launch they save).
+template <typename T> bool lt_pair(const Ctx *c, const Hier<T> &H, int l) {
+  static_assert(sizeof(T) == 4, "FP32 levels only");
+  if (c->o.flags & VLP2D_FLAG_NO_LT_PAIR) return false;
+  if (!lds_ascent(c, H, l) || !lds_ascent(c, H, l + 1)) return false;
+  const Lvl<T> &f = H.lv[l], &g = H.lv[l + 1];
+  return f.eq == g.eq && f.pw2 == g.pw2;
+}
+
+template <typename T> void up_lds2(Ctx *c, Lvl<T> &f, Lvl<T> &g, const Lvl<T> &h, int lvl) {
+  launch(c, KC_UP_LDS, lvl, sizeof(T) * (3.0 * nodes(f) + 3.0 * nodes(g) + nodes(h)), [&] {
+    switch (c->o.nu2) {
+      case 1: launch_lt_up2_ns<T, 1>(c, f, g, h); break;
+      case 2: launch_lt_up2_ns<T, 2>(c, f, g, h); break;
+      default: launch_lt_up2_ns<T, 3>(c, f, g, h); break;
+    }
+  });
+}
+
 template <typename T> void coarse(Ctx *c, Hier<T> &H, bool zero_first, double *snap) {
   CoarseArgs<T> a;
   std::memset(&a, 0, sizeof(a));
@@ -447,6 +483,14 @@ int vcycle_fused(Ctx *c, Hier<T> &H, bool zero0, int epi, const MixView &mx, dou
   coarse(c, H, lc > 0 || zero0, lc == 0 ? snap : nullptr);
   int np = 0;
   for (int l = lc - 1; l >= to; --l) {
+    if constexpr (sizeof(T) == 4) {
+      if (l - 1 >= to && lt_pair(c, H, l - 1)) {
+        // levels l and l - 1 in one launch (level l's result stays in LDS)
+        up_lds2(c, H.lv[l - 1], H.lv[l], H.lv[l + 1], l - 1);
+        --l;
+        continue;
+      }
+    }
     if (lds_ascent(c, H, l)) {
       up_lds(c, H.lv[l], H.lv[l + 1], l);
       continue;

@@ -54,13 +54,13 @@ inline LtGrid lt_grid(int nx, int ny, int H) {
 // Every load and LDS read below is unconditional (clamped addresses, the
 // value selected afterwards), so a wavefront issues all of them back to back
 // and waits once; only stores are predicated.
-template <typename T, int NS, bool EQ, bool PW2>
-__device__ __forceinline__ void lt_up_body(const LevelView<T> &a, const T *__restrict__ uc, int ncx, int ncy, int pc,
-                                           int tiles_j, T *su) {
+// lt_pass: one pass over the staged tile with origin (gi0, cb) (both even) of
+// level a: stage a.uin plus the prolongation of the coarse correction
+// cval(I, J) (coarse grid ncx x ncy), NS sweeps in su; OUT: write the output
+// nodes to a.uout, else the result stays in su only.
+template <typename T, int NS, bool EQ, bool PW2, bool OUT, typename CV>
+__device__ __forceinline__ void lt_pass(const LevelView<T> &a, int gi0, int cb, int ncx, int ncy, CV &&cval, T *su) {
   constexpr int H = lt_halo(NS), TI = kLtRows - 2 * H, TJ = SCOLS - 2 * H;
-  const int t = blockIdx.x;
-  const int ti = t / tiles_j, tj = t - ti * tiles_j;
-  const int gi0 = ti * TI - H, cb = tj * TJ - H;
   const int w = (int)(threadIdx.x >> 6), m = (int)(threadIdx.x & 63);
   const int gj = cb + 2 * m;
   const bool ld = gj >= 0 && gj <= a.pitch - 2;
@@ -81,13 +81,13 @@ __device__ __forceinline__ void lt_up_body(const LevelView<T> &a, const T *__res
     Pair<T> u = ld_pair(row_of(a.uin, gic, a.pitch), boff);
     const Pair<T> f = ld_pair(row_of(a.f, gic, a.fpitch), boff);
     const int I0 = std::min(gic >> 1, ncx - 1), I1 = std::min(I0 + 1, ncx - 1);
-    const T c00 = uc[I0 * pc + J0], c01 = uc[I0 * pc + J1];
+    const T c00 = cval(I0, J0), c01 = cval(I0, J1);
     T q0, q1;
     if ((k & 1) == 0) {  // even row
       q0 = c00;
       q1 = (c00 + c01) * T(0.5);
     } else {
-      const T c10 = uc[I1 * pc + J0], c11 = uc[I1 * pc + J1];
+      const T c10 = cval(I1, J0), c11 = cval(I1, J1);
       q0 = (c00 + c10) * T(0.5);
       q1 = (c00 + c01 + c10 + c11) * T(0.25);
     }
@@ -137,6 +137,7 @@ __device__ __forceinline__ void lt_up_body(const LevelView<T> &a, const T *__res
     half(std::integral_constant<int, 1>{});
   }
 
+  if constexpr (!OUT) return;
   // output nodes: rows [H, H + TI), lanes [H/2, H/2 + TJ/2), interior nodes
   const bool oc = m >= H / 2 && m < H / 2 + TJ / 2;
 #pragma unroll
@@ -150,6 +151,16 @@ __device__ __forceinline__ void lt_up_body(const LevelView<T> &a, const T *__res
   }
 }
 
+template <typename T, int NS, bool EQ, bool PW2>
+__device__ __forceinline__ void lt_up_body(const LevelView<T> &a, const T *__restrict__ uc, int ncx, int ncy, int pc,
+                                           int tiles_j, T *su) {
+  constexpr int H = lt_halo(NS), TI = kLtRows - 2 * H, TJ = SCOLS - 2 * H;
+  const int t = blockIdx.x;
+  const int ti = t / tiles_j, tj = t - ti * tiles_j;
+  lt_pass<T, NS, EQ, PW2, true>(a, ti * TI - H, tj * TJ - H, ncx, ncy, [&](int I, int J) { return uc[I * pc + J]; },
+                                su);
+}
+
 // Ascent of a zero-initialised level whose pre-smoothed iterate the descent
 // stored (a.uin): prolongation + NS post-smoothing sweeps -> a.uout.
 template <typename T, int NS, bool EQ>
@@ -158,6 +169,39 @@ __global__ void __launch_bounds__(64 * kLtWaves) k_lt_up(LevelView<T> a, const T
   __shared__ __align__(16) T su[kLtRows * SCOLS];
   if (a.pw2) lt_up_body<T, NS, EQ, true>(a, uc, ncx, ncy, pc, tiles_j, su);
   else lt_up_body<T, NS, EQ, false>(a, uc, ncx, ncy, pc, tiles_j, su);
+}
+
+// Two levels in one launch: the ascent of level b (the coarser, l + 1; its
+// stored pre-smoothed iterate b.uin, right-hand side b.f, correction uc2 of
+// level l + 2) and then of level a (l), whose tile is the launch's.  Each
+// workgroup first runs b's pass over the coarse tile that holds every coarse
+// node its fine tile's prolongation reads, exact after the NS sweeps (origin
+// even, H coarse nodes further out on every side; neighbouring workgroups
+// compute overlapping coarse tiles, the same values), keeps it in LDS (sc),
+// and prolongates from there.  b's result is never written: the cycle reads
+// level l + 1 only through this prolongation.  Same operations per node as
+// k_lt_up on each level, so the same bits, with one launch instead of two.
+template <typename T, int NS, bool EQ>
+__global__ void __launch_bounds__(64 * kLtWaves) k_lt_up2(LevelView<T> a, LevelView<T> b, const T *__restrict__ uc2,
+                                                         int ncx2, int ncy2, int pc2, int tiles_j) {
+  __shared__ __align__(16) T su[kLtRows * SCOLS];
+  __shared__ __align__(16) T sc[kLtRows * SCOLS];
+  constexpr int H = lt_halo(NS), TI = kLtRows - 2 * H, TJ = SCOLS - 2 * H;
+  static_assert(2 * H < 15, "the coarse tile holds the fine tile's footprint");
+  const int t = blockIdx.x;
+  const int ti = t / tiles_j, tj = t - ti * tiles_j;
+  const int gi0 = ti * TI - H, cb = tj * TJ - H;
+  // coarse tile origin: even, at most H above / left of the first coarse
+  // node the fine tile reads (>> 1 rounds down for negative origins)
+  const int ci0 = ((gi0 >> 1) - H) & ~1, cj0 = ((cb >> 1) - H) & ~1;
+  auto pass = [&](auto pw2) {
+    constexpr bool PW2 = decltype(pw2)::value;
+    lt_pass<T, NS, EQ, PW2, false>(b, ci0, cj0, ncx2, ncy2, [&](int I, int J) { return uc2[I * pc2 + J]; }, sc);
+    lt_pass<T, NS, EQ, PW2, true>(a, gi0, cb, b.nx, b.ny,
+                                  [&](int I, int J) { return sc[(I - ci0) * SCOLS + (J - cj0)]; }, su);
+  };
+  if (a.pw2) pass(std::true_type{});
+  else pass(std::false_type{});
 }
 
 }  // namespace vlp2d
