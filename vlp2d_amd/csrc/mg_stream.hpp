@@ -766,7 +766,8 @@ __device__ __forceinline__ void sup_body(const LevelView<T> &a, const STile &tl,
   static_assert(ND == 0 || (EPI == 3 && SM == SM_GS), "the next step's descent follows the refinement epilogue");
   // residual radius around the output region the folded descent reads
   constexpr int RX = ND > 0 ? 2 * ND + 1 : 0, D = ST + 1, SD = 2 * ND;
-  // (it prefetches nothing: its ring only needs the rows in flight, 1 step)
+  // (its right-hand side is written one step before the sweeps read it, not
+  // prefetched: distance 1 sizes its ring)
   using St2 = SweepState<T, (ND > 0 ? ND : 1), 1>;
   static_assert(ND == 0 || P % St2::P == 0, "folded descent ring period");
   static_assert(St::S == SE, "sweep levels of the state");
